@@ -1,0 +1,337 @@
+// Element kernels: value-only stiffness assembly, local stiffness export,
+// Newton tangent, Lagrange vector.
+//
+// One thread per tetrahedron. Node coordinates are read from SoA arrays, the
+// basis gradients and volume come from one 3x3 determinant (closed form of
+// Femeko's abcd, src/FEM3D.jl:5-17), the 10 unique entries of
+// K = VE f (b b' + c c' + d d') (src/FEM3D.jl:59-76) are formed in registers
+// and added into the CSR values through the element's 16 precomputed slots.
+// Elements are stored colour by colour; a launch per colour makes every add
+// conflict free (no atomics, deterministic, and entry (i,j) and (j,i) receive
+// identical sums, so the matrix is exactly symmetric). The atomic variant
+// covers all elements in one launch with hardware FP64 atomics.
+#include "device.h"
+#include "internal.h"
+
+namespace fk {
+
+__global__ void k_fill_f64(double *__restrict__ a, int64_t n, double v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_f64_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = static_cast<float>(a[i]);
+}
+
+// coef[e] = f[perm[e]]; counts non-finite entries
+__global__ void k_gather_coef(const double *__restrict__ f, const int32_t *__restrict__ perm, int64_t nt,
+                              double *__restrict__ coef, int32_t *__restrict__ bad) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nt) return;
+    const double v = f[perm[e]];
+    coef[e] = v;
+    if (!isfinite(v)) atomicAdd(bad, 1);
+}
+
+// Local matrix of element e, 10 unique values in the order
+// (0,0) (0,1) (0,2) (0,3) (1,1) (1,2) (1,3) (2,2) (2,3) (3,3).
+__device__ __forceinline__ void local_values(const Grad &G, double w, double k[10]) {
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j) k[q++] = w * dot3(G.g[i], G.g[j]);
+}
+
+template <bool ATOMIC>
+__device__ __forceinline__ void scatter16(double *__restrict__ val, const int32_t *__restrict__ slot, int64_t e,
+                                          const double k[10]) {
+    const int4 *s4 = reinterpret_cast<const int4 *>(slot + 16 * e);
+    const int4 r0 = s4[0], r1 = s4[1], r2 = s4[2], r3 = s4[3];
+    const int s[16] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w,
+                       r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z, r3.w};
+    // upper-triangle index of (i,j)
+    constexpr int U[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double v = k[U[i][j]];
+            if (ATOMIC)
+                unsafeAtomicAdd(&val[s[4 * i + j]], v);
+            else
+                val[s[4 * i + j]] += v;
+        }
+}
+
+template <bool ATOMIC>
+__global__ __launch_bounds__(256) void k_assemble(const int4 *__restrict__ tet, const double *__restrict__ X,
+                                                  const double *__restrict__ Y, const double *__restrict__ Z,
+                                                  const double *__restrict__ coef, const int32_t *__restrict__ slot,
+                                                  double *__restrict__ val, int64_t e0, int64_t e1) {
+    const int64_t e = e0 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    const Grad G = element_gradients(X, Y, Z, tet[e]);
+    const double w = fabs(G.det) * (1.0 / 6.0) * coef[e];
+    double k[10];
+    local_values(G, w, k);
+    scatter16<ATOMIC>(val, slot, e, k);
+}
+
+// Newton tangent of src/FEM3D.jl:79-119:
+// At_ij = VE dmu/|H| (H . grad_i)(H . grad_j), H and dmu in caller order.
+__global__ __launch_bounds__(256) void k_tangent(const int4 *__restrict__ tet, const double *__restrict__ X,
+                                                 const double *__restrict__ Y, const double *__restrict__ Z,
+                                                 const int32_t *__restrict__ perm, const double *__restrict__ H,
+                                                 const double *__restrict__ dmu, const int32_t *__restrict__ slot,
+                                                 double *__restrict__ val, int64_t e0, int64_t e1) {
+    const int64_t e = e0 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    const Grad G = element_gradients(X, Y, Z, tet[e]);
+    const int64_t o = perm[e];
+    const double h[3] = {H[3 * o], H[3 * o + 1], H[3 * o + 2]};
+    const double hn = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    const double w = hn > 0.0 ? fabs(G.det) * (1.0 / 6.0) * dmu[o] / hn : 0.0;
+    double hg[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) hg[i] = dot3(h, G.g[i]);
+    double k[10];
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j) k[q++] = w * hg[i] * hg[j];
+    scatter16<false>(val, slot, e, k);
+}
+
+// Ak[:,k] = vec(VE f (b b' + c c' + d d')), 16 x nt column-major, caller order
+__global__ __launch_bounds__(256) void k_local_stiffness(const int4 *__restrict__ tet, const double *__restrict__ X,
+                                                         const double *__restrict__ Y, const double *__restrict__ Z,
+                                                         const int32_t *__restrict__ perm,
+                                                         const double *__restrict__ coef, int64_t nt,
+                                                         double *__restrict__ Ak) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nt) return;
+    const Grad G = element_gradients(X, Y, Z, tet[e]);
+    const double w = fabs(G.det) * (1.0 / 6.0) * (coef ? coef[e] : 1.0);
+    double k[10];
+    local_values(G, w, k);
+    constexpr int U[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+    double *out = Ak + 16 * static_cast<int64_t>(perm[e]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[i + 4 * j] = k[U[i][j]];
+}
+
+// C[nds] += VE/4 (src/FEM3D.jl:274-281), one colour per launch
+__global__ __launch_bounds__(256) void k_lagrange(const int4 *__restrict__ tet, const double *__restrict__ X,
+                                                  const double *__restrict__ Y, const double *__restrict__ Z,
+                                                  double *__restrict__ C, int64_t e0, int64_t e1) {
+    const int64_t e = e0 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    const int4 nd = tet[e];
+    const Grad G = element_gradients(X, Y, Z, nd);
+    const double v = fabs(G.det) * (1.0 / 6.0) / 4;
+    C[nd.x] += v;
+    C[nd.y] += v;
+    C[nd.z] += v;
+    C[nd.w] += v;
+}
+
+// ---------------------------------------------------------------------------
+void upload_coefficient(fk_mesh *m, const double *f) {
+    hipStream_t st = m->ctx->stream;
+    DevBuf<double> df(m->nt);
+    DevBuf<int32_t> bad(1);
+    FK_HIP(hipMemcpyAsync(df.get(), f, m->nt * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), st));
+    m->coef.ensure(m->nt);
+    k_gather_coef<<<grid_for(m->nt, 256), 256, 0, st>>>(df.get(), m->perm.get(), m->nt, m->coef.get(), bad.get());
+    FK_HIP(hipGetLastError());
+    int32_t h = 0;
+    FK_HIP(hipMemcpyAsync(&h, bad.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    if (h) {
+        m->have_coef = false;
+        fail(FK_ERR_NONFINITE, std::to_string(h) + " non-finite coefficient value(s)");
+    }
+    m->have_coef = true;
+}
+
+// Enqueue the value assembly on the context stream (no host sync).
+void assemble(fk_mesh *m, int method) {
+    if (!m->have_coef) fail(FK_ERR_STATE, "no coefficient set");
+    hipStream_t st = m->ctx->stream;
+    m->val64.ensure(m->nnz);
+    k_fill_f64<<<grid_for(m->nnz, 256), 256, 0, st>>>(m->val64.get(), m->nnz, 0.0);
+    if (method == FK_ASM_ATOMIC) {
+        k_assemble<true><<<grid_for(m->nt, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
+                                                              m->coef.get(), m->slot.get(), m->val64.get(), 0,
+                                                              m->nt);
+    } else {
+        for (int c = 0; c < m->ncolours; ++c) {
+            const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
+            if (e1 <= e0) continue;
+            k_assemble<false><<<grid_for(e1 - e0, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(),
+                                                                      m->Z.get(), m->coef.get(), m->slot.get(),
+                                                                      m->val64.get(), e0, e1);
+        }
+    }
+    FK_HIP(hipGetLastError());
+    m->have_values = true;
+    m->have_val32 = false;
+    m->dinv64_valid = false;
+    m->dinv32_valid = false;
+}
+
+void make_val32(fk_mesh *m) {
+    if (m->have_val32) return;
+    hipStream_t st = m->ctx->stream;
+    m->val32.ensure(m->nnz);
+    k_f64_to_f32<<<grid_for(m->nnz, 256), 256, 0, st>>>(m->val64.get(), m->val32.get(), m->nnz);
+    FK_HIP(hipGetLastError());
+    m->have_val32 = true;
+}
+
+void local_stiffness(fk_mesh *m, const double *f, double *Ak_host) {
+    hipStream_t st = m->ctx->stream;
+    DevBuf<double> coef;
+    if (f) {
+        DevBuf<double> df(m->nt);
+        DevBuf<int32_t> bad(1);
+        coef.alloc(m->nt);
+        FK_HIP(hipMemcpyAsync(df.get(), f, m->nt * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), st));
+        k_gather_coef<<<grid_for(m->nt, 256), 256, 0, st>>>(df.get(), m->perm.get(), m->nt, coef.get(), bad.get());
+        FK_HIP(hipGetLastError());
+        FK_HIP(hipStreamSynchronize(st));
+    }
+    DevBuf<double> Ak(16 * m->nt);
+    k_local_stiffness<<<grid_for(m->nt, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
+                                                            m->perm.get(), f ? coef.get() : nullptr, m->nt,
+                                                            Ak.get());
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipMemcpyAsync(Ak_host, Ak.get(), 16 * m->nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+}
+
+void tangential(fk_mesh *m, const double *H, const double *dmu, double *nzval_host) {
+    hipStream_t st = m->ctx->stream;
+    DevBuf<double> dH(3 * m->nt), ddmu(m->nt), val(m->nnz);
+    FK_HIP(hipMemcpyAsync(dH.get(), H, 3 * m->nt * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(ddmu.get(), dmu, m->nt * sizeof(double), hipMemcpyHostToDevice, st));
+    k_fill_f64<<<grid_for(m->nnz, 256), 256, 0, st>>>(val.get(), m->nnz, 0.0);
+    for (int c = 0; c < m->ncolours; ++c) {
+        const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
+        if (e1 <= e0) continue;
+        k_tangent<<<grid_for(e1 - e0, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
+                                                          m->perm.get(), dH.get(), ddmu.get(), m->slot.get(),
+                                                          val.get(), e0, e1);
+    }
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipMemcpyAsync(nzval_host, val.get(), m->nnz * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+}
+
+void lagrange(fk_mesh *m, double *C_host) {
+    hipStream_t st = m->ctx->stream;
+    DevBuf<double> C(m->nv);
+    k_fill_f64<<<grid_for(m->nv, 256), 256, 0, st>>>(C.get(), m->nv, 0.0);
+    for (int c = 0; c < m->ncolours; ++c) {
+        const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
+        if (e1 <= e0) continue;
+        k_lagrange<<<grid_for(e1 - e0, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
+                                                           C.get(), e0, e1);
+    }
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipMemcpyAsync(C_host, C.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace fk
+
+// ---------------------------------------------------------------------------
+// C ABI: coefficients and assembly
+using fk::guard;
+
+extern "C" {
+
+int fk_set_coefficient(fk_mesh *m, const double *f) {
+    if (!m || !f) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_set_coefficient", [&] {
+        fk::set_device(m->ctx);
+        fk::upload_coefficient(m, f);
+    });
+}
+
+int fk_assemble(fk_mesh *m, int32_t method) {
+    if (!m || (method != FK_ASM_COLOURED && method != FK_ASM_ATOMIC)) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_assemble", [&] {
+        fk::set_device(m->ctx);
+        fk::assemble(m, method);
+    });
+}
+
+int fk_values(fk_mesh *m, double *values) {
+    if (!m || !values) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_values", [&] {
+        if (!m->have_values) fk::fail(FK_ERR_STATE, "matrix not assembled");
+        fk::set_device(m->ctx);
+        FK_HIP(hipMemcpyAsync(values, m->val64.get(), m->nnz * sizeof(double), hipMemcpyDeviceToHost,
+                              m->ctx->stream));
+        FK_HIP(hipStreamSynchronize(m->ctx->stream));
+    });
+}
+
+int fk_values_f32(fk_mesh *m, float *values) {
+    if (!m || !values) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_values_f32", [&] {
+        if (!m->have_values) fk::fail(FK_ERR_STATE, "matrix not assembled");
+        fk::set_device(m->ctx);
+        fk::make_val32(m);
+        FK_HIP(hipMemcpyAsync(values, m->val32.get(), m->nnz * sizeof(float), hipMemcpyDeviceToHost,
+                              m->ctx->stream));
+        FK_HIP(hipStreamSynchronize(m->ctx->stream));
+    });
+}
+
+int fk_stiffness_matrix(fk_mesh *m, const double *f, int32_t base, int32_t *colptr, int32_t *rowval,
+                        double *nzval) {
+    if (!m || !f || !colptr || !rowval || !nzval || (base != 0 && base != 1)) return FK_ERR_INVALID_ARG;
+    int s = fk_set_coefficient(m, f);
+    if (s) return s;
+    s = fk_assemble(m, FK_ASM_COLOURED);
+    if (s) return s;
+    s = fk_pattern(m, base, colptr, rowval);
+    if (s) return s;
+    return fk_values(m, nzval);
+}
+
+int fk_local_stiffness(fk_mesh *m, const double *f, double *Ak) {
+    if (!m || !Ak) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_local_stiffness", [&] {
+        fk::set_device(m->ctx);
+        fk::local_stiffness(m, f, Ak);
+    });
+}
+
+int fk_tangential_stiffness(fk_mesh *m, const double *H, const double *dmu, double *nzval) {
+    if (!m || !H || !dmu || !nzval) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_tangential_stiffness", [&] {
+        fk::set_device(m->ctx);
+        fk::tangential(m, H, dmu, nzval);
+    });
+}
+
+int fk_lagrange(fk_mesh *m, double *C) {
+    if (!m || !C) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_lagrange", [&] {
+        fk::set_device(m->ctx);
+        fk::lagrange(m, C);
+    });
+}
+
+}  // extern "C"

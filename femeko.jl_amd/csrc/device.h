@@ -1,0 +1,114 @@
+// Device helpers shared by the femeko_hip kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fk {
+
+// Basis gradients of a P1 tetrahedron in closed form. With edges
+// e_k = P_k - P_0, det = e1 . (e2 x e3); grad(lambda_1) = (e2 x e3)/det,
+// grad(lambda_2) = (e3 x e1)/det, grad(lambda_3) = (e1 x e2)/det and
+// grad(lambda_0) = -(sum of the others). These are the b, c, d that Femeko's
+// abcd obtains from a 4x4 solve per node (src/FEM3D.jl:5-17); |det|/6 is
+// elementVolume (src/mesh.jl:484-505).
+struct Grad {
+    double g[4][3];
+    double det;
+};
+
+__device__ __forceinline__ Grad element_gradients(const double *__restrict__ X, const double *__restrict__ Y,
+                                                  const double *__restrict__ Z, int4 nd) {
+    const double x0 = X[nd.x], y0 = Y[nd.x], z0 = Z[nd.x];
+    const double ax = X[nd.y] - x0, ay = Y[nd.y] - y0, az = Z[nd.y] - z0;
+    const double bx = X[nd.z] - x0, by = Y[nd.z] - y0, bz = Z[nd.z] - z0;
+    const double cx = X[nd.w] - x0, cy = Y[nd.w] - y0, cz = Z[nd.w] - z0;
+    // cofactor vectors
+    const double c1x = by * cz - bz * cy, c1y = bz * cx - bx * cz, c1z = bx * cy - by * cx;  // b x c
+    const double c2x = cy * az - cz * ay, c2y = cz * ax - cx * az, c2z = cx * ay - cy * ax;  // c x a
+    const double c3x = ay * bz - az * by, c3y = az * bx - ax * bz, c3z = ax * by - ay * bx;  // a x b
+    Grad G;
+    G.det = ax * c1x + ay * c1y + az * c1z;
+    const double inv = 1.0 / G.det;
+    G.g[1][0] = c1x * inv; G.g[1][1] = c1y * inv; G.g[1][2] = c1z * inv;
+    G.g[2][0] = c2x * inv; G.g[2][1] = c2y * inv; G.g[2][2] = c2z * inv;
+    G.g[3][0] = c3x * inv; G.g[3][1] = c3y * inv; G.g[3][2] = c3z * inv;
+    G.g[0][0] = -(G.g[1][0] + G.g[2][0] + G.g[3][0]);
+    G.g[0][1] = -(G.g[1][1] + G.g[2][1] + G.g[3][1]);
+    G.g[0][2] = -(G.g[1][2] + G.g[2][2] + G.g[3][2]);
+    return G;
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// ---- deterministic reductions (fixed operation order) ---------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sum over the block; every thread gets the result. sh: BLOCK/64 + 1 doubles.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < BLOCK / 64; ++i) s += sh[i];
+        sh[BLOCK / 64] = s;
+    }
+    __syncthreads();
+    const double s = sh[BLOCK / 64];
+    __syncthreads();
+    return s;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_max(double v, double *sh) {
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = sh[0];
+#pragma unroll
+        for (int i = 1; i < BLOCK / 64; ++i) s = fmax(s, sh[i]);
+        sh[BLOCK / 64] = s;
+    }
+    __syncthreads();
+    const double s = sh[BLOCK / 64];
+    __syncthreads();
+    return s;
+}
+
+// Sum of n block partials, same order in every block.
+template <int BLOCK>
+__device__ __forceinline__ double sum_partials(const double *__restrict__ part, int n, double *sh) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += BLOCK) s += part[i];
+    return block_sum<BLOCK>(s, sh);
+}
+
+// Blocks are handed to the 8 XCDs round-robin; give each XCD a contiguous
+// 1/8 of the logical chunks so its L2 holds one band of rows (and the x
+// entries those rows gather).
+__device__ __forceinline__ int logical_block(int nblocks) {
+    const int b = blockIdx.x;
+    if ((nblocks & 7) != 0) return b;
+    return (b & 7) * (nblocks >> 3) + (b >> 3);
+}
+
+}  // namespace fk

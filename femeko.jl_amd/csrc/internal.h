@@ -1,0 +1,221 @@
+// Internal declarations of the femeko_hip library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "femeko_hip.h"
+
+namespace fk {
+
+// ---------------------------------------------------------------------------
+// errors: thrown inside the library, turned into status codes at the C ABI
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+#define FK_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            ::fk::fail(e_ == hipErrorOutOfMemory ? FK_ERR_OOM : FK_ERR_HIP,                   \
+                       std::string(#call) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffer with ownership
+template <typename T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), n(o.n) { o.ptr = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; n = o.n; o.ptr = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        FK_HIP(hipMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count) { if (n < count) alloc(count); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    T *get() const { return ptr; }
+    explicit operator bool() const { return ptr != nullptr; }
+};
+
+// pinned host buffer
+template <typename T>
+struct HostBuf {
+    T *ptr = nullptr;
+    size_t n = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf() { if (ptr) (void)hipHostFree(ptr); }
+    void alloc(size_t count) {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        FK_HIP(hipHostMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// state of one PCG run, read by every kernel of the iteration
+struct CgState {
+    int32_t done;       // 1 once converged / stopped: later kernels return at once
+    int32_t status;     // FK_OK, FK_ERR_NOT_CONVERGED, FK_ERR_SINGULAR
+    int64_t iters;      // completed iterations
+    double thresh;      // rtol^2 * ||b||^2
+    double bb;          // ||b||^2
+    double rr;          // ||r||^2 at stop
+    int64_t max_iter;   // iteration limit of this run
+    double pad[2];
+};
+
+constexpr int kBlock = 256;      // threads per block of the solver kernels
+constexpr int kNumXcd = 8;       // MI355X accelerator complex dies
+
+// Solver workspace of one precision.
+template <typename T>
+struct PcgWork {
+    DevBuf<T> x, r, p, q, dinv;
+    DevBuf<double> part_pq, part_rz, part_rr, part_bb;  // part_rz holds 2 x nblocks
+    DevBuf<CgState> state;
+    hipGraphExec_t graph = nullptr;  // captured chunk of iterations
+    int graph_iters = 0;
+    int graph_lanes = 0;
+    void reset_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_iters = 0;
+    }
+    ~PcgWork() { reset_graph(); }
+};
+
+struct TimingEvents {
+    bool enabled = false;
+    std::vector<hipEvent_t> ev;  // pairs
+    size_t used = 0;
+    int64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+}  // namespace fk
+
+struct fk_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    std::string err;
+    fk::TimingEvents timing;
+};
+
+struct fk_mesh {
+    fk_context *ctx = nullptr;
+    int64_t nv = 0, nt = 0, nnz = 0;
+    int64_t unused_nodes = 0;
+    int32_t ncolours = 0;
+    std::vector<int64_t> colour_off;  // ncolours + 1, offsets in colour order
+
+    // geometry and connectivity (elements in colour order)
+    fk::DevBuf<double> X, Y, Z;       // node coordinates, SoA
+    fk::DevBuf<int4> tet;             // 0-based nodes per element
+    fk::DevBuf<int32_t> perm;         // colour position -> caller element index
+    // CSR pattern
+    fk::DevBuf<int32_t> rowptr, colidx, diag, slot;  // slot: 16 per element (colour order)
+    // coefficients and values
+    fk::DevBuf<double> coef;          // per element (colour order)
+    bool have_coef = false;
+    fk::DevBuf<double> val64;
+    fk::DevBuf<float> val32;
+    bool have_values = false, have_val32 = false;
+    // Dirichlet data
+    fk::DevBuf<uint8_t> mask;
+    fk::DevBuf<double> g;
+    bool have_dirichlet = false;
+    int64_t nfixed = 0;
+    bool dinv64_valid = false;        // dinv buffers match current values + mask
+    bool dinv32_valid = false;
+    // potential and residual
+    fk::DevBuf<double> phi, r64;
+    bool have_phi = false;
+    fk::DevBuf<double> scratch_nv;    // nv doubles
+    fk::DevBuf<double> scratch_nt;    // per element scratch
+    fk::DevBuf<double> part_misc;     // block partials of field/energy/residual kernels
+    fk::HostBuf<fk::CgState> hstate;  // pinned copies of the CG state
+    fk::HostBuf<double> hpart;        // pinned copies of partials
+
+    int lanes = 16;                   // lanes per CSR row in SpMV
+    int nblocks = 0;                  // persistent grid of the solver kernels
+    int64_t chunk = 0;                // rows per logical block
+
+    fk::PcgWork<double> w64;
+    fk::PcgWork<float> w32;
+};
+
+namespace fk {
+
+// ---- helpers implemented in the .hip files --------------------------------
+void set_device(fk_context *ctx);
+void build_mesh(fk_mesh *m, const double *p, const int32_t *t, int32_t base, const double *VE);
+void element_volumes(fk_mesh *m, double *VE_host);
+void assemble(fk_mesh *m, int method);
+void make_val32(fk_mesh *m);
+void local_stiffness(fk_mesh *m, const double *f, double *Ak_host);
+void tangential(fk_mesh *m, const double *H, const double *dmu, double *nzval_host);
+void lagrange(fk_mesh *m, double *C_host);
+void upload_coefficient(fk_mesh *m, const double *f);
+void set_dirichlet(fk_mesh *m, const uint8_t *mask, const double *g);
+void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inner_rtol, int max_outer,
+           int warm, int64_t *iters, int32_t *outer, double *relres);
+void solve_rhs(fk_mesh *m, const double *b, double rtol, int64_t max_iter, double *x, int64_t *iters,
+               double *relres);
+void field(fk_mesh *m, double *H, double *Hn, double *energy);
+void setup_partition(fk_mesh *m);
+
+// Run body; map library errors to a status and remember the message.
+template <typename F>
+int guard(fk_context *ctx, const char *where, F &&body) {
+    try {
+        body();
+        if (ctx) ctx->err.clear();
+        return FK_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->err = std::string(where) + ": " + e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->err = std::string(where) + ": host allocation failed";
+        return FK_ERR_OOM;
+    } catch (...) {
+        if (ctx) ctx->err = std::string(where) + ": unexpected error";
+        return FK_ERR_HIP;
+    }
+}
+
+// launch helpers
+inline unsigned grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
+}  // namespace fk

@@ -1,0 +1,232 @@
+"""Handles over the C ABI: a GPU context and a device-resident mesh."""
+from __future__ import annotations
+
+import atexit
+import ctypes
+import weakref
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import FemekoHIPError, dptr, fptr, iptr, uptr
+
+PRECISIONS = {"f64": _lib.FK_PREC_F64, "f32": _lib.FK_PREC_F32_REFINED,
+              "fp64": _lib.FK_PREC_F64, "f32-refined": _lib.FK_PREC_F32_REFINED}
+METHODS = {"coloured": _lib.FK_ASM_COLOURED, "colored": _lib.FK_ASM_COLOURED, "atomic": _lib.FK_ASM_ATOMIC}
+
+
+class Context:
+    """A GPU and the stream every call on it runs on (fk_context)."""
+
+    def __init__(self, device: int = 0):
+        self._L = _lib.load()
+        h = ctypes.c_void_p()
+        s = self._L.fk_context_create(int(device), ctypes.byref(h))
+        if s:
+            raise FemekoHIPError(s, f"fk_context_create(device={device}): "
+                                    f"{self._L.fk_status_string(s).decode()}")
+        self.handle = h
+        self.device = device
+
+    def check(self, status: int, where: str) -> None:
+        if status:
+            msg = self._L.fk_context_last_error(self.handle).decode() or self._L.fk_status_string(status).decode()
+            raise FemekoHIPError(status, f"{where}: {msg}")
+
+    def synchronize(self) -> None:
+        self.check(self._L.fk_context_synchronize(self.handle), "fk_context_synchronize")
+
+    def set_timing(self, enable: bool) -> None:
+        self.check(self._L.fk_set_timing(self.handle, int(bool(enable))), "fk_set_timing")
+
+    def spmv_timing(self):
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0.0)
+        self.check(self._L.fk_spmv_timing(self.handle, ctypes.byref(n), ctypes.byref(ms)), "fk_spmv_timing")
+        return int(n.value), float(ms.value)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self._L.fk_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = {}
+_live_meshes = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    # free device memory while the HIP runtime is still up: meshes, then contexts
+    for dm in list(_live_meshes):
+        dm.close()
+    for ctx in list(_default.values()):
+        ctx.close()
+
+
+def context(device: int = 0) -> Context:
+    """Process-wide context of a device (created on first use)."""
+    if device not in _default:
+        _default[device] = Context(device)
+    return _default[device]
+
+
+@dataclass
+class SolveReport:
+    iterations: int
+    outer: int
+    relres: float
+
+
+def _f64(a, n=None, name="array"):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, f"{name} has {a.size} entries, expected {n}")
+    return a
+
+
+class DeviceMesh:
+    """A MESH resident on the GPU: coordinates, connectivity in colour order,
+    CSR pattern and element -> slot map, coefficients, values, Dirichlet data,
+    potential (fk_mesh)."""
+
+    def __init__(self, mesh, ctx: Context | None = None, check_volumes: bool = True, index_base: int = 1):
+        self.ctx = ctx or context(0)
+        self._L = self.ctx._L
+        self.nv = int(mesh.p.shape[1])
+        self.nt = int(mesh.t.shape[1])
+        p = np.ascontiguousarray(np.asarray(mesh.p, dtype=np.float64).T)   # (nv,3) == 3 x nv column-major
+        t = np.ascontiguousarray(np.asarray(mesh.t, dtype=np.int32).T)     # (nt,4) == 4 x nt column-major
+        VE = _f64(mesh.VE, self.nt, "VE") if (check_volumes and getattr(mesh, "VE", None) is not None) else None
+        h = ctypes.c_void_p()
+        s = self._L.fk_mesh_create(self.ctx.handle, dptr(p), self.nv, iptr(t), self.nt, int(index_base),
+                                   dptr(VE), ctypes.byref(h))
+        self.ctx.check(s, "fk_mesh_create")
+        self.handle = h
+        _live_meshes.add(self)
+        nnz = ctypes.c_int64(0)
+        nc = ctypes.c_int32(0)
+        unused = ctypes.c_int64(0)
+        self.ctx.check(self._L.fk_mesh_info(h, ctypes.byref(nnz), ctypes.byref(nc), ctypes.byref(unused)),
+                       "fk_mesh_info")
+        self.nnz = int(nnz.value)
+        self.ncolours = int(nc.value)
+        self.unused_nodes = int(unused.value)
+
+    # -- pattern / assembly --------------------------------------------------
+    def pattern(self, base: int = 1):
+        rowptr = np.empty(self.nv + 1, np.int32)
+        colidx = np.empty(self.nnz, np.int32)
+        self.ctx.check(self._L.fk_pattern(self.handle, int(base), iptr(rowptr), iptr(colidx)), "fk_pattern")
+        return rowptr, colidx
+
+    def set_coefficient(self, f) -> None:
+        f = _f64(f, self.nt, "f")
+        self.ctx.check(self._L.fk_set_coefficient(self.handle, dptr(f)), "fk_set_coefficient")
+
+    def assemble(self, method: str = "coloured") -> None:
+        self.ctx.check(self._L.fk_assemble(self.handle, METHODS[method]), "fk_assemble")
+
+    def values(self) -> np.ndarray:
+        v = np.empty(self.nnz)
+        self.ctx.check(self._L.fk_values(self.handle, dptr(v)), "fk_values")
+        return v
+
+    def values_f32(self) -> np.ndarray:
+        v = np.empty(self.nnz, np.float32)
+        self.ctx.check(self._L.fk_values_f32(self.handle, fptr(v)), "fk_values_f32")
+        return v
+
+    def local_stiffness(self, f=None) -> np.ndarray:
+        Ak = np.empty((self.nt, 16))
+        fa = None if f is None else _f64(f, self.nt, "f")
+        self.ctx.check(self._L.fk_local_stiffness(self.handle, dptr(fa), dptr(Ak)), "fk_local_stiffness")
+        return Ak.T
+
+    def tangential_values(self, H, dmu) -> np.ndarray:
+        Hc = _f64(np.asarray(H, dtype=np.float64).T, 3 * self.nt, "H")
+        d = _f64(dmu, self.nt, "dmu")
+        v = np.empty(self.nnz)
+        self.ctx.check(self._L.fk_tangential_stiffness(self.handle, dptr(Hc), dptr(d), dptr(v)),
+                       "fk_tangential_stiffness")
+        return v
+
+    def lagrange(self) -> np.ndarray:
+        C = np.empty(self.nv)
+        self.ctx.check(self._L.fk_lagrange(self.handle, dptr(C)), "fk_lagrange")
+        return C
+
+    def volumes(self) -> np.ndarray:
+        v = np.empty(self.nt)
+        self.ctx.check(self._L.fk_mesh_volumes(self.handle, dptr(v)), "fk_mesh_volumes")
+        return v
+
+    # -- Dirichlet data and solve ----------------------------------------------
+    def set_dirichlet(self, mask, g) -> None:
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.size != self.nv:
+            raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "mask size")
+        g = _f64(g, self.nv, "g")
+        self.ctx.check(self._L.fk_set_dirichlet(self.handle, uptr(m), dptr(g)), "fk_set_dirichlet")
+
+    def solve(self, precision: str = "f64", rtol: float = 1e-10, max_iter: int = 1_000_000,
+              inner_rtol: float = 1e-5, max_outer: int = 50, warm_start: bool = False,
+              raise_on_error: bool = True) -> SolveReport:
+        it = ctypes.c_int64(0)
+        outer = ctypes.c_int32(0)
+        rel = ctypes.c_double(0.0)
+        s = self._L.fk_solve(self.handle, PRECISIONS[precision], float(rtol), int(max_iter), float(inner_rtol),
+                             int(max_outer), int(bool(warm_start)), ctypes.byref(it), ctypes.byref(outer),
+                             ctypes.byref(rel))
+        rep = SolveReport(int(it.value), int(outer.value), float(rel.value))
+        if raise_on_error:
+            self.ctx.check(s, "fk_solve")
+        rep.status = s
+        return rep
+
+    def solve_rhs(self, b, rtol: float = 1e-10, max_iter: int = 1_000_000):
+        b = _f64(b, self.nv, "b")
+        x = np.empty(self.nv)
+        it = ctypes.c_int64(0)
+        rel = ctypes.c_double(0.0)
+        self.ctx.check(self._L.fk_solve_rhs(self.handle, dptr(b), float(rtol), int(max_iter), dptr(x),
+                                            ctypes.byref(it), ctypes.byref(rel)), "fk_solve_rhs")
+        return x, int(it.value), float(rel.value)
+
+    def potential(self) -> np.ndarray:
+        u = np.empty(self.nv)
+        self.ctx.check(self._L.fk_potential(self.handle, dptr(u)), "fk_potential")
+        return u
+
+    def set_potential(self, u) -> None:
+        u = _f64(u, self.nv, "u")
+        self.ctx.check(self._L.fk_set_potential(self.handle, dptr(u)), "fk_set_potential")
+
+    def field(self, want_H: bool = True, want_Hn: bool = True, want_energy: bool = False):
+        H = np.empty((self.nt, 3)) if want_H else None
+        Hn = np.empty(self.nt) if want_Hn else None
+        E = ctypes.c_double(0.0)
+        self.ctx.check(self._L.fk_field(self.handle, dptr(H), dptr(Hn),
+                                        ctypes.byref(E) if want_energy else None), "fk_field")
+        return (H.T if H is not None else None), Hn, (float(E.value) if want_energy else None)
+
+    def set_lanes(self, lanes: int) -> None:
+        self.ctx.check(self._L.fk_set_lanes_per_row(self.handle, int(lanes)), "fk_set_lanes_per_row")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self._L.fk_mesh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,165 @@
+/*
+ * femeko_hip.h -- C ABI of the MI355X (gfx950) P1 magnetostatics library.
+ *
+ * Drop-in native boundary for Femeko.jl's 3D P1 path: Julia (or any C FFI)
+ * keeps its MESH and drivers and calls these functions where the drivers call
+ * stiffnessMatrix / localStiffnessMatrix / tangentialStiffnessMatrix, the
+ * sparse solve and the H-field loop. The model is the existing reference
+ * bridge `scalarPotential` (Femeko_jl_C++/Magnetostatics/magnetostatics.cpp:
+ * 49-60, bound by @ccall in Femeko_jl_C++/Magnetostatics/run.jl:66-79):
+ * plain pointers and sizes, caller-owned host buffers in and out.
+ *
+ * Conventions
+ *   - Every function returns an int status (FK_OK == 0). Nothing throws
+ *     across this ABI (the reference throws C++ exceptions through extern "C",
+ *     magnetostatics.cpp:29-30,36-37). fk_context_last_error() explains the
+ *     last failure on a context.
+ *   - Host arrays use Femeko's layouts: p is 3 x nv column-major Float64,
+ *     t is 4 x nt column-major Int32 (src/mesh.jl:9-59). Node numbers may be
+ *     1-based (Julia, index_base = 1) or 0-based (index_base = 0); the
+ *     library never needs a shifted copy (the reference makes one,
+ *     run.jl:50-52).
+ *   - Per-element arrays (f, mu, H, |H|, gradients) are in the caller's
+ *     element order. Per-node arrays are in the caller's node order.
+ *   - Device memory is owned by the handles; the mesh, coefficients, matrix
+ *     values, Dirichlet data and potential stay resident on the GPU between
+ *     calls.
+ *   - A context is not re-entrant: call it from one thread at a time.
+ *     Different contexts may be used concurrently.
+ */
+#ifndef FEMEKO_HIP_H
+#define FEMEKO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FK_OK 0
+#define FK_ERR_INVALID_ARG 1   /* null pointer, bad size, bad enum value */
+#define FK_ERR_HIP 2           /* a HIP runtime call failed */
+#define FK_ERR_DEGENERATE 3    /* zero-volume / non-finite element, repeated node */
+#define FK_ERR_INDEX 4         /* node number out of range */
+#define FK_ERR_NONFINITE 5     /* NaN/Inf coefficient or field (NewtonRaphson.jl:210-212) */
+#define FK_ERR_NOT_CONVERGED 6 /* iteration limit reached before the tolerance */
+#define FK_ERR_SINGULAR 7      /* free node without stiffness, CG breakdown, no Dirichlet node */
+#define FK_ERR_OOM 8           /* device or host allocation failed */
+#define FK_ERR_STATE 9         /* call order: e.g. solve before assemble */
+#define FK_ERR_UNSUPPORTED 10  /* mesh beyond the library's limits */
+
+/* precision of the linear solve */
+#define FK_PREC_F64 1          /* FP64 matrix, FP64 Jacobi-PCG */
+#define FK_PREC_F32_REFINED 2  /* FP32 matrix + inner PCG, FP64 residual refinement */
+
+/* scatter method of the element assembly */
+#define FK_ASM_COLOURED 0      /* mesh-coloured conflict-free adds (deterministic) */
+#define FK_ASM_ATOMIC 1        /* hardware FP64 atomics (measured alternative) */
+
+typedef struct fk_context fk_context;
+typedef struct fk_mesh fk_mesh;
+
+/* ---- context ------------------------------------------------------------ */
+int fk_version(void);
+const char *fk_status_string(int status);
+/* Select GPU `device`, create the stream all work of this context runs on. */
+int fk_context_create(int device, fk_context **out);
+int fk_context_destroy(fk_context *ctx);
+const char *fk_context_last_error(const fk_context *ctx);
+int fk_context_synchronize(fk_context *ctx);
+
+/* ---- mesh ----------------------------------------------------------------
+ * Upload p (3 x nv) and t (4 x nt), colour the elements, build the CSR
+ * pattern and the element -> CSR-slot map on the GPU. Replaces the mesh data
+ * every FEM3D.jl P1 function reads (src/mesh.jl:9-59) and the fixed-sparsity
+ * COO index build of Magnetostatics/NewtonRaphson.jl:115-127.
+ * VE may be NULL; if given it is checked against the recomputed volumes
+ * (elementVolume, src/mesh.jl:484-505) and a mismatch above 1e-10 relative
+ * returns FK_ERR_INVALID_ARG. */
+int fk_mesh_create(fk_context *ctx, const double *p, int64_t nv, const int32_t *t, int64_t nt,
+                   int32_t index_base, const double *VE, fk_mesh **out);
+int fk_mesh_destroy(fk_mesh *mesh);
+/* nnz of the pattern, number of element colours, nodes used by no element */
+int fk_mesh_info(const fk_mesh *mesh, int64_t *nnz, int32_t *ncolours, int64_t *unused_nodes);
+/* Element volumes |det|/6 (mesh.VE fill, src/mesh.jl:216-219). VE: nt. */
+int fk_mesh_volumes(fk_mesh *mesh, double *VE);
+
+/* ---- pattern and assembly ------------------------------------------------
+ * CSR pattern of the stiffness matrix: the sorted unique node pairs of all
+ * elements, explicit zeros kept -- exactly the structure of Julia's
+ * sparse(rowIDs, colIDs, ...) in NewtonRaphson.jl:161. The pattern is
+ * symmetric, so rowptr/colidx are also the CSC colptr/rowval of
+ * SparseMatrixCSC. rowptr: nv+1, colidx: nnz. */
+int fk_pattern(fk_mesh *mesh, int32_t index_base, int32_t *rowptr, int32_t *colidx);
+/* Per-element coefficient f (mu), nt values, caller's element order.
+ * Non-finite values return FK_ERR_NONFINITE. */
+int fk_set_coefficient(fk_mesh *mesh, const double *f);
+/* Assemble A(f) values on the GPU (FP64; FP32 copy made when a mixed solve
+ * needs it). method: FK_ASM_COLOURED or FK_ASM_ATOMIC. */
+int fk_assemble(fk_mesh *mesh, int32_t method);
+/* Copy the assembled FP64 values (nnz, CSR order == CSC order). */
+int fk_values(fk_mesh *mesh, double *values);
+int fk_values_f32(fk_mesh *mesh, float *values);
+/* stiffnessMatrix(mesh, f) -> SparseMatrixCSC (src/FEM3D.jl:40-56,
+ * src/FEMc.cpp:102-133): set f, assemble, copy pattern and values.
+ * colptr: nv+1, rowval/nzval: nnz (fk_mesh_info). */
+int fk_stiffness_matrix(fk_mesh *mesh, const double *f, int32_t index_base, int32_t *colptr,
+                        int32_t *rowval, double *nzval);
+/* localStiffnessMatrix(mesh, f) -> Ak 16 x nt column-major
+ * (src/FEM3D.jl:59-76, src/FEMc.cpp:76-99). f may be NULL (ones). */
+int fk_local_stiffness(fk_mesh *mesh, const double *f, double *Ak);
+/* tangentialStiffnessMatrix(mesh, H_vec, dmu) (src/FEM3D.jl:79-119): values
+ * on the same pattern. H is 3 x nt, dmu nt. Elements with |H| == 0
+ * contribute 0 (the reference divides by |H| there, FEM3D.jl:98). */
+int fk_tangential_stiffness(fk_mesh *mesh, const double *H, const double *dmu, double *nzval);
+/* lagrange(mesh) (src/FEM3D.jl:274-281): C[nds] += VE/4. C: nv. */
+int fk_lagrange(fk_mesh *mesh, double *C);
+
+/* ---- Dirichlet data and solve --------------------------------------------
+ * Fixed nodes (mask != 0) keep phi = g; free nodes solve
+ * A[free,free] x = -A[free,fixed] g (Dirichlet elimination as in
+ * Fluids/fluids3D.jl:142-174, done with a row/column mask, no submatrix). */
+int fk_set_dirichlet(fk_mesh *mesh, const uint8_t *mask, const double *g);
+/* Jacobi-preconditioned CG (the solver of magnetostatics.cpp:26-38 and
+ * run.jl:141) on the assembled matrix, stopping at ||r||_2 < rtol ||b||_2.
+ * precision FK_PREC_F64: one FP64 PCG to rtol.
+ * precision FK_PREC_F32_REFINED: FP32 PCG to inner_rtol inside an FP64
+ * residual/update loop until the true FP64 residual meets rtol or max_outer.
+ * warm_start != 0 starts from the resident potential. The potential stays
+ * on the GPU (fk_potential copies it out).
+ * iterations: total PCG iterations; outer: refinement steps; relres: final
+ * ||r||/||b|| (FP64 true residual for the refined solve). */
+int fk_solve(fk_mesh *mesh, int32_t precision, double rtol, int64_t max_iter, double inner_rtol,
+             int32_t max_outer, int32_t warm_start, int64_t *iterations, int32_t *outer,
+             double *relres);
+/* Solve A[free,free] x = b[free] with x = 0 on fixed nodes (b: nv, x: nv,
+ * FP64 PCG). The matrix is the last assembled one. Used for Newton updates
+ * and adjoint solves. */
+int fk_solve_rhs(fk_mesh *mesh, const double *b, double rtol, int64_t max_iter, double *x,
+                 int64_t *iterations, double *relres);
+int fk_potential(fk_mesh *mesh, double *phi);
+int fk_set_potential(fk_mesh *mesh, const double *phi);
+
+/* ---- field -----------------------------------------------------------------
+ * H = -grad(phi) per element from the resident potential
+ * (Magnetostatics/magneticField.jl:111-130, correct z component as in
+ * manyBodyIntersecting.jl:241-266). H: 3 x nt (may be NULL), Hn: |H| (may be
+ * NULL), energy: 1/2 sum mu |H|^2 VE with the current coefficient (may be
+ * NULL; src/magneticProperties.jl:268-273 with the element permeability). */
+int fk_field(fk_mesh *mesh, double *H, double *Hn, double *energy);
+
+/* ---- instrumentation -------------------------------------------------------
+ * Average device time of the sparse matrix-vector kernel over the last solve
+ * (HIP events on the context stream around each launch), when enabled with
+ * fk_set_timing(ctx, 1). Returns launches and total milliseconds. */
+int fk_set_timing(fk_context *ctx, int32_t enable);
+int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms);
+/* SpMV tuning: lanes per row (4, 8, 16, 32, 64) of the CSR kernels. */
+int fk_set_lanes_per_row(fk_mesh *mesh, int32_t lanes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEMEKO_HIP_H */

@@ -1,0 +1,366 @@
+"""Python face of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+FP64 restatement of Femeko.jl's P1 magnetostatics path (liboracle.so, built
+from femeko_oracle.c by oracle/Makefile) plus numpy pieces for the parts that
+have no C twin (energy, Froehlich law, Newton loop, adjoint). It checks the HIP
+library in tests/ and is timed as the CPU baseline in bench.py; the shipped
+library never imports it.
+
+Every function names the reference lines it restates. Pieces without a
+reference counterpart (adjoint, Froehlich workload law) say so.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+MU0 = np.pi * 4e-7  # Magnetostatics/magneticField.jl:38
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_up = ctypes.POINTER(ctypes.c_uint8)
+_i64 = ctypes.c_int64
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.or_abcd.argtypes = [_dp, _ip, ctypes.c_int32, _dp]
+        L.or_element_volume.argtypes = [_dp, _ip]
+        L.or_element_volume.restype = ctypes.c_double
+        L.or_element_gradients.argtypes = [_dp, _ip, _i64, _dp, ctypes.c_int]
+        L.or_local_stiffness.argtypes = [_dp, _ip, _i64, _dp, _dp, _dp, ctypes.c_int]
+        L.or_tangent_local.argtypes = [_dp, _ip, _i64, _dp, _dp, _dp, _dp, ctypes.c_int]
+        L.or_sparse_csc.argtypes = [_i64, _i64, _ip, _dp, _dp, _ip, _ip, _dp]
+        L.or_sparse_csc.restype = _i64
+        L.or_spmv_csc.argtypes = [_i64, _ip, _ip, _dp, _dp, _dp]
+        L.or_pcg_free.argtypes = [_i64, _ip, _ip, _dp, _up, _dp, ctypes.c_double, _i64, _dp,
+                                  ctypes.POINTER(_i64), _dp, ctypes.c_int, ctypes.c_int]
+        L.or_pcg_dirichlet.argtypes = L.or_pcg_free.argtypes
+        L.or_pcg_iterations.argtypes = [_i64, _ip, _ip, _dp, _up, _dp, _i64, _dp, ctypes.c_int]
+        L.or_field.argtypes = [_dp, _ip, _i64, _dp, _dp, _dp, ctypes.c_int]
+        L.or_lagrange.argtypes = [_i64, _i64, _ip, _dp, _dp]
+        L.or_boundary_integral.argtypes = [_i64, _i64, _ip, _dp, _dp, _dp, _ip, ctypes.c_int, _dp]
+        L.or_num_threads_available.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _u(a):
+    return a.ctypes.data_as(_up)
+
+
+def _p(mesh):
+    """p as a contiguous 3 x nv column-major buffer (x,y,z per node)."""
+    return np.ascontiguousarray(mesh.p.T)
+
+
+def _t(mesh):
+    return np.ascontiguousarray(mesh.t.T, dtype=np.int32)
+
+
+def threads() -> int:
+    return int(lib().or_num_threads_available())
+
+
+# --------------------------------------------------------------------------
+def abcd(p: np.ndarray, nodes, nd: int):
+    """src/FEM3D.jl:5-17 -> (a, b, c, d). p is (3, nv); nodes 1-based."""
+    pp = np.ascontiguousarray(np.asarray(p, dtype=np.float64).T)
+    nn = np.ascontiguousarray(np.asarray(nodes, dtype=np.int32))
+    r = np.zeros(4)
+    s = lib().or_abcd(_d(pp), _i(nn), int(nd), _d(r))
+    if s:
+        raise ValueError("singular element")
+    return tuple(r)
+
+
+def element_gradients(mesh, nthreads: int = 0) -> np.ndarray:
+    """(nt, 4, 3) basis gradients of every element, via abcd."""
+    G = np.empty((mesh.nt, 4, 3))
+    s = lib().or_element_gradients(_d(_p(mesh)), _i(_t(mesh)), mesh.nt, _d(G), nthreads or threads())
+    if s:
+        raise ValueError("singular element")
+    return G
+
+
+def localStiffnessMatrix(mesh, f=None, nthreads: int = 0) -> np.ndarray:
+    """src/FEM3D.jl:59-76 -> Ak (16, nt), column k = vec(VE f (bb'+cc'+dd'))."""
+    Ak = np.empty((mesh.nt, 16))
+    fp = None if f is None else _d(np.ascontiguousarray(f, dtype=np.float64))
+    s = lib().or_local_stiffness(_d(_p(mesh)), _i(_t(mesh)), mesh.nt,
+                                 _d(np.ascontiguousarray(mesh.VE)), fp, _d(Ak), nthreads or threads())
+    if s:
+        raise ValueError("singular element")
+    return Ak.T
+
+
+def tangential_local(mesh, H: np.ndarray, dmu: np.ndarray, nthreads: int = 0) -> np.ndarray:
+    """Element matrices of tangentialStiffnessMatrix, src/FEM3D.jl:79-108. H is (3, nt)."""
+    Ak = np.empty((mesh.nt, 16))
+    Hc = np.ascontiguousarray(np.asarray(H, dtype=np.float64).T)
+    s = lib().or_tangent_local(_d(_p(mesh)), _i(_t(mesh)), mesh.nt, _d(np.ascontiguousarray(mesh.VE)),
+                               _d(Hc), _d(np.ascontiguousarray(dmu, dtype=np.float64)), _d(Ak),
+                               nthreads or threads())
+    if s:
+        raise ValueError("singular element")
+    return Ak.T
+
+
+@dataclass
+class CSC:
+    """Julia SparseMatrixCSC arrays (1-based colptr / rowval)."""
+
+    n: int
+    colptr: np.ndarray
+    rowval: np.ndarray
+    nzval: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.nzval.size)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csc_matrix((self.nzval, self.rowval - 1, self.colptr - 1), shape=(self.n, self.n))
+
+
+def sparse_from_local(mesh, Ak: np.ndarray, mu=None) -> CSC:
+    """sparse(rowIDs, colIDs, Ak[n,k]*mu[k]) (Magnetostatics/NewtonRaphson.jl:115-127,154-161)."""
+    nv, nt = mesh.nv, mesh.nt
+    Akc = np.ascontiguousarray(np.asarray(Ak).T)  # (nt, 16)
+    colptr = np.empty(nv + 1, np.int32)
+    rowval = np.empty(16 * nt, np.int32)
+    nzval = np.empty(16 * nt)
+    mup = None if mu is None else _d(np.ascontiguousarray(mu, dtype=np.float64))
+    nnz = lib().or_sparse_csc(nv, nt, _i(_t(mesh)), _d(Akc), mup, _i(colptr), _i(rowval), _d(nzval))
+    if nnz < 0:
+        raise MemoryError("oracle sparse build")
+    return CSC(nv, colptr, rowval[:nnz].copy(), nzval[:nnz].copy())
+
+
+def stiffnessMatrix(mesh, f=None, nthreads: int = 0) -> CSC:
+    """src/FEM3D.jl:40-56 with the zero-keeping single-call sparse convention."""
+    return sparse_from_local(mesh, localStiffnessMatrix(mesh, f, nthreads))
+
+
+def tangentialStiffnessMatrix(mesh, H, dmu, nthreads: int = 0) -> CSC:
+    """src/FEM3D.jl:79-119 (zero-keeping convention, |H| = 0 elements contribute 0)."""
+    return sparse_from_local(mesh, tangential_local(mesh, H, dmu, nthreads))
+
+
+def spmv(A: CSC, x: np.ndarray) -> np.ndarray:
+    y = np.empty(A.n)
+    lib().or_spmv_csc(A.n, _i(A.colptr), _i(A.rowval), _d(A.nzval),
+                      _d(np.ascontiguousarray(x, dtype=np.float64)), _d(y))
+    return y
+
+
+def pcg_free(A: CSC, mask: np.ndarray, b: np.ndarray, rtol: float, maxiter: int = 100000,
+             x0=None, exact: bool = True, nthreads: int = 0):
+    """Jacobi-PCG on A[free,free] x = b[free], x = 0 on fixed nodes."""
+    x = np.zeros(A.n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = _i64(0)
+    rel = ctypes.c_double(0.0)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    s = lib().or_pcg_free(A.n, _i(A.colptr), _i(A.rowval), _d(A.nzval), _u(m),
+                          _d(np.ascontiguousarray(b, dtype=np.float64)), rtol, maxiter, _d(x),
+                          ctypes.byref(it), ctypes.byref(rel), nthreads or threads(), int(exact))
+    return x, int(it.value), float(rel.value), int(s)
+
+
+def pcg_dirichlet(A: CSC, mask: np.ndarray, g: np.ndarray, rtol: float, maxiter: int = 100000,
+                  exact: bool = True, nthreads: int = 0):
+    """Dirichlet elimination (Fluids/fluids3D.jl:142-174) + Jacobi-PCG. Returns phi, iters, relres, status."""
+    phi = np.zeros(A.n)
+    it = _i64(0)
+    rel = ctypes.c_double(0.0)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    s = lib().or_pcg_dirichlet(A.n, _i(A.colptr), _i(A.rowval), _d(A.nzval), _u(m),
+                               _d(np.ascontiguousarray(g, dtype=np.float64)), rtol, maxiter, _d(phi),
+                               ctypes.byref(it), ctypes.byref(rel), nthreads or threads(), int(exact))
+    return phi, int(it.value), float(rel.value), int(s)
+
+
+def field(mesh, u: np.ndarray, nthreads: int = 0):
+    """H recovery, magneticField.jl:111-130 (correct z): returns H (3, nt) and |H| (nt)."""
+    H = np.empty((mesh.nt, 3))
+    Hn = np.empty(mesh.nt)
+    s = lib().or_field(_d(_p(mesh)), _i(_t(mesh)), mesh.nt, _d(np.ascontiguousarray(u, dtype=np.float64)),
+                       _d(H), _d(Hn), nthreads or threads())
+    if s:
+        raise ValueError("singular element")
+    return H.T, Hn
+
+
+def energy(mesh, Hn: np.ndarray, mu: np.ndarray) -> float:
+    """Linear magnetic energy 1/2 sum_e mu_e |H_e|^2 VE_e (the free-space density
+    of src/magneticProperties.jl:268-273 with the element permeability)."""
+    return float(np.sum(0.5 * mu * Hn * Hn * mesh.VE))
+
+
+def lagrange(mesh) -> np.ndarray:
+    """src/FEM3D.jl:274-281."""
+    C = np.empty(mesh.nv)
+    lib().or_lagrange(mesh.nv, mesh.nt, _i(_t(mesh)), _d(np.ascontiguousarray(mesh.VE)), _d(C))
+    return C
+
+
+def BoundaryIntegral(mesh, F, shell_id) -> np.ndarray:
+    """src/FEM3D.jl:284-297."""
+    RHS = np.empty(mesh.nv)
+    sT = np.ascontiguousarray(mesh.surfaceT.T, dtype=np.int32)
+    nrm = np.ascontiguousarray(mesh.normal.T, dtype=np.float64)
+    ids = np.ascontiguousarray(np.atleast_1d(shell_id), dtype=np.int32)
+    lib().or_boundary_integral(mesh.nv, mesh.ns, _i(sT), _d(nrm), _d(np.ascontiguousarray(mesh.AE)),
+                               _d(np.ascontiguousarray(F, dtype=np.float64)), _i(ids), int(ids.size), _d(RHS))
+    return RHS
+
+
+def lineSearch(u: np.ndarray, du: np.ndarray, A: CSC, RHS: np.ndarray, minStep: float = 1e-4) -> float:
+    """src/FEM3D.jl:123-152: backtracking alpha <- 0.9 alpha; mutates u."""
+    alf = 1.0
+    residual_old = np.linalg.norm(RHS - spmv(A, u))
+    u_trial = u + du
+    residue = np.linalg.norm(RHS - spmv(A, u_trial))
+    while residue > residual_old and alf > minStep:
+        alf *= 0.9
+        u_trial = u + alf * du
+        residue = np.linalg.norm(RHS - spmv(A, u_trial))
+    if not alf < minStep:
+        u[:] = u_trial
+    return float(residue)
+
+
+# --------------------------------------------------------------------------
+# Froehlich permeability law of the nonlinear workload (no reference
+# counterpart: Femeko evaluates tabulated splines, NewtonRaphson.jl:57-63).
+def froehlich(Hn: np.ndarray, Ms: float, a: float):
+    """mu = mu0 (1 + Ms/(a+|H|)), dmu/d|H| = -mu0 Ms/(a+|H|)^2."""
+    d = a + Hn
+    return MU0 * (1.0 + Ms / d), -MU0 * Ms / (d * d)
+
+
+def newton_froehlich(mesh, mask, g, magnet: np.ndarray, Ms: float, a: float,
+                     rtol: float = 1e-10, tol_mu: float = 1e-6, max_outer: int = 20,
+                     newton: bool = True, switch: float = 0.25):
+    """Picard/Newton mu-update loop of Magnetostatics/NewtonRaphson.jl:148-212
+    with Dirichlet data instead of the Neumann+Lagrange saddle system and the
+    correct H_z. Iteration 1 starts from mu = mu0 with no tangent (the
+    reference's att == 1 branch). An iteration adds the tangent
+    tangentialStiffnessMatrix (a Newton step) once the previous relative mu
+    change is below `switch`, otherwise it is a Picard step; far from the
+    solution undamped Newton oscillates for this law, which the reference
+    counters with lineSearch. Stops when max |mu_new - mu| / mu_new over magnet
+    elements < tol_mu. Returns dict(phi, H, Hn, mu, iters, cg_iters, dmu_hist,
+    steps)."""
+    Ak = localStiffnessMatrix(mesh)
+    mu = np.full(mesh.nt, MU0)
+    dmu = np.zeros(mesh.nt)
+    u = np.where(mask.astype(bool), g, 0.0)
+    H = np.zeros((3, mesh.nt))
+    hist = []
+    steps = []
+    cg_total = 0
+    it = 0
+    dev = np.inf
+    for it in range(1, max_outer + 1):
+        A = sparse_from_local(mesh, Ak, mu)
+        use_newton = newton and dev < switch
+        steps.append("N" if use_newton else "P")
+        if use_newton:
+            At = tangentialStiffnessMatrix(mesh, H, dmu)
+            J = CSC(A.n, A.colptr, A.rowval, A.nzval + At.nzval)
+        else:
+            J = A
+        res = -spmv(A, u)
+        du, cgit, _rel, s = pcg_free(J, mask, res, rtol)
+        if s:
+            raise RuntimeError(f"oracle PCG status {s}")
+        cg_total += cgit
+        u = u + du
+        H, Hn = field(mesh, u)
+        mu_new, dmu_new = froehlich(Hn, Ms, a)
+        mu_new = np.where(magnet, mu_new, MU0)
+        dmu = np.where(magnet, dmu_new, 0.0)
+        dev = float(np.max(np.abs(mu_new - mu)[magnet] / mu_new[magnet])) if magnet.any() else 0.0
+        hist.append(dev)
+        mu = mu_new
+        if not (np.all(np.isfinite(mu)) and np.all(np.isfinite(dmu))):
+            raise FloatingPointError("Nans/Infs in mu")  # NewtonRaphson.jl:210-212
+        if dev < tol_mu:
+            break
+    H, Hn = field(mesh, u)
+    return {"phi": u, "H": H, "Hn": Hn, "mu": mu, "iters": it, "cg_iters": cg_total, "dmu_hist": hist,
+            "steps": steps}
+
+
+# --------------------------------------------------------------------------
+# Adjoint of the field-target objective (no reference counterpart).
+def adjoint_gradient(mesh, mask, g, rho, target: np.ndarray, Hstar, rtol: float = 1e-13,
+                     mur=lambda r: 1.0 + 999.0 * r ** 3, dmur=lambda r: 3.0 * 999.0 * r ** 2):
+    """J = sum_{e in target} VE_e |H_e - H*|^2 with mu_e = mu0 mur(rho_e).
+
+    State:   A(rho)[free,free] phi_f = -A[free,fixed] g.
+    Adjoint: A[free,free] lam_f = -(dJ/dphi)_f, lam = 0 on fixed nodes.
+    Grad:    dJ/drho_e = mu0 mur'(rho_e) VE_e (sum lam_i grad_i).(sum phi_i grad_i).
+    Returns (J, grad, phi, lam)."""
+    mu = MU0 * mur(rho)
+    A = stiffnessMatrix(mesh, mu)
+    phi, _it, _rel, s = pcg_dirichlet(A, mask, g, rtol)
+    if s:
+        raise RuntimeError(f"oracle PCG status {s}")
+    G = element_gradients(mesh)  # (nt,4,3)
+    t0 = mesh.t.T - 1  # (nt,4)
+    H = -np.einsum("ki,kic->kc", phi[t0], G)
+    d = H - np.asarray(Hstar)[None, :]
+    w = np.where(target, mesh.VE, 0.0)
+    J = float(np.sum(w * np.sum(d * d, axis=1)))
+    # dJ/dphi_i = sum_e 2 VE (H-H*) . dH/dphi_i = sum_e -2 VE (H-H*).grad_i
+    contrib = -2.0 * w[:, None] * np.einsum("kc,kic->ki", d, G)
+    dJ = np.zeros(mesh.nv)
+    np.add.at(dJ, t0.ravel(), contrib.ravel())
+    lam, _it, _rel, s = pcg_free(A, mask, -dJ, rtol)
+    if s:
+        raise RuntimeError(f"oracle PCG status {s}")
+    glam = np.einsum("ki,kic->kc", lam[t0], G)
+    gphi = np.einsum("ki,kic->kc", phi[t0], G)
+    grad = MU0 * dmur(rho) * mesh.VE * np.sum(glam * gphi, axis=1)
+    return J, grad, phi, lam
+
+
+def objective(mesh, mask, g, rho, target, Hstar, rtol: float = 1e-13,
+              mur=lambda r: 1.0 + 999.0 * r ** 3) -> float:
+    """J alone (for finite differences)."""
+    mu = MU0 * mur(rho)
+    A = stiffnessMatrix(mesh, mu)
+    phi, _it, _rel, s = pcg_dirichlet(A, mask, g, rtol)
+    if s:
+        raise RuntimeError(f"oracle PCG status {s}")
+    H, _Hn = field(mesh, phi)
+    d = H.T - np.asarray(Hstar)[None, :]
+    return float(np.sum(np.where(target, mesh.VE, 0.0) * np.sum(d * d, axis=1)))

@@ -1,0 +1,286 @@
+"""GPU parity of the HIP library against the CPU oracle, through the C ABI.
+
+Bars (written next to each check):
+  - CSR/CSC pattern: bit-exact against the oracle's Julia sparse() arrays.
+  - FP64 values: |a_gpu - a_ref| <= 1e-12 * sum|element terms| per entry.
+  - potential: the GPU solves to rtol 1e-10 and is compared with a direct
+    solve; ||phi - phi_ref|| / ||phi_ref|| <= 1e-9.
+  - H, |H|, energy from the same potential: 1e-12 relative.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from femeko_hip import _lib
+from femeko_hip import workloads as W
+from femeko_hip.device import DeviceMesh, context
+from femeko_hip._lib import FemekoHIPError
+from femeko_hip.mesh import MESH, boundary_node_mask, element_volumes, kuhn_box_mesh
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "linear20.npz")
+
+
+def direct_solution(A_csc_scipy, mask, g):
+    S = A_csc_scipy.tocsr()
+    free = mask == 0
+    phi = g.copy()
+    phi[free] = sla.spsolve(S[free][:, free].tocsc(), -S[free][:, ~free] @ g[~free])
+    return phi
+
+
+def abs_terms(oracle, mesh, f):
+    """sum over elements of |local entry| per CSC entry (scale of the value metric)"""
+    return oracle.sparse_from_local(mesh, np.abs(oracle.localStiffnessMatrix(mesh, f))).nzval
+
+
+@pytest.fixture(scope="module")
+def lin20(oracle):
+    pr = W.linear(20)
+    A = oracle.stiffnessMatrix(pr.mesh, pr.mu)
+    dm = DeviceMesh(pr.mesh, context(0))
+    dm.set_coefficient(pr.mu)
+    dm.assemble()
+    dm.set_dirichlet(pr.mask, pr.g)
+    yield pr, A, dm
+    dm.close()
+
+
+# ---------------------------------------------------------------------------
+# pattern and values
+def test_pattern_bit_exact_linear20(lin20):
+    pr, A, dm = lin20
+    rowptr, colidx = dm.pattern(base=1)
+    assert np.array_equal(rowptr, A.colptr)
+    assert np.array_equal(colidx, A.rowval)
+    assert dm.nnz == 128581
+
+
+@pytest.mark.parametrize("n,jit,shuffle", [(1, 0.0, False), (3, 0.0, True), (5, 0.2, True)])
+def test_pattern_bit_exact_small_and_shuffled(oracle, n, jit, shuffle):
+    m = kuhn_box_mesh(n, jit, seed=n)
+    if shuffle:
+        # renumber nodes and reorder elements: the library must not rely on
+        # the structured numbering
+        rng = np.random.default_rng(n)
+        pn = rng.permutation(m.nv)
+        inv = np.empty_like(pn)
+        inv[pn] = np.arange(m.nv)
+        p = np.asfortranarray(m.p[:, pn])
+        t = np.asfortranarray((inv[m.t - 1] + 1).astype(np.int32)[:, rng.permutation(m.nt)])
+        m = MESH(p=p, t=t, VE=element_volumes(p, t))
+    A = oracle.stiffnessMatrix(m)
+    dm = DeviceMesh(m, context(0))
+    try:
+        rowptr, colidx = dm.pattern(base=1)
+        assert np.array_equal(rowptr, A.colptr) and np.array_equal(colidx, A.rowval)
+        r0, c0 = dm.pattern(base=0)
+        assert np.array_equal(r0 + 1, rowptr) and np.array_equal(c0 + 1, colidx)
+        f = np.linspace(0.5, 2.0, m.nt)
+        dm.set_coefficient(f)
+        dm.assemble()
+        v = dm.values()
+        Af = oracle.stiffnessMatrix(m, f)
+        bound = 1e-12 * abs_terms(oracle, m, f)
+        assert np.all(np.abs(v - Af.nzval) <= bound)
+    finally:
+        dm.close()
+
+
+def test_values_fp64_metric(oracle, lin20):
+    pr, A, dm = lin20
+    v = dm.values()
+    bound = abs_terms(oracle, pr.mesh, pr.mu)
+    err = np.abs(v - A.nzval)
+    assert np.all(err <= 1e-12 * bound), float((err / bound).max())
+    assert np.linalg.norm(v - A.nzval) / np.linalg.norm(A.nzval) < 1e-14
+    golden = np.load(GOLDEN)
+    assert np.all(np.abs(v - golden["nzval"]) <= 1e-12 * bound)
+
+
+def test_values_symmetric_deterministic_and_atomic(oracle, lin20):
+    pr, A, dm = lin20
+    v1 = dm.values()
+    dm.assemble("coloured")
+    v2 = dm.values()
+    assert np.array_equal(v1.view(np.uint64), v2.view(np.uint64))  # bitwise reproducible
+    rowptr, colidx = dm.pattern(base=0)
+    S = sp.csr_matrix((v1, colidx, rowptr), shape=(dm.nv, dm.nv))
+    assert (S != S.T).nnz == 0  # exactly symmetric
+    dm.assemble("atomic")
+    va = dm.values()
+    bound = abs_terms(oracle, pr.mesh, pr.mu)
+    assert np.all(np.abs(va - A.nzval) <= 1e-12 * bound)
+    dm.assemble("coloured")
+    f32 = dm.values_f32()
+    assert np.array_equal(f32, v1.astype(np.float32))
+
+
+def test_local_stiffness_volumes_lagrange(oracle, lin20):
+    pr, A, dm = lin20
+    m = pr.mesh
+    Ak = dm.local_stiffness(pr.mu)
+    Ak_ref = oracle.localStiffnessMatrix(m, pr.mu)
+    scale = np.abs(Ak_ref).max(axis=0)
+    assert np.all(np.abs(Ak - Ak_ref) <= 1e-12 * scale[None, :])
+    Ak1 = dm.local_stiffness(None)
+    assert np.all(np.abs(Ak1 - oracle.localStiffnessMatrix(m)) <= 1e-12 * np.abs(Ak1).max(axis=0)[None, :])
+    VE = dm.volumes()
+    np.testing.assert_allclose(VE, m.VE, rtol=1e-13)
+    C = dm.lagrange()
+    np.testing.assert_allclose(C, oracle.lagrange(m), rtol=1e-13)
+
+
+def test_tangent_values(oracle, lin20):
+    pr, A, dm = lin20
+    m = pr.mesh
+    rng = np.random.default_rng(3)
+    H = rng.normal(size=(3, m.nt)) * 1e3
+    H[:, ::50] = 0.0  # |H| = 0 elements contribute nothing
+    dmu = -np.abs(rng.normal(size=m.nt)) * 1e-9
+    v = dm.tangential_values(H, dmu)
+    ref = oracle.tangentialStiffnessMatrix(m, H, dmu).nzval
+    scale = oracle.sparse_from_local(m, np.abs(oracle.tangential_local(m, H, dmu))).nzval
+    assert np.all(np.abs(v - ref) <= 1e-12 * scale + 1e-300)
+
+
+# ---------------------------------------------------------------------------
+# solve and field
+def test_solve_f64_linear20(oracle, lin20):
+    pr, A, dm = lin20
+    golden = np.load(GOLDEN)
+    rep = dm.solve("f64", rtol=1e-10)
+    phi = dm.potential()
+    ref = golden["phi"]
+    assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) < 1e-9
+    assert rep.relres < 1e-10
+    # same algorithm as the oracle: iteration counts agree within the spread that
+    # last-bit differences cause at this permeability contrast (about +-3)
+    it_ref = int(golden["pcg_iters_1e10"])
+    assert abs(rep.iterations - it_ref) <= max(3, 0.05 * it_ref)
+    # fixed nodes carry g exactly
+    assert np.array_equal(phi[pr.mask == 1], pr.g[pr.mask == 1])
+
+
+def test_solve_f32_refined_linear20(lin20):
+    pr, A, dm = lin20
+    golden = np.load(GOLDEN)
+    rep = dm.solve("f32", rtol=1e-10, inner_rtol=1e-5)
+    phi = dm.potential()
+    ref = golden["phi"]
+    assert rep.relres < 1e-10 and rep.outer >= 2
+    assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) < 1e-9
+
+
+def test_true_residual_of_gpu_solution(lin20):
+    pr, A, dm = lin20
+    dm.solve("f64", rtol=1e-10)
+    phi = dm.potential()
+    S = A.to_scipy().tocsr()
+    free = pr.mask == 0
+    r = (S @ phi)[free]
+    b = (S @ pr.g)[free]
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 2e-10
+
+
+def test_field_and_energy(oracle, lin20):
+    pr, A, dm = lin20
+    golden = np.load(GOLDEN)
+    dm.set_potential(golden["phi"])
+    H, Hn, E = dm.field(want_energy=True)
+    Href, Hnref = oracle.field(pr.mesh, golden["phi"])
+    scale = np.abs(Hnref).max()
+    assert np.abs(H - Href).max() <= 1e-12 * scale
+    assert np.abs(Hn - golden["Hn"]).max() <= 1e-12 * scale
+    assert E == pytest.approx(float(golden["energy"]), rel=1e-12)
+    # energy identity: 1/2 phi' A phi from the GPU matrix equals the field energy
+    rowptr, colidx = dm.pattern(base=0)
+    S = sp.csr_matrix((dm.values(), colidx, rowptr), shape=(dm.nv, dm.nv))
+    assert 0.5 * golden["phi"] @ (S @ golden["phi"]) == pytest.approx(E, rel=1e-12)
+
+
+def test_patch_test_on_gpu():
+    m = kuhn_box_mesh(12, 0.2, seed=21)
+    H0 = np.array([0.4, 1.0, -0.3])
+    mask = boundary_node_mask(m)
+    g = -(H0 @ m.p)
+    dm = DeviceMesh(m, context(0))
+    try:
+        dm.set_coefficient(np.full(m.nt, 3.0))
+        dm.assemble()
+        dm.set_dirichlet(mask, np.where(mask == 1, g, 0.0))
+        for prec in ("f64", "f32"):
+            dm.solve(prec, rtol=1e-12)
+            phi = dm.potential()
+            assert np.abs(phi - g).max() < 1e-9
+            H, Hn, _ = dm.field()
+            assert np.abs(H - H0[:, None]).max() < 1e-7
+    finally:
+        dm.close()
+
+
+def test_solve_rhs_matches_oracle(oracle, lin20):
+    pr, A, dm = lin20
+    rng = np.random.default_rng(5)
+    b = rng.normal(size=dm.nv) * 1e-7
+    x, it, rel = dm.solve_rhs(b, rtol=1e-12)
+    xr, itr, relr, s = oracle.pcg_free(A, pr.mask, b, 1e-13)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-9
+    assert np.all(x[pr.mask == 1] == 0.0)
+
+
+# ---------------------------------------------------------------------------
+# error paths (all forced from the host)
+def test_error_statuses():
+    m = kuhn_box_mesh(3, 0.1, seed=2)
+    ctx = context(0)
+    bad = MESH(p=m.p, t=np.asfortranarray(m.t.copy()), VE=m.VE)
+    bad.t[2, 5] = m.nv + 7
+    with pytest.raises(FemekoHIPError) as ei:
+        DeviceMesh(bad, ctx)
+    assert ei.value.status == _lib.FK_ERR_INDEX
+    rep = MESH(p=m.p, t=np.asfortranarray(m.t.copy()), VE=m.VE)
+    rep.t[1, 0] = rep.t[0, 0]
+    with pytest.raises(FemekoHIPError) as ei:
+        DeviceMesh(rep, ctx, check_volumes=False)
+    assert ei.value.status == _lib.FK_ERR_DEGENERATE
+    flat = MESH(p=np.asfortranarray(m.p.copy()), t=m.t, VE=m.VE)
+    nds = m.t[:, 0] - 1
+    flat.p[2, nds] = 0.0  # all four nodes of element 1 coplanar
+    flat.p[2, nds] = flat.p[2, nds[0]]
+    with pytest.raises(FemekoHIPError) as ei:
+        DeviceMesh(flat, ctx, check_volumes=False)
+    assert ei.value.status in (_lib.FK_ERR_DEGENERATE, _lib.FK_ERR_INDEX)
+    wrongVE = MESH(p=m.p, t=m.t, VE=m.VE * 1.01)
+    with pytest.raises(FemekoHIPError) as ei:
+        DeviceMesh(wrongVE, ctx)
+    assert ei.value.status == _lib.FK_ERR_INVALID_ARG
+    dm = DeviceMesh(m, ctx)
+    try:
+        with pytest.raises(FemekoHIPError) as ei:
+            dm.assemble()
+        assert ei.value.status == _lib.FK_ERR_STATE
+        mu = np.ones(m.nt)
+        mu[3] = np.nan
+        with pytest.raises(FemekoHIPError) as ei:
+            dm.set_coefficient(mu)
+        assert ei.value.status == _lib.FK_ERR_NONFINITE
+        dm.set_coefficient(np.ones(m.nt))
+        dm.assemble()
+        with pytest.raises(FemekoHIPError) as ei:
+            dm.solve("f64")
+        assert ei.value.status == _lib.FK_ERR_STATE
+        dm.set_dirichlet(np.zeros(m.nv, np.uint8), np.zeros(m.nv))
+        with pytest.raises(FemekoHIPError) as ei:
+            dm.solve("f64")
+        assert ei.value.status == _lib.FK_ERR_SINGULAR
+        mask = boundary_node_mask(m)
+        dm.set_dirichlet(mask, np.where(mask == 1, m.p[0], 0.0))
+        rep = dm.solve("f64", rtol=1e-14, max_iter=2, raise_on_error=False)
+        assert rep.status == _lib.FK_ERR_NOT_CONVERGED and rep.iterations == 2
+        rep = dm.solve("f64", rtol=1e-12)
+        assert rep.relres < 1e-12
+    finally:
+        dm.close()

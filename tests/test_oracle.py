@@ -1,0 +1,388 @@
+"""Pins of the CPU oracle (oracle/femeko_oracle.c, oracle/oracle.py).
+
+The reference cannot run here (no Julia, empty Eigen submodule), so every
+oracle function is checked against something it does not compute itself:
+closed forms on the unjittered Kuhn tetrahedron, LAPACK solves of the abcd
+system, brute-force pair sets, scipy's COO->CSC conversion and direct solver,
+the P1 patch test, algebraic identities and finite differences.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from femeko_hip import workloads as W
+from femeko_hip.mesh import MESH, MU0, boundary_faces, boundary_node_mask, element_volumes, kuhn_box_mesh, kuhn_counts
+
+
+def kuhn_tet(h=0.25):
+    p = np.asfortranarray(np.array([[0, 0, 0], [h, 0, 0], [h, h, 0], [h, h, h]], float).T)
+    t = np.asfortranarray(np.array([[1, 2, 3, 4]], np.int32).T)
+    return MESH(p=p, t=t, VE=element_volumes(p, t))
+
+
+# ---------------------------------------------------------------------------
+# A1/A2: abcd and element volumes
+def test_abcd_closed_form_kuhn_tet(oracle):
+    h = 0.25
+    m = kuhn_tet(h)
+    expect = np.array([[-1, 0, 0], [1, -1, 0], [0, 1, -1], [0, 0, 1]], float) / h
+    for i in range(4):
+        a, b, c, d = oracle.abcd(m.p, m.t[:, 0], i + 1)
+        np.testing.assert_allclose([b, c, d], expect[i], rtol=0, atol=1e-14 / h)
+        assert abs(a - (1.0 if i == 0 else 0.0)) < 1e-14  # lambda_i at the origin
+    assert m.VE[0] == pytest.approx(h ** 3 / 6, rel=1e-15)
+
+
+def test_abcd_matches_lapack_solve(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        P = rng.normal(size=(3, 4))
+        p = np.asfortranarray(P)
+        nodes = np.array([1, 2, 3, 4], np.int32)
+        for nd in range(1, 5):
+            order = [nd] + [k for k in nodes if k != nd]
+            M = np.column_stack([np.ones(4), P[:, np.array(order) - 1].T])
+            ref = np.linalg.solve(M, np.eye(4)[0])
+            got = np.array(oracle.abcd(p, nodes, nd))
+            np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
+
+
+def test_gradient_invariants(oracle):
+    m = kuhn_box_mesh(4, 0.2, seed=3)
+    G = oracle.element_gradients(m)           # (nt,4,3)
+    t0 = m.t.T - 1
+    scale = np.abs(G).max()
+    np.testing.assert_allclose(G.sum(axis=1), 0.0, atol=1e-13 * scale)
+    X = m.p.T[t0]                              # (nt,4,3)
+    for j in range(4):
+        lam = np.einsum("kic,kc->ki", G, X[:, j] - X[:, 0])  # grad_i . (x_j - x_0)
+        expect = np.zeros(4)
+        expect[j] += 1
+        expect[0] -= 1
+        np.testing.assert_allclose(lam, np.broadcast_to(expect, lam.shape), atol=1e-12)
+
+
+def test_volumes_sum_to_box(oracle):
+    m = kuhn_box_mesh(6, 0.2, seed=4)
+    assert m.VE.sum() == pytest.approx(8.0, rel=1e-13)
+    lib = oracle.lib()
+    p = np.ascontiguousarray(m.p.T)
+    for k in (0, 17, m.nt - 1):
+        nds = np.ascontiguousarray(m.t[:, k], dtype=np.int32)
+        assert lib.or_element_volume(oracle._d(p), oracle._i(nds)) == m.VE[k]
+
+
+# ---------------------------------------------------------------------------
+# A3: local stiffness
+def test_local_stiffness_closed_form(oracle):
+    h = 0.5
+    m = kuhn_box_mesh(1, 0.0, lo=0.0, hi=h)
+    Ak = oracle.localStiffnessMatrix(m)
+    path = h / 6 * np.array([[1, -1, 0, 0], [-1, 2, -1, 0], [0, -1, 2, -1], [0, 0, -1, 1]], float)
+    odd = (False, True, True, False, False, True)
+    for k in range(6):
+        K = Ak[:, k].reshape(4, 4, order="F")
+        expect = path.copy()
+        if odd[k]:  # last two nodes swapped to keep the orientation positive
+            expect = expect[[0, 1, 3, 2]][:, [0, 1, 3, 2]]
+        np.testing.assert_allclose(K, expect, atol=1e-15)
+
+
+def test_local_stiffness_sym_rowsum_psd(oracle):
+    m = kuhn_box_mesh(3, 0.2, seed=5)
+    f = np.linspace(1, 3, m.nt)
+    Ak = oracle.localStiffnessMatrix(m, f)
+    for k in range(0, m.nt, 7):
+        K = Ak[:, k].reshape(4, 4, order="F")
+        assert np.array_equal(K, K.T)
+        np.testing.assert_allclose(K.sum(axis=1), 0, atol=1e-14 * np.abs(K).max())
+        ev = np.linalg.eigvalsh(K)
+        assert ev.min() > -1e-13 * ev.max()
+        assert np.sum(ev > 1e-10 * ev.max()) == 3   # null space = constants
+
+
+# ---------------------------------------------------------------------------
+# A4/A5: sparse pattern and values
+@pytest.mark.parametrize("n,jit", [(1, 0.0), (2, 0.2), (3, 0.2), (5, 0.0)])
+def test_sparse_pattern_bruteforce(oracle, n, jit):
+    m = kuhn_box_mesh(n, jit, seed=n)
+    A = oracle.stiffnessMatrix(m)
+    pairs = set()
+    for k in range(m.nt):
+        for a in m.t[:, k]:
+            for b in m.t[:, k]:
+                pairs.add((int(b), int(a)))   # (col, row)
+    pairs = sorted(pairs)
+    cols = np.array([c for c, _ in pairs])
+    rows = np.array([r for _, r in pairs], np.int32)
+    colptr = np.concatenate([[1], 1 + np.cumsum(np.bincount(cols - 1, minlength=m.nv))]).astype(np.int32)
+    assert np.array_equal(A.colptr, colptr)
+    assert np.array_equal(A.rowval, rows)
+    assert A.nnz == kuhn_counts(n)["nnz"]
+
+
+def test_sparse_values_match_scipy_coo(oracle):
+    pr = W.linear(8)
+    m = pr.mesh
+    Ak = oracle.localStiffnessMatrix(m)
+    A = oracle.sparse_from_local(m, Ak, pr.mu)
+    I = []
+    J = []
+    V = []
+    for n in range(16):
+        i, j = divmod(n, 4)
+        I.append(m.t[i] - 1)
+        J.append(m.t[j] - 1)
+        V.append(Ak[n] * pr.mu)
+    S = sp.coo_matrix((np.concatenate(V), (np.concatenate(I), np.concatenate(J))), shape=(m.nv, m.nv)).tocsc()
+    S.sort_indices()
+    assert np.array_equal(S.indptr + 1, A.colptr)
+    assert np.array_equal(S.indices + 1, A.rowval)
+    np.testing.assert_allclose(A.nzval, S.data, rtol=1e-14, atol=1e-14 * np.abs(S.data).max())
+
+
+def test_unjittered_stencil_and_zero_rowsums(oracle):
+    n = 4
+    h = 2.0 / n
+    m = kuhn_box_mesh(n, 0.0)
+    S = oracle.stiffnessMatrix(m).to_scipy().tocsr()
+    np.testing.assert_allclose(S @ np.ones(m.nv), 0.0, atol=1e-14)
+    # interior node: 7-point Laplacian times h, face and body diagonals stored as exact zeros
+    i = j = k = 2
+    r = i + (n + 1) * (j + (n + 1) * k)
+    row = S.getrow(r)
+    vals = dict(zip(row.indices, row.data))
+    assert len(vals) == 15
+    assert vals[r] == pytest.approx(6 * h, rel=1e-14)
+    for d in (1, n + 1, (n + 1) ** 2):
+        assert vals[r + d] == pytest.approx(-h, rel=1e-14)
+        assert vals[r - d] == pytest.approx(-h, rel=1e-14)
+    others = [v for c, v in vals.items() if abs(c - r) not in (0, 1, n + 1, (n + 1) ** 2)]
+    assert len(others) == 8 and max(abs(v) for v in others) < 1e-15
+
+
+def test_config_mesh_exact_zeros_only_between_fixed_nodes(oracle):
+    # Femeko's incremental A += sparse(...) (src/FEM3D.jl:47-53) drops entries
+    # whose sum is exactly 0.0; the single-call convention kept here
+    # (NewtonRaphson.jl:161) stores them. On the jittered workload mesh the
+    # only exact zeros are face diagonals lying in the flat outer faces, i.e.
+    # entries between two Dirichlet nodes that the solve never reads.
+    pr = W.linear(20)
+    A = oracle.stiffnessMatrix(pr.mesh, pr.mu)
+    col = np.repeat(np.arange(A.n), np.diff(A.colptr))
+    row = A.rowval - 1
+    z = A.nzval == 0.0
+    assert np.count_nonzero(z) == 468
+    assert np.all(pr.mask[row[z]] == 1) and np.all(pr.mask[col[z]] == 1)
+
+
+# ---------------------------------------------------------------------------
+# A7/A8/A9: Dirichlet solve, fields, energy
+def test_patch_test_linear_field(oracle):
+    m = kuhn_box_mesh(6, 0.2, seed=6)
+    H0 = np.array([0.3, -1.2, 0.7])
+    mask = boundary_node_mask(m)
+    g = -(H0 @ m.p)
+    A = oracle.stiffnessMatrix(m, np.full(m.nt, 2.5))
+    phi, it, rel, s = oracle.pcg_dirichlet(A, mask, np.where(mask == 1, g, 0.0), 1e-14)
+    assert s == 0
+    np.testing.assert_allclose(phi, g, atol=1e-12)
+    H, Hn = oracle.field(m, phi)
+    np.testing.assert_allclose(H, np.broadcast_to(H0[:, None], H.shape), atol=1e-10)
+
+
+def test_layered_patch_test(oracle):
+    # mu depends on x through slabs bounded by grid planes; x is not jittered,
+    # so the piecewise-linear exact potential with mu_i phi'_i = const is in
+    # the P1 space and the discrete solution reproduces it
+    n = 8
+    m = kuhn_box_mesh(n, 0.25, seed=7)
+    h = 2.0 / n
+    m.p[0] = -1.0 + h * np.round((m.p[0] + 1.0) / h)
+    from femeko_hip.mesh import centroids
+    cx = centroids(m)[0]
+    edges = np.array([-1.0, -0.5, 0.25, 1.0])
+    mus = np.array([1.0, 40.0, 3.0])
+    slab = np.searchsorted(edges, cx) - 1
+    mu = mus[slab]
+    # exact potential: flux q = -mu phi' constant, phi(-1) = 0, phi(1) = -2
+    L = np.diff(edges)
+    q = 2.0 / np.sum(L / mus)
+    breaks = np.concatenate([[0.0], np.cumsum(-q * L / mus)])
+
+    def exact(x):
+        s = np.clip(np.searchsorted(edges, x, side="right") - 1, 0, 2)
+        return breaks[s] - q / mus[s] * (x - edges[s])
+
+    m.VE = element_volumes(m.p, m.t)
+    A = oracle.stiffnessMatrix(m, mu)
+    mask = boundary_node_mask(m)
+    ex = exact(m.p[0])
+    phi, it, rel, s = oracle.pcg_dirichlet(A, mask, np.where(mask == 1, ex, 0.0), 1e-14)
+    assert s == 0
+    np.testing.assert_allclose(phi, ex, atol=1e-11)
+
+
+def test_pcg_matches_direct_solve(oracle):
+    pr = W.linear(20)
+    A = oracle.stiffnessMatrix(pr.mesh, pr.mu)
+    S = A.to_scipy().tocsr()
+    free = pr.mask == 0
+    b = -S[free][:, ~free] @ pr.g[~free]
+    ref = pr.g.copy()
+    ref[free] = sla.spsolve(S[free][:, free].tocsc(), b)
+    phi, it, rel, s = oracle.pcg_dirichlet(A, pr.mask, pr.g, 1e-14)
+    assert s == 0
+    assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) < 1e-12
+    phi10, it10, rel10, s10 = oracle.pcg_dirichlet(A, pr.mask, pr.g, 1e-10)
+    assert s10 == 0 and rel10 < 1e-10
+    assert np.linalg.norm(phi10 - ref) / np.linalg.norm(ref) < 1e-9
+
+
+def test_pcg_iteration_count_matches_scipy_cg(oracle):
+    pr = W.linear(12)
+    A = oracle.stiffnessMatrix(pr.mesh, pr.mu)
+    S = A.to_scipy().tocsr()
+    free = pr.mask == 0
+    Aff = S[free][:, free]
+    b = -S[free][:, ~free] @ pr.g[~free]
+    d = Aff.diagonal()
+    M = sp.diags(1.0 / d)
+    count = [0]
+
+    def cb(_x):
+        count[0] += 1
+
+    _x, info = sla.cg(Aff, b, rtol=1e-10, atol=0.0, M=M, callback=cb, maxiter=10000)
+    assert info == 0
+    phi, it, rel, s = oracle.pcg_dirichlet(A, pr.mask, pr.g, 1e-10)
+    assert abs(it - count[0]) <= 2
+
+
+def test_energy_identity(oracle):
+    # 1/2 phi' A phi == 1/2 sum mu |H|^2 VE for any nodal phi: assembly vs field
+    pr = W.linear(8)
+    A = oracle.stiffnessMatrix(pr.mesh, pr.mu).to_scipy()
+    rng = np.random.default_rng(8)
+    phi = rng.normal(size=pr.mesh.nv)
+    H, Hn = oracle.field(pr.mesh, phi)
+    e1 = 0.5 * phi @ (A @ phi)
+    e2 = oracle.energy(pr.mesh, Hn, pr.mu)
+    assert e1 == pytest.approx(e2, rel=1e-12)
+
+
+def test_lagrange_and_boundary_integral(oracle):
+    m = boundary_faces(kuhn_box_mesh(4, 0.2, seed=9))
+    C = oracle.lagrange(m)
+    assert C.sum() == pytest.approx(m.VE.sum(), rel=1e-14)
+    # closed surface: sum_s n_s A_s = 0, so the flux load sums to zero
+    F = np.array([1.0, 2.0, -0.5])
+    R = oracle.BoundaryIntegral(m, F, m.shell_id)
+    assert abs(R.sum()) < 1e-13
+    # only the x = +1 face carries F = e_x flux: total = area 4
+    Fx = np.array([1.0, 0, 0])
+    Rx = oracle.BoundaryIntegral(m, Fx, m.shell_id)
+    on_pos = m.p[0] == 1.0
+    assert Rx[on_pos].sum() == pytest.approx(4.0, rel=1e-13)
+
+
+def test_line_search(oracle):
+    pr = W.linear(6)
+    A = oracle.stiffnessMatrix(pr.mesh, pr.mu)
+    S = A.to_scipy().tocsr()
+    rng = np.random.default_rng(10)
+    RHS = rng.normal(size=pr.mesh.nv) * 1e-8
+    u = np.zeros(pr.mesh.nv)
+    du = np.linalg.solve(S.toarray() + np.eye(pr.mesh.nv) * 1e-3 * S.diagonal().max(), RHS)
+    r0 = np.linalg.norm(RHS)
+    res = oracle.lineSearch(u, 50.0 * du, A, RHS)
+    # a 50x overshoot is cut back until the residual drops below the old one
+    assert res < r0
+    assert np.linalg.norm(RHS - S @ u) == pytest.approx(res, rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# A6/A10: tangent and the mu update
+def test_froehlich_derivative_fd(oracle):
+    H = np.array([0.0, 10.0, 500.0, 3e4])
+    Ms, a = 1.2e6, 500.0
+    _mu, dmu = oracle.froehlich(H, Ms, a)
+    eps = 1e-3
+    fd = (oracle.froehlich(H + eps, Ms, a)[0] - oracle.froehlich(H - eps, Ms, a)[0]) / (2 * eps)
+    np.testing.assert_allclose(dmu, fd, rtol=1e-6)
+
+
+def test_tangent_directional_derivative(oracle):
+    m = kuhn_box_mesh(4, 0.2, seed=11)
+    Ms, a = 1.2e6, 500.0
+    rng = np.random.default_rng(11)
+    phi = rng.normal(size=m.nv) * 300.0
+    delta = rng.normal(size=m.nv)
+    Ak = oracle.localStiffnessMatrix(m)
+
+    def R(u):
+        _H, Hn = oracle.field(m, u)
+        mu, _d = oracle.froehlich(Hn, Ms, a)
+        return oracle.spmv(oracle.sparse_from_local(m, Ak, mu), u)
+
+    H, Hn = oracle.field(m, phi)
+    mu, dmu = oracle.froehlich(Hn, Ms, a)
+    A = oracle.sparse_from_local(m, Ak, mu)
+    At = oracle.tangentialStiffnessMatrix(m, H, dmu)
+    J = oracle.CSC(A.n, A.colptr, A.rowval, A.nzval + At.nzval)
+    eps = 1e-4
+    fd = (R(phi + eps * delta) - R(phi - eps * delta)) / (2 * eps)
+    Jd = oracle.spmv(J, delta)
+    assert np.linalg.norm(fd - Jd) / np.linalg.norm(Jd) < 1e-6
+    # without the tangent the derivative is wrong by far more
+    Ad = oracle.spmv(A, delta)
+    assert np.linalg.norm(fd - Ad) / np.linalg.norm(Jd) > 1e-3
+
+
+def test_newton_froehlich_fixed_point(oracle):
+    pr = W.froehlich(n=8, seed=12)
+    out = oracle.newton_froehlich(pr.mesh, pr.mask, pr.g, pr.magnet, pr.meta["Ms"], pr.meta["a"])
+    assert out["iters"] < 20
+    assert out["dmu_hist"][-1] < 1e-6
+    # at the fixed point mu == mu(|H|) and A(mu) phi == 0 on free rows
+    mu, _d = oracle.froehlich(out["Hn"], pr.meta["Ms"], pr.meta["a"])
+    mu = np.where(pr.magnet, mu, MU0)
+    np.testing.assert_allclose(out["mu"], mu, rtol=1e-5)
+    A = oracle.stiffnessMatrix(pr.mesh, out["mu"]).to_scipy()
+    r = (A @ out["phi"])[pr.mask == 0]
+    bref = (A @ pr.g)[pr.mask == 0]
+    assert np.linalg.norm(r) / np.linalg.norm(bref) < 1e-6
+    # Newton converges faster than the Picard iteration on the same problem
+    pic = oracle.newton_froehlich(pr.mesh, pr.mask, pr.g, pr.magnet, pr.meta["Ms"], pr.meta["a"], newton=False,
+                                  max_outer=40)
+    assert pic["iters"] > out["iters"]
+
+
+# ---------------------------------------------------------------------------
+# A14: adjoint
+def test_adjoint_gradient_central_differences(oracle):
+    pr = W.adjoint(n=6, seed=13)
+    m = pr.mesh
+    rho = pr.meta["rho"]
+    target = pr.magnet
+    Hs = pr.meta["Hstar"]
+    J, grad, _phi, _lam = oracle.adjoint_gradient(m, pr.mask, pr.g, rho, target, Hs)
+    rng = np.random.default_rng(13)
+    idx = rng.choice(m.nt, 8, replace=False)
+    h = 1e-6
+    for e in idx:
+        rp = rho.copy()
+        rp[e] += h
+        rm = rho.copy()
+        rm[e] -= h
+        fd = (oracle.objective(m, pr.mask, pr.g, rp, target, Hs) - oracle.objective(m, pr.mask, pr.g, rm, target, Hs)) / (2 * h)
+        assert fd == pytest.approx(grad[e], rel=2e-4, abs=1e-9 * np.abs(grad).max())
+
+
+def test_kuhn_counts():
+    for n in (1, 2, 7, 20):
+        m = kuhn_box_mesh(n)
+        c = kuhn_counts(n)
+        assert m.nt == c["nt"] and m.nv == c["nv"]
+        assert int(boundary_node_mask(m).sum()) == c["dirichlet"]
