@@ -146,6 +146,9 @@ struct fk_mesh {
     // coefficients and values
     fk::DevBuf<double> coef;          // per element (colour order)
     bool have_coef = false;
+    fk::DevBuf<double> dmu;           // d mu / d|H| per element (colour order), nonlinear loop
+    fk::DevBuf<uint8_t> region;       // nonlinear material region (colour order)
+    fk::DevBuf<uint8_t> target;       // objective region of the adjoint (colour order)
     fk::DevBuf<double> val64;
     fk::DevBuf<float> val32;
     bool have_values = false, have_val32 = false;
@@ -192,6 +195,11 @@ void solve_rhs(fk_mesh *m, const double *b, double rtol, int64_t max_iter, doubl
                double *relres);
 void field(fk_mesh *m, double *H, double *Hn, double *energy);
 void setup_partition(fk_mesh *m);
+void ensure_work_f64(fk_mesh *m);
+void compute_dinv_f64(fk_mesh *m);
+CgState run_pcg_f64(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref);
+void phi_axpy_f64(fk_mesh *m, const double *x, double s);
+void k_init_phi_dev(fk_mesh *m, int keep_free);
 
 // Run body; map library errors to a status and remember the message.
 template <typename F>

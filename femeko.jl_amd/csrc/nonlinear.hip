@@ -1,0 +1,341 @@
+// Nonlinear magnetostatics on the GPU: the Picard/Newton mu(|H|) loop of
+// Magnetostatics/NewtonRaphson.jl:148-212 with the Froehlich law
+// mu = mu0 (1 + Ms/(a + |H|)) in the magnet region, and lineSearch
+// (src/FEM3D.jl:123-152).
+//
+// Per outer iteration, everything stays on the device:
+//   k_assemble_newton (one launch per colour): gradients, H = -grad(phi) of
+//       the current potential, K_e(mu) plus (Newton step) the tangent
+//       VE mu'/|H| (H.g_i)(H.g_j) of src/FEM3D.jl:79-119 into the CSR values,
+//       and the element residual -K_e(mu) phi_e into r -- one pass, no
+//       separate SpMV for the residual;
+//   FP64 Jacobi-PCG for the update du (rtol relative to ||r||);
+//   phi += du;
+//   k_field_mu: H of the new potential, mu and mu' from the law (the
+//       "mu update fused into the element gradient kernel"), block maxima of
+//       |mu_new - mu| / mu_new over the magnet and a non-finite counter.
+// The host reads one max per iteration to decide convergence and whether the
+// next step is Newton (previous change below `switch_tol`) or Picard;
+// iteration 1 starts from mu = mu0 with no tangent, as the reference's
+// att == 1 branch.
+#include <algorithm>
+#include <cmath>
+
+#include "device.h"
+#include "internal.h"
+
+namespace fk {
+
+__global__ void k_fill_f64b(double *__restrict__ a, int64_t n, double v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_gather_u8(const uint8_t *__restrict__ src, const int32_t *__restrict__ perm, int64_t nt,
+                            uint8_t *__restrict__ dst) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < nt) dst[e] = src[perm[e]] ? 1 : 0;
+}
+
+__global__ void k_scatter_coef(const double *__restrict__ coef, const int32_t *__restrict__ perm, int64_t nt,
+                               double *__restrict__ out) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < nt) out[perm[e]] = coef[e];
+}
+
+__global__ __launch_bounds__(256) void k_assemble_newton(const int4 *__restrict__ tet, const double *__restrict__ X,
+                                                         const double *__restrict__ Y, const double *__restrict__ Z,
+                                                         const double *__restrict__ mu,
+                                                         const double *__restrict__ dmu,
+                                                         const double *__restrict__ phi,
+                                                         const int32_t *__restrict__ slot, double *__restrict__ val,
+                                                         double *__restrict__ r, int newton, int64_t e0, int64_t e1) {
+    const int64_t e = e0 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    const int4 nd = tet[e];
+    const Grad G = element_gradients(X, Y, Z, nd);
+    const double vol = fabs(G.det) * (1.0 / 6.0);
+    const double u[4] = {phi[nd.x], phi[nd.y], phi[nd.z], phi[nd.w]};
+    const double w = vol * mu[e];
+    double K[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j) {
+            K[i][j] = w * dot3(G.g[i], G.g[j]);
+            K[j][i] = K[i][j];
+        }
+    // element residual -K u (before adding the tangent)
+    double ru[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ru[i] = -(K[i][0] * u[0] + K[i][1] * u[1] + K[i][2] * u[2] + K[i][3] * u[3]);
+    if (newton) {
+        double h[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) h[c] = -(u[0] * G.g[0][c] + u[1] * G.g[1][c] + u[2] * G.g[2][c] + u[3] * G.g[3][c]);
+        const double hn = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+        const double wt = hn > 0.0 ? vol * dmu[e] / hn : 0.0;
+        double hg[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) hg[i] = dot3(h, G.g[i]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) K[i][j] += wt * hg[i] * hg[j];
+    }
+    const int4 *s4 = reinterpret_cast<const int4 *>(slot + 16 * e);
+    const int4 q0 = s4[0], q1 = s4[1], q2 = s4[2], q3 = s4[3];
+    const int s[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                       q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) val[s[4 * i + j]] += K[i][j];
+    r[nd.x] += ru[0];
+    r[nd.y] += ru[1];
+    r[nd.z] += ru[2];
+    r[nd.w] += ru[3];
+}
+
+// H of phi, Froehlich mu and mu' in the region, block max of the relative
+// change, count of non-finite results
+__global__ __launch_bounds__(kBlock) void k_field_mu(const int4 *__restrict__ tet, const double *__restrict__ X,
+                                                     const double *__restrict__ Y, const double *__restrict__ Z,
+                                                     const double *__restrict__ phi,
+                                                     const uint8_t *__restrict__ region, double mu0, double Ms,
+                                                     double a, int64_t nt, double *__restrict__ mu,
+                                                     double *__restrict__ dmu, double *__restrict__ part,
+                                                     int32_t *__restrict__ bad) {
+    __shared__ double sh[kBlock / 64 + 1];
+    double dev = 0.0;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; e < nt;
+         e += static_cast<int64_t>(gridDim.x) * kBlock) {
+        if (!region[e]) continue;
+        const int4 nd = tet[e];
+        const Grad G = element_gradients(X, Y, Z, nd);
+        const double u0 = phi[nd.x], u1 = phi[nd.y], u2 = phi[nd.z], u3 = phi[nd.w];
+        double hh = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double h = -(u0 * G.g[0][c] + u1 * G.g[1][c] + u2 * G.g[2][c] + u3 * G.g[3][c]);
+            hh += h * h;
+        }
+        const double d = a + sqrt(hh);
+        const double mnew = mu0 * (1.0 + Ms / d);
+        const double dnew = -mu0 * Ms / (d * d);
+        if (!isfinite(mnew) || !isfinite(dnew)) atomicAdd(bad, 1);
+        dev = fmax(dev, fabs(mnew - mu[e]) / mnew);
+        mu[e] = mnew;
+        dmu[e] = dnew;
+    }
+    const double m = block_max<kBlock>(dev, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// y = u + alpha du (trial of the line search)
+__global__ void k_axpy_to(const double *__restrict__ u, const double *__restrict__ du, double alpha,
+                          double *__restrict__ y, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = u[i] + alpha * du[i];
+}
+
+// partials of ||rhs - A x||^2 over all rows (the unmasked residual of lineSearch)
+__global__ __launch_bounds__(kBlock) void k_resid_norm(const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ col,
+                                                       const double *__restrict__ val, const double *__restrict__ x,
+                                                       const double *__restrict__ rhs, double *__restrict__ part,
+                                                       int64_t n) {
+    __shared__ double sh[kBlock / 64 + 1];
+    constexpr int L = 16, RPB = kBlock / L;
+    const int sub = threadIdx.x / L, lane = threadIdx.x % L;
+    double acc = 0.0;
+    for (int64_t rb = static_cast<int64_t>(blockIdx.x) * RPB; rb < n; rb += static_cast<int64_t>(gridDim.x) * RPB) {
+        const int64_t row = rb + sub;
+        double s = 0.0;
+        if (row < n) {
+            const int32_t a0 = rowptr[row], a1 = rowptr[row + 1];
+            for (int32_t k = a0 + lane; k < a1; k += L) s += val[k] * x[col[k]];
+        }
+#pragma unroll
+        for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (row < n && lane == 0) {
+            const double d = rhs[row] - s;
+            acc += d * d;
+        }
+    }
+    const double tot = block_sum<kBlock>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------------------
+static double host_max(const double *p, int n) {
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) m = std::max(m, p[i]);
+    return m;
+}
+
+static double host_sum(const double *p, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += p[i];
+    return s;
+}
+
+void nonlinear_froehlich(fk_mesh *m, const uint8_t *region_host, double Ms, double a, double rtol, double tol_mu,
+                         int max_outer, double switch_tol, int32_t *outer_out, int64_t *cg_out, double *dev_out) {
+    if (!m->have_dirichlet) fail(FK_ERR_STATE, "no Dirichlet data set");
+    if (m->nfixed == 0) fail(FK_ERR_SINGULAR, "no fixed node: the pure Neumann system is singular");
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = m->nv, nt = m->nt;
+    const double mu0 = 3.141592653589793 * 4e-7;
+    // region in colour order
+    {
+        DevBuf<uint8_t> tmp(nt);
+        m->region.ensure(nt);
+        FK_HIP(hipMemcpyAsync(tmp.get(), region_host, nt, hipMemcpyHostToDevice, st));
+        k_gather_u8<<<grid_for(nt, 256), 256, 0, st>>>(tmp.get(), m->perm.get(), nt, m->region.get());
+        FK_HIP(hipGetLastError());
+        FK_HIP(hipStreamSynchronize(st));
+    }
+    m->coef.ensure(nt);
+    m->dmu.ensure(nt);
+    m->val64.ensure(m->nnz);
+    m->phi.ensure(nv);
+    m->r64.ensure(nv);
+    ensure_work_f64(m);
+    k_fill_f64b<<<grid_for(nt, 256), 256, 0, st>>>(m->coef.get(), nt, mu0);
+    k_fill_f64b<<<grid_for(nt, 256), 256, 0, st>>>(m->dmu.get(), nt, 0.0);
+    k_init_phi_dev(m, 0);  // phi = g on fixed nodes, 0 elsewhere
+    m->have_coef = true;
+    m->have_phi = true;
+    const int nb = std::max(1, m->nblocks);
+    m->part_misc.ensure(nb);
+    DevBuf<int32_t> bad(1);
+    double dev = INFINITY;
+    int32_t it = 0;
+    int64_t cg = 0;
+    PcgWork<double> &w = m->w64;
+    for (it = 1; it <= max_outer; ++it) {
+        const int newton = dev < switch_tol ? 1 : 0;
+        k_fill_f64b<<<grid_for(m->nnz, 256), 256, 0, st>>>(m->val64.get(), m->nnz, 0.0);
+        k_fill_f64b<<<grid_for(nv, 256), 256, 0, st>>>(w.r.get(), nv, 0.0);
+        for (int c = 0; c < m->ncolours; ++c) {
+            const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
+            if (e1 <= e0) continue;
+            k_assemble_newton<<<grid_for(e1 - e0, 256), 256, 0, st>>>(
+                m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->coef.get(), m->dmu.get(), m->phi.get(),
+                m->slot.get(), m->val64.get(), w.r.get(), newton, e0, e1);
+        }
+        FK_HIP(hipGetLastError());
+        m->have_values = true;
+        m->have_val32 = false;
+        m->dinv64_valid = false;
+        m->dinv32_valid = false;
+        compute_dinv_f64(m);
+        m->dinv64_valid = true;
+        const CgState s = run_pcg_f64(m, rtol, 1000000, -1.0);
+        cg += s.iters;
+        if (s.status != FK_OK && s.status != FK_ERR_NOT_CONVERGED)
+            fail(s.status, std::string("inner solve: ") + fk_status_string(s.status));
+        phi_axpy_f64(m, w.x.get(), 1.0);
+        FK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), st));
+        k_field_mu<<<nb, kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->phi.get(),
+                                          m->region.get(), mu0, Ms, a, nt, m->coef.get(), m->dmu.get(),
+                                          m->part_misc.get(), bad.get());
+        FK_HIP(hipGetLastError());
+        int32_t hbad = 0;
+        FK_HIP(hipMemcpyAsync(m->hpart.ptr, m->part_misc.get(), nb * sizeof(double), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        if (hbad) fail(FK_ERR_NONFINITE, "Nans/Infs in mu");  // NewtonRaphson.jl:210-212
+        dev = host_max(m->hpart.ptr, nb);
+        if (dev < tol_mu) break;
+    }
+    // the values hold A + At of the last step: reassemble A(mu) for later use
+    m->have_values = false;
+    if (outer_out) *outer_out = std::min(it, max_outer);
+    if (cg_out) *cg_out = cg;
+    if (dev_out) *dev_out = dev;
+    if (!(dev < tol_mu)) fail(FK_ERR_NOT_CONVERGED, "mu update did not reach tol_mu");
+}
+
+double line_search(fk_mesh *m, double *u_host, const double *du_host, const double *rhs_host, double min_step) {
+    if (!m->have_values) fail(FK_ERR_STATE, "matrix not assembled");
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = m->nv;
+    DevBuf<double> u(nv), du(nv), rhs(nv), y(nv);
+    FK_HIP(hipMemcpyAsync(u.get(), u_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(du.get(), du_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(rhs.get(), rhs_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
+    const int nb = std::max(1, m->nblocks);
+    m->part_misc.ensure(nb);
+    auto norm_of = [&](const double *x) {
+        k_resid_norm<<<nb, kBlock, 0, st>>>(m->rowptr.get(), m->colidx.get(), m->val64.get(), x, rhs.get(),
+                                            m->part_misc.get(), nv);
+        FK_HIP(hipGetLastError());
+        FK_HIP(hipMemcpyAsync(m->hpart.ptr, m->part_misc.get(), nb * sizeof(double), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        return std::sqrt(host_sum(m->hpart.ptr, nb));
+    };
+    // src/FEM3D.jl:123-152
+    double alf = 1.0;
+    const double residual_old = norm_of(u.get());
+    k_axpy_to<<<grid_for(nv, 256), 256, 0, st>>>(u.get(), du.get(), 1.0, y.get(), nv);
+    double residue = norm_of(y.get());
+    while (residue > residual_old && alf > min_step) {
+        alf *= 0.9;
+        k_axpy_to<<<grid_for(nv, 256), 256, 0, st>>>(u.get(), du.get(), alf, y.get(), nv);
+        residue = norm_of(y.get());
+    }
+    if (!(alf < min_step)) {
+        FK_HIP(hipMemcpyAsync(u_host, y.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+    }
+    return residue;
+}
+
+void get_coefficient(fk_mesh *m, double *f_host) {
+    if (!m->have_coef) fail(FK_ERR_STATE, "no coefficient");
+    hipStream_t st = m->ctx->stream;
+    DevBuf<double> out(m->nt);
+    k_scatter_coef<<<grid_for(m->nt, 256), 256, 0, st>>>(m->coef.get(), m->perm.get(), m->nt, out.get());
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipMemcpyAsync(f_host, out.get(), m->nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace fk
+
+using fk::guard;
+
+extern "C" {
+
+int fk_nonlinear_froehlich(fk_mesh *m, const uint8_t *region, double Ms, double a, double rtol, double tol_mu,
+                           int32_t max_outer, double switch_tol, int32_t *outer, int64_t *cg_iterations,
+                           double *final_change) {
+    if (!m || !region || !(Ms >= 0.0) || !(a > 0.0) || !(rtol > 0.0) || !(tol_mu > 0.0) || max_outer < 1)
+        return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_nonlinear_froehlich", [&] {
+        fk::set_device(m->ctx);
+        fk::nonlinear_froehlich(m, region, Ms, a, rtol, tol_mu, max_outer, switch_tol, outer, cg_iterations,
+                                final_change);
+    });
+}
+
+int fk_line_search(fk_mesh *m, double *u, const double *du, const double *rhs, double min_step, double *residue) {
+    if (!m || !u || !du || !rhs) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_line_search", [&] {
+        fk::set_device(m->ctx);
+        const double r = fk::line_search(m, u, du, rhs, min_step);
+        if (residue) *residue = r;
+    });
+}
+
+int fk_get_coefficient(fk_mesh *m, double *f) {
+    if (!m || !f) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_get_coefficient", [&] {
+        fk::set_device(m->ctx);
+        fk::get_coefficient(m, f);
+    });
+}
+
+}  // extern "C"

@@ -511,6 +511,22 @@ static double neg_residual(fk_mesh *m, const double *phi, double *r) {
     return s;
 }
 
+// ---- entry points for the other translation units (nonlinear loop, adjoint)
+void ensure_work_f64(fk_mesh *m) { ensure_work<double>(m); }
+void compute_dinv_f64(fk_mesh *m) { compute_dinv<double>(m); }
+CgState run_pcg_f64(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref) {
+    return run_pcg<double>(m, rtol, max_iter, bb_ref);
+}
+void phi_axpy_f64(fk_mesh *m, const double *x, double s) {
+    k_phi_axpy<double><<<grid_for(m->nv, 256), 256, 0, m->ctx->stream>>>(m->phi.get(), x, s, m->nv);
+    FK_HIP(hipGetLastError());
+}
+void k_init_phi_dev(fk_mesh *m, int keep_free) {
+    k_init_phi<<<grid_for(m->nv, 256), 256, 0, m->ctx->stream>>>(m->phi.get(), m->g.get(), m->mask.get(), m->nv,
+                                                                 keep_free);
+    FK_HIP(hipGetLastError());
+}
+
 void set_dirichlet(fk_mesh *m, const uint8_t *mask, const double *g) {
     hipStream_t st = m->ctx->stream;
     int64_t nfixed = 0;

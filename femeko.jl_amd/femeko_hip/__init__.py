@@ -6,6 +6,6 @@ drop-ins, device for the handles, mesh/workloads for the synthetic meshes.
 """
 from ._lib import FemekoHIPError, LIB_PATH  # noqa: F401
 from .device import Context, DeviceMesh, SolveReport, context  # noqa: F401
-from .fem3d import (device_mesh, invalidate, lagrange, localStiffnessMatrix, magneticField,  # noqa: F401
-                    solveDirichlet, stiffnessMatrix, tangentialStiffnessMatrix)
+from .fem3d import (device_mesh, invalidate, lagrange, lineSearch, localStiffnessMatrix,  # noqa: F401
+                    magneticField, solveDirichlet, stiffnessMatrix, tangentialStiffnessMatrix)
 from .mesh import MESH, MU0, kuhn_box_mesh  # noqa: F401

@@ -73,6 +73,11 @@ SIGNATURES = {
     "fk_potential": (_c.c_int, [_vp, _dp]),
     "fk_set_potential": (_c.c_int, [_vp, _dp]),
     "fk_field": (_c.c_int, [_vp, _dp, _dp, _dp]),
+    "fk_nonlinear_froehlich": (_c.c_int, [_vp, _up, _c.c_double, _c.c_double, _c.c_double, _c.c_double,
+                                          _c.c_int32, _c.c_double, _ip, _i64p, _dp]),
+    "fk_line_search": (_c.c_int, [_vp, _dp, _dp, _dp, _c.c_double, _dp]),
+    "fk_get_coefficient": (_c.c_int, [_vp, _dp]),
+    "fk_adjoint_field_target": (_c.c_int, [_vp, _up, _dp, _c.c_double, _dp, _dp, _i64p]),
     "fk_set_timing": (_c.c_int, [_vp, _c.c_int32]),
     "fk_spmv_timing": (_c.c_int, [_vp, _i64p, _dp]),
     "fk_set_lanes_per_row": (_c.c_int, [_vp, _c.c_int32]),

@@ -217,6 +217,51 @@ class DeviceMesh:
                                         ctypes.byref(E) if want_energy else None), "fk_field")
         return (H.T if H is not None else None), Hn, (float(E.value) if want_energy else None)
 
+    def get_coefficient(self) -> np.ndarray:
+        f = np.empty(self.nt)
+        self.ctx.check(self._L.fk_get_coefficient(self.handle, dptr(f)), "fk_get_coefficient")
+        return f
+
+    def nonlinear_froehlich(self, region, Ms: float, a: float, rtol: float = 1e-10, tol_mu: float = 1e-6,
+                            max_outer: int = 20, switch_tol: float = 0.25, raise_on_error: bool = True):
+        """Picard/Newton mu(|H|) loop on the GPU. Returns dict(outer, cg_iterations, change, status)."""
+        reg = np.ascontiguousarray(region, dtype=np.uint8)
+        if reg.size != self.nt:
+            raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "region size")
+        outer = ctypes.c_int32(0)
+        cg = ctypes.c_int64(0)
+        ch = ctypes.c_double(0.0)
+        s = self._L.fk_nonlinear_froehlich(self.handle, uptr(reg), float(Ms), float(a), float(rtol), float(tol_mu),
+                                           int(max_outer), float(switch_tol), ctypes.byref(outer),
+                                           ctypes.byref(cg), ctypes.byref(ch))
+        if raise_on_error:
+            self.ctx.check(s, "fk_nonlinear_froehlich")
+        return {"outer": int(outer.value), "cg_iterations": int(cg.value), "change": float(ch.value), "status": s}
+
+    def adjoint_field_target(self, target, Hstar, rtol: float = 1e-10):
+        """(J, dJ/dmu, adjoint PCG iterations) for J = sum_target VE |H - H*|^2 at the resident potential."""
+        tg = np.ascontiguousarray(target, dtype=np.uint8)
+        if tg.size != self.nt:
+            raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "target size")
+        hs = _f64(Hstar, 3, "Hstar")
+        J = ctypes.c_double(0.0)
+        it = ctypes.c_int64(0)
+        grad = np.empty(self.nt)
+        self.ctx.check(self._L.fk_adjoint_field_target(self.handle, uptr(tg), dptr(hs), float(rtol), ctypes.byref(J),
+                                                       dptr(grad), ctypes.byref(it)), "fk_adjoint_field_target")
+        return float(J.value), grad, int(it.value)
+
+    def line_search(self, u, du, rhs, min_step: float = 1e-4) -> float:
+        """lineSearch(u, du, A, RHS, minStep): updates u in place, returns the residue."""
+        if not (isinstance(u, np.ndarray) and u.dtype == np.float64 and u.flags.c_contiguous and u.size == self.nv):
+            raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "u must be a contiguous float64 array of nv entries")
+        du = _f64(du, self.nv, "du")
+        rhs = _f64(rhs, self.nv, "rhs")
+        res = ctypes.c_double(0.0)
+        self.ctx.check(self._L.fk_line_search(self.handle, dptr(u), dptr(du), dptr(rhs), float(min_step),
+                                              ctypes.byref(res)), "fk_line_search")
+        return float(res.value)
+
     def set_lanes(self, lanes: int) -> None:
         self.ctx.check(self._L.fk_set_lanes_per_row(self.handle, int(lanes)), "fk_set_lanes_per_row")
 

@@ -40,10 +40,26 @@ def invalidate(mesh) -> None:
         mesh._fk_device = None
 
 
-def _csc(dm: DeviceMesh, values: np.ndarray) -> sp.csc_matrix:
+def _csc(dm: DeviceMesh, values: np.ndarray, resident: bool = False) -> sp.csc_matrix:
     rowptr, colidx = dm.pattern(base=0)
     # the pattern and the values are exactly symmetric: CSR arrays == CSC arrays
-    return sp.csc_matrix((values, colidx, rowptr), shape=(dm.nv, dm.nv))
+    A = sp.csc_matrix((values, colidx, rowptr), shape=(dm.nv, dm.nv))
+    if resident:
+        A.fk_mesh = dm  # the device holds these values: lineSearch runs there
+    return A
+
+
+def lineSearch(u, du, A, RHS, minStep: float = 1e-4) -> float:
+    """src/FEM3D.jl:123-152 on the GPU: alpha <- 0.9 alpha until
+    ||RHS - A(u + alpha du)|| <= ||RHS - A u||; mutates u. A is a matrix
+    returned by stiffnessMatrix (its values are resident on the GPU) or a
+    MESH whose last assembled matrix is meant."""
+    dm = getattr(A, "fk_mesh", None)
+    if dm is None and hasattr(A, "p") and hasattr(A, "t"):
+        dm = device_mesh(A)
+    if dm is None:
+        raise TypeError("lineSearch needs the matrix returned by stiffnessMatrix or a MESH")
+    return dm.line_search(u, du, RHS, minStep)
 
 
 def stiffnessMatrix(mesh, f=None) -> sp.csc_matrix:
@@ -51,7 +67,7 @@ def stiffnessMatrix(mesh, f=None) -> sp.csc_matrix:
     dm = device_mesh(mesh)
     dm.set_coefficient(np.ones(dm.nt) if f is None else f)
     dm.assemble()
-    return _csc(dm, dm.values())
+    return _csc(dm, dm.values(), resident=True)
 
 
 def localStiffnessMatrix(mesh, f=None) -> np.ndarray:

@@ -149,6 +149,41 @@ int fk_set_potential(fk_mesh *mesh, const double *phi);
  * NULL; src/magneticProperties.jl:268-273 with the element permeability). */
 int fk_field(fk_mesh *mesh, double *H, double *Hn, double *energy);
 
+/* ---- nonlinear loop and line search ----------------------------------------
+ * Picard/Newton mu(|H|) loop of Magnetostatics/NewtonRaphson.jl:148-212 with
+ * the Froehlich law mu = mu0 (1 + Ms/(a+|H|)) on elements with region != 0
+ * (caller order) and mu0 elsewhere, under the Dirichlet data already set.
+ * Starts from mu = mu0 (no tangent, as att == 1 there); an iteration adds the
+ * tangent of tangentialStiffnessMatrix (src/FEM3D.jl:79-119) once the last
+ * relative mu change is below switch_tol. Each FP64 Jacobi-PCG update solve
+ * runs to rtol relative to the current residual. Stops when
+ * max |mu_new - mu| / mu_new over the region < tol_mu (FK_ERR_NOT_CONVERGED
+ * after max_outer iterations). Non-finite mu returns FK_ERR_NONFINITE
+ * (NewtonRaphson.jl:210-212). The potential and mu stay resident
+ * (fk_potential, fk_get_coefficient, fk_field). */
+int fk_nonlinear_froehlich(fk_mesh *mesh, const uint8_t *region, double Ms, double a, double rtol,
+                           double tol_mu, int32_t max_outer, double switch_tol, int32_t *outer,
+                           int64_t *cg_iterations, double *final_change);
+/* lineSearch(u, du, A, RHS, minStep) (src/FEM3D.jl:123-152) with A the last
+ * assembled matrix (all rows): alpha <- 0.9 alpha until
+ * ||RHS - A(u + alpha du)|| <= ||RHS - A u||; u (nv) is updated in place
+ * unless alpha fell below min_step. residue: the last trial's norm. */
+int fk_line_search(fk_mesh *mesh, double *u, const double *du, const double *rhs, double min_step,
+                   double *residue);
+/* Current per-element coefficient (mu) in caller order. f: nt. */
+int fk_get_coefficient(fk_mesh *mesh, double *f);
+
+/* ---- adjoint ---------------------------------------------------------------
+ * No counterpart in the reference: sensitivity of the field-target objective
+ * J = sum_{e: target[e]} VE_e |H_e - Hstar|^2 (H from the resident potential
+ * of the last fk_solve) with respect to the per-element coefficient mu:
+ * adjoint solve A[free,free] lam = -(dJ/dphi)[free] (FP64 Jacobi-PCG to
+ * rtol), then dJ/dmu_e = VE_e (sum lam_i grad_i).(sum phi_j grad_j).
+ * target: nt (caller order), Hstar: 3, dJdmu: nt. The caller chains
+ * dmu/drho (a ChainRulesCore rrule in the Julia wrapper). */
+int fk_adjoint_field_target(fk_mesh *mesh, const uint8_t *target, const double *Hstar, double rtol,
+                            double *J, double *dJdmu, int64_t *iterations);
+
 /* ---- instrumentation -------------------------------------------------------
  * Average device time of the sparse matrix-vector kernel over the last solve
  * (HIP events on the context stream around each launch), when enabled with
