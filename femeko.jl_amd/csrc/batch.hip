@@ -1,0 +1,817 @@
+// Batched solves: many independent meshes, up to 4 right-hand sides each, in
+// one set of launches.
+//
+// A batch of meshes is one disconnected mesh (node numbers offset per mesh),
+// so upload, colouring, pattern, slot map and assembly are those of a single
+// fk_mesh: its stiffness matrix is block diagonal and one launch per colour
+// assembles every mesh of the batch. Only the solver differs: every
+// (mesh, right-hand side) pair is its own PCG system with its own scalars.
+//   - The persistent grid's logical blocks are cut at mesh boundaries
+//     (table blk -> mesh, row range), so each block's partial sums belong to
+//     one mesh and a mesh's blocks are contiguous; per-system alpha/beta are
+//     reductions over that range, in a fixed order.
+//   - The right-hand sides of a mesh share the matrix: vectors are stored node
+//     major with the nrhs values of a node together, one gather of the column
+//     serves all of them (SpMM), matrix traffic is paid once per nrhs solves.
+//   - FP32 inner PCG with FP64 residual refinement per system; systems that
+//     have converged are skipped, blocks whose systems are all done return at
+//     once; chunks of iterations run from a captured hipGraph.
+//   - Lanes per row and block size come from an autotuner that times the
+//     SpMM kernel on the batch itself.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "device.h"
+#include "internal.h"
+
+struct fk_batch {
+    fk_context *ctx = nullptr;
+    fk_mesh *mesh = nullptr;  // the concatenated mesh (owned)
+    int32_t nmesh = 0, nrhs = 1, nsys = 0;
+    std::vector<int64_t> node_off, elem_off;
+    // logical block table
+    int nblocks = 0;
+    fk::DevBuf<int32_t> blk_mesh;
+    fk::DevBuf<int64_t> blk_r0, blk_r1;
+    fk::DevBuf<int32_t> mesh_b0, mesh_b1;
+    std::vector<int32_t> h_mesh_b0, h_mesh_b1;
+    // Dirichlet values and potentials, node major [node][rhs]
+    fk::DevBuf<double> g, phi, r64;
+    bool have_dirichlet = false, have_phi = false;
+    // FP32 inner work, node major
+    fk::DevBuf<float> x, r, p, q;
+    // partials [logical block][rhs]
+    fk::DevBuf<double> part_pq, part_rz, part_rr, part_bb;
+    // per-system state
+    fk::DevBuf<int32_t> done, status, active, ndone;
+    fk::DevBuf<int64_t> iters;
+    fk::DevBuf<double> thresh, scale;
+    fk::HostBuf<double> hpart;
+    fk::HostBuf<int32_t> hcount;
+    hipGraphExec_t graph = nullptr;
+    int graph_cfg = -1;
+    int lanes = 8, block = 256;
+    ~fk_batch() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        delete mesh;
+    }
+};
+
+namespace fk {
+
+struct BatchArgs {
+    const int32_t *blk_mesh;
+    const int64_t *blk_r0, *blk_r1;
+    const int32_t *mesh_b0, *mesh_b1;
+    int nblocks, nrhs;
+};
+
+__device__ __forceinline__ int batch_logical_block(int nblocks) { return logical_block(nblocks); }
+
+__device__ __forceinline__ bool mesh_all_done(const int32_t *__restrict__ done, int mesh, int nrhs) {
+    bool all = true;
+    for (int k = 0; k < nrhs; ++k) all &= done[mesh * nrhs + k] != 0;
+    return all;
+}
+
+// sum over the logical blocks [b0, b1) of part[b*nrhs + k], fixed order
+template <int BLOCK>
+__device__ __forceinline__ double seg_sum(const double *__restrict__ part, int b0, int b1, int k, int nrhs,
+                                          double *sh) {
+    double s = 0.0;
+    for (int b = b0 + static_cast<int>(threadIdx.x); b < b1; b += BLOCK) s += part[b * nrhs + k];
+    return block_sum<BLOCK>(s, sh);
+}
+
+// K1: q = A p for all right-hand sides of the block's mesh, partials of p.q
+template <int BLOCK, int LPR>
+__global__ __launch_bounds__(BLOCK) void kb_spmm(BatchArgs ba, const int32_t *__restrict__ rowptr,
+                                                 const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                 const float *__restrict__ p, float *__restrict__ q,
+                                                 double *__restrict__ part, const int32_t *__restrict__ done) {
+    __shared__ double sh[BLOCK / 64 + 1];
+    const int lb = batch_logical_block(ba.nblocks);
+    const int mesh = ba.blk_mesh[lb];
+    const int nrhs = ba.nrhs;
+    if (mesh_all_done(done, mesh, nrhs)) return;
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    const int sub = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    constexpr int RPB = BLOCK / LPR;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t rb = r0; rb < r1; rb += RPB) {
+        const int64_t row = rb + sub;
+        const bool valid = row < r1;
+        float s[4] = {0.f, 0.f, 0.f, 0.f};
+        if (valid) {
+            const int32_t a = rowptr[row], b = rowptr[row + 1];
+            for (int32_t k = a + lane; k < b; k += LPR) {
+                const float v = val[k];
+                const float *pc = p + static_cast<int64_t>(col[k]) * nrhs;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j < nrhs) s[j] += v * pc[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1) s[j] += __shfl_xor(s[j], o, 64);
+        }
+        if (valid && lane == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (j < nrhs) {
+                    q[row * nrhs + j] = s[j];
+                    acc[j] += static_cast<double>(p[row * nrhs + j]) * static_cast<double>(s[j]);
+                }
+        }
+    }
+    for (int j = 0; j < nrhs; ++j) {
+        const double t = block_sum<BLOCK>(acc[j], sh);
+        if (threadIdx.x == 0) part[lb * nrhs + j] = t;
+    }
+}
+
+// K2: per system alpha, x += alpha p, r -= alpha q, partials of r.z, r.r
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__restrict__ x, float *__restrict__ r,
+                                                      const float *__restrict__ p, const float *__restrict__ q,
+                                                      const float *__restrict__ dinv,
+                                                      const double *__restrict__ part_pq,
+                                                      const double *__restrict__ rz_cur, double *__restrict__ rz_next,
+                                                      double *__restrict__ part_rr, const int32_t *__restrict__ done) {
+    __shared__ double sh[BLOCK / 64 + 1];
+    const int lb = batch_logical_block(ba.nblocks);
+    const int mesh = ba.blk_mesh[lb];
+    const int nrhs = ba.nrhs;
+    if (mesh_all_done(done, mesh, nrhs)) return;
+    const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
+    float alpha[4] = {0.f, 0.f, 0.f, 0.f};
+    bool act[4] = {false, false, false, false};
+    for (int j = 0; j < nrhs; ++j) {
+        const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, nrhs, sh);
+        const double pq = seg_sum<BLOCK>(part_pq, b0, b1, j, nrhs, sh);
+        act[j] = !done[mesh * nrhs + j] && pq > 0.0 && isfinite(pq);
+        alpha[j] = act[j] ? static_cast<float>(rho / pq) : 0.f;
+    }
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    double rz[4] = {0.0, 0.0, 0.0, 0.0}, rr[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        const float d = dinv[i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= nrhs) continue;
+            const int64_t o = i * nrhs + j;
+            float ri = r[o];
+            if (act[j]) {
+                x[o] += alpha[j] * p[o];
+                ri = d != 0.f ? ri - alpha[j] * q[o] : 0.f;
+                r[o] = ri;
+            }
+            const float z = d * ri;
+            rz[j] += static_cast<double>(ri) * static_cast<double>(z);
+            rr[j] += static_cast<double>(ri) * static_cast<double>(ri);
+        }
+    }
+    for (int j = 0; j < nrhs; ++j) {
+        const double a = block_sum<BLOCK>(rz[j], sh);
+        const double b = block_sum<BLOCK>(rr[j], sh);
+        if (threadIdx.x == 0) {
+            rz_next[lb * nrhs + j] = a;
+            part_rr[lb * nrhs + j] = b;
+        }
+    }
+}
+
+// K3: per system convergence test (first block of the mesh), p = D^-1 r + beta p
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__restrict__ p, const float *__restrict__ r,
+                                                     const float *__restrict__ dinv,
+                                                     const double *__restrict__ rz_next,
+                                                     const double *__restrict__ rz_cur,
+                                                     const double *__restrict__ part_rr, int32_t *__restrict__ done,
+                                                     int32_t *__restrict__ status, int64_t *__restrict__ iters,
+                                                     const double *__restrict__ thresh, int64_t max_iter,
+                                                     int32_t *__restrict__ ndone) {
+    __shared__ double sh[BLOCK / 64 + 1];
+    const int lb = batch_logical_block(ba.nblocks);
+    const int mesh = ba.blk_mesh[lb];
+    const int nrhs = ba.nrhs;
+    if (mesh_all_done(done, mesh, nrhs)) return;
+    const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
+    float beta[4] = {0.f, 0.f, 0.f, 0.f};
+    bool act[4] = {false, false, false, false};
+    for (int j = 0; j < nrhs; ++j) {
+        const int s = mesh * nrhs + j;
+        act[j] = !done[s];
+        const double rho_new = seg_sum<BLOCK>(rz_next, b0, b1, j, nrhs, sh);
+        const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, nrhs, sh);
+        beta[j] = act[j] ? static_cast<float>(rho_new / rho) : 0.f;
+        if (lb == b0) {
+            const double rr = seg_sum<BLOCK>(part_rr, b0, b1, j, nrhs, sh);
+            if (threadIdx.x == 0 && act[j]) {
+                const int64_t it = iters[s] + 1;
+                iters[s] = it;
+                int fin = 0;
+                if (rr < thresh[s]) {
+                    fin = 1;
+                } else if (!isfinite(rr)) {
+                    status[s] = FK_ERR_NONFINITE;
+                    fin = 1;
+                } else if (it >= max_iter) {
+                    status[s] = FK_ERR_NOT_CONVERGED;
+                    fin = 1;
+                }
+                if (fin) {
+                    done[s] = 1;
+                    atomicAdd(ndone, 1);
+                }
+            }
+        }
+    }
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        const float d = dinv[i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= nrhs || !act[j]) continue;
+            const int64_t o = i * nrhs + j;
+            p[o] = d * r[o] + beta[j] * p[o];
+        }
+    }
+}
+
+// r64 = -mask(A64 phi) for every system, partials of |r|^2 per [block][rhs]
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kb_neg_residual(BatchArgs ba, const int32_t *__restrict__ rowptr,
+                                                         const int32_t *__restrict__ col,
+                                                         const double *__restrict__ val,
+                                                         const double *__restrict__ phi,
+                                                         const uint8_t *__restrict__ mask, double *__restrict__ r,
+                                                         double *__restrict__ part) {
+    __shared__ double sh[BLOCK / 64 + 1];
+    constexpr int LPR = 16, RPB = BLOCK / LPR;
+    const int lb = batch_logical_block(ba.nblocks);
+    const int nrhs = ba.nrhs;
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    const int sub = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t rb = r0; rb < r1; rb += RPB) {
+        const int64_t row = rb + sub;
+        const bool valid = row < r1;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        if (valid) {
+            const int32_t a = rowptr[row], b = rowptr[row + 1];
+            for (int32_t k = a + lane; k < b; k += LPR) {
+                const double v = val[k];
+                const double *pc = phi + static_cast<int64_t>(col[k]) * nrhs;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j < nrhs) s[j] += v * pc[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1) s[j] += __shfl_xor(s[j], o, 64);
+        }
+        if (valid && lane == 0) {
+            const bool fixed = mask[row] != 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (j < nrhs) {
+                    const double v = fixed ? 0.0 : -s[j];
+                    r[row * nrhs + j] = v;
+                    acc[j] += v * v;
+                }
+        }
+    }
+    for (int j = 0; j < nrhs; ++j) {
+        const double t = block_sum<BLOCK>(acc[j], sh);
+        if (threadIdx.x == 0) part[lb * nrhs + j] = t;
+    }
+}
+
+// inner init: r32 = r64 / scale (active systems), x = 0, p = D^-1 r; partials
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kb_init(BatchArgs ba, const double *__restrict__ r64,
+                                                 const double *__restrict__ scale, const int32_t *__restrict__ active,
+                                                 const float *__restrict__ dinv, float *__restrict__ x,
+                                                 float *__restrict__ r, float *__restrict__ p,
+                                                 double *__restrict__ part_rz, double *__restrict__ part_bb) {
+    __shared__ double sh[BLOCK / 64 + 1];
+    const int lb = batch_logical_block(ba.nblocks);
+    const int mesh = ba.blk_mesh[lb];
+    const int nrhs = ba.nrhs;
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    double rz[4] = {0.0, 0.0, 0.0, 0.0}, bb[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        const float d = dinv[i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= nrhs) continue;
+            const int s = mesh * nrhs + j;
+            const int64_t o = i * nrhs + j;
+            const float ri = (active[s] && d != 0.f) ? static_cast<float>(r64[o] / scale[s]) : 0.f;
+            r[o] = ri;
+            x[o] = 0.f;
+            const float z = d * ri;
+            p[o] = z;
+            rz[j] += static_cast<double>(ri) * static_cast<double>(z);
+            bb[j] += static_cast<double>(ri) * static_cast<double>(ri);
+        }
+    }
+    for (int j = 0; j < nrhs; ++j) {
+        const double a = block_sum<BLOCK>(rz[j], sh);
+        const double b = block_sum<BLOCK>(bb[j], sh);
+        if (threadIdx.x == 0) {
+            part_rz[lb * nrhs + j] = a;
+            part_bb[lb * nrhs + j] = b;
+        }
+    }
+}
+
+// one thread per system: thresholds and initial done flags
+__global__ void kb_start(BatchArgs ba, const double *__restrict__ part_bb, const int32_t *__restrict__ active,
+                         double rtol, int32_t *__restrict__ done, int32_t *__restrict__ status,
+                         int64_t *__restrict__ iters, double *__restrict__ thresh, int32_t *__restrict__ ndone,
+                         int nsys) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    const int mesh = s / ba.nrhs, j = s % ba.nrhs;
+    double bb = 0.0;
+    for (int b = ba.mesh_b0[mesh]; b < ba.mesh_b1[mesh]; ++b) bb += part_bb[b * ba.nrhs + j];
+    thresh[s] = rtol * rtol * bb;
+    iters[s] = 0;
+    status[s] = FK_OK;
+    const int d = (!active[s] || bb == 0.0) ? 1 : 0;
+    done[s] = d;
+    if (d) atomicAdd(ndone, 1);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kb_phi_update(BatchArgs ba, double *__restrict__ phi,
+                                                       const float *__restrict__ x, const double *__restrict__ scale,
+                                                       const int32_t *__restrict__ active) {
+    const int lb = batch_logical_block(ba.nblocks);
+    const int mesh = ba.blk_mesh[lb];
+    const int nrhs = ba.nrhs;
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK)
+        for (int j = 0; j < nrhs; ++j) {
+            const int s = mesh * nrhs + j;
+            if (active[s]) phi[i * nrhs + j] += scale[s] * static_cast<double>(x[i * nrhs + j]);
+        }
+}
+
+__global__ void kb_init_phi(double *__restrict__ phi, const double *__restrict__ g, const uint8_t *__restrict__ mask,
+                            int64_t nv, int nrhs) {
+    const int64_t o = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (o >= nv * nrhs) return;
+    phi[o] = mask[o / nrhs] ? g[o] : 0.0;
+}
+
+__global__ void kb_zero_i32(int32_t *__restrict__ a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+static BatchArgs args_of(fk_batch *b) {
+    return BatchArgs{b->blk_mesh.get(), b->blk_r0.get(), b->blk_r1.get(), b->mesh_b0.get(), b->mesh_b1.get(),
+                     b->nblocks, b->nrhs};
+}
+
+// logical blocks cut at mesh boundaries, ~4 per CU overall
+static void build_block_table(fk_batch *b) {
+    const int64_t total = b->node_off.back();
+    const int64_t target_blocks = std::max<int64_t>(b->nmesh, 4 * static_cast<int64_t>(b->ctx->num_cus));
+    const int64_t chunk = std::max<int64_t>(256, (total + target_blocks - 1) / target_blocks);
+    std::vector<int32_t> bm;
+    std::vector<int64_t> r0, r1;
+    b->h_mesh_b0.assign(b->nmesh, 0);
+    b->h_mesh_b1.assign(b->nmesh, 0);
+    for (int m = 0; m < b->nmesh; ++m) {
+        const int64_t a = b->node_off[m], e = b->node_off[m + 1];
+        const int64_t nb = std::max<int64_t>(1, (e - a + chunk - 1) / chunk);
+        const int64_t c = (e - a + nb - 1) / nb;
+        b->h_mesh_b0[m] = static_cast<int32_t>(bm.size());
+        for (int64_t k = 0; k < nb; ++k) {
+            bm.push_back(m);
+            r0.push_back(std::min(e, a + k * c));
+            r1.push_back(std::min(e, a + (k + 1) * c));
+        }
+        b->h_mesh_b1[m] = static_cast<int32_t>(bm.size());
+    }
+    b->nblocks = static_cast<int>(bm.size());
+    b->blk_mesh.alloc(bm.size());
+    b->blk_r0.alloc(bm.size());
+    b->blk_r1.alloc(bm.size());
+    b->mesh_b0.alloc(b->nmesh);
+    b->mesh_b1.alloc(b->nmesh);
+    FK_HIP(hipMemcpy(b->blk_mesh.get(), bm.data(), bm.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    FK_HIP(hipMemcpy(b->blk_r0.get(), r0.data(), r0.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    FK_HIP(hipMemcpy(b->blk_r1.get(), r1.data(), r1.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    FK_HIP(hipMemcpy(b->mesh_b0.get(), b->h_mesh_b0.data(), b->nmesh * sizeof(int32_t), hipMemcpyHostToDevice));
+    FK_HIP(hipMemcpy(b->mesh_b1.get(), b->h_mesh_b1.data(), b->nmesh * sizeof(int32_t), hipMemcpyHostToDevice));
+}
+
+template <int BLOCK, int LPR>
+static void launch_spmm(fk_batch *b, int parity_unused) {
+    (void)parity_unused;
+    fk_mesh *m = b->mesh;
+    kb_spmm<BLOCK, LPR><<<b->nblocks, BLOCK, 0, b->ctx->stream>>>(args_of(b), m->rowptr.get(), m->colidx.get(),
+                                                                 m->val32.get(), b->p.get(), b->q.get(),
+                                                                 b->part_pq.get(), b->done.get());
+}
+
+template <int BLOCK>
+static void launch_spmm_lanes(fk_batch *b, int lanes) {
+    switch (lanes) {
+        case 4: launch_spmm<BLOCK, 4>(b, 0); break;
+        case 16: launch_spmm<BLOCK, 16>(b, 0); break;
+        default: launch_spmm<BLOCK, 8>(b, 0); break;
+    }
+}
+
+static void spmm(fk_batch *b, int block, int lanes) {
+    switch (block) {
+        case 128: launch_spmm_lanes<128>(b, lanes); break;
+        case 512: launch_spmm_lanes<512>(b, lanes); break;
+        default: launch_spmm_lanes<256>(b, lanes); break;
+    }
+}
+
+template <int BLOCK>
+static void iteration_tail(fk_batch *b, int parity, int64_t max_iter) {
+    hipStream_t st = b->ctx->stream;
+    const int nb = b->nblocks, nr = b->nrhs;
+    double *rz_cur = b->part_rz.get() + (parity ? nb * nr : 0);
+    double *rz_next = b->part_rz.get() + (parity ? 0 : nb * nr);
+    fk_mesh *m = b->mesh;
+    kb_update_xr<BLOCK><<<nb, BLOCK, 0, st>>>(args_of(b), b->x.get(), b->r.get(), b->p.get(), b->q.get(),
+                                              m->w32.dinv.get(), b->part_pq.get(), rz_cur, rz_next,
+                                              b->part_rr.get(), b->done.get());
+    kb_update_p<BLOCK><<<nb, BLOCK, 0, st>>>(args_of(b), b->p.get(), b->r.get(), m->w32.dinv.get(), rz_next, rz_cur,
+                                             b->part_rr.get(), b->done.get(), b->status.get(), b->iters.get(),
+                                             b->thresh.get(), max_iter, b->ndone.get());
+}
+
+static void enqueue_batch_iteration(fk_batch *b, int parity, int64_t max_iter) {
+    spmm(b, b->block, b->lanes);
+    switch (b->block) {
+        case 128: iteration_tail<128>(b, parity, max_iter); break;
+        case 512: iteration_tail<512>(b, parity, max_iter); break;
+        default: iteration_tail<256>(b, parity, max_iter); break;
+    }
+}
+
+static void neg_residual_batch(fk_batch *b, const double *phi) {
+    fk_mesh *m = b->mesh;
+    kb_neg_residual<256><<<b->nblocks, 256, 0, b->ctx->stream>>>(args_of(b), m->rowptr.get(), m->colidx.get(),
+                                                                m->val64.get(), phi, m->mask.get(), b->r64.get(),
+                                                                b->part_rr.get());
+    FK_HIP(hipGetLastError());
+}
+
+// per-system sums of part_rr (host)
+static void system_norms(fk_batch *b, std::vector<double> &out) {
+    hipStream_t st = b->ctx->stream;
+    const size_t n = static_cast<size_t>(b->nblocks) * b->nrhs;
+    FK_HIP(hipMemcpyAsync(b->hpart.ptr, b->part_rr.get(), n * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    out.assign(b->nsys, 0.0);
+    for (int m = 0; m < b->nmesh; ++m)
+        for (int j = 0; j < b->nrhs; ++j) {
+            double s = 0.0;
+            for (int k = b->h_mesh_b0[m]; k < b->h_mesh_b1[m]; ++k) s += b->hpart.ptr[k * b->nrhs + j];
+            out[m * b->nrhs + j] = s;
+        }
+}
+
+static void inner_solve(fk_batch *b, double inner_rtol, int64_t max_iter) {
+    hipStream_t st = b->ctx->stream;
+    fk_mesh *m = b->mesh;
+    const int nb = b->nblocks;
+    kb_zero_i32<<<1, 64, 0, st>>>(b->ndone.get(), 1);
+    kb_init<256><<<nb, 256, 0, st>>>(args_of(b), b->r64.get(), b->scale.get(), b->active.get(), m->w32.dinv.get(),
+                                     b->x.get(), b->r.get(), b->p.get(), b->part_rz.get(), b->part_bb.get());
+    kb_start<<<grid_for(b->nsys, 64), 64, 0, st>>>(args_of(b), b->part_bb.get(), b->active.get(), inner_rtol,
+                                                   b->done.get(), b->status.get(), b->iters.get(), b->thresh.get(),
+                                                   b->ndone.get(), b->nsys);
+    FK_HIP(hipGetLastError());
+    const int G = 16;
+    const int cfg = b->block * 1000 + b->lanes;
+    if (!b->graph || b->graph_cfg != cfg) {
+        if (b->graph) (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        hipGraph_t graph;
+        FK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        // max_iter lives in the kernel arguments: capture with a large cap and
+        // enforce the caller's limit from the host between chunks
+        for (int k = 0; k < G; ++k) enqueue_batch_iteration(b, k & 1, int64_t(1) << 40);
+        FK_HIP(hipStreamEndCapture(st, &graph));
+        hipError_t e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        FK_HIP(e);
+        b->graph_cfg = cfg;
+    }
+    int64_t launched = 0;
+    while (true) {
+        FK_HIP(hipGraphLaunch(b->graph, st));
+        launched += G;
+        FK_HIP(hipMemcpyAsync(b->hcount.ptr, b->ndone.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        if (b->hcount.ptr[0] >= b->nsys || launched >= max_iter) break;
+    }
+}
+
+void batch_solve(fk_batch *b, double rtol, double inner_rtol, int max_outer, int64_t max_iter, int64_t *iters_out,
+                 int32_t *outer_out, double *relres_out) {
+    fk_mesh *m = b->mesh;
+    if (!m->have_values) fail(FK_ERR_STATE, "batch not assembled");
+    if (!b->have_dirichlet) fail(FK_ERR_STATE, "no Dirichlet data set");
+    hipStream_t st = b->ctx->stream;
+    const int64_t nv = m->nv;
+    const int nr = b->nrhs, S = b->nsys;
+    make_val32(m);
+    if (!m->dinv32_valid) {
+        // dinv of the FP32 values, masked by the batch's fixed nodes
+        m->w32.dinv.ensure(nv);
+        compute_dinv_f32(m);
+        m->dinv32_valid = true;
+    }
+    b->phi.ensure(nv * nr);
+    b->r64.ensure(nv * nr);
+    kb_init_phi<<<grid_for(nv * nr, 256), 256, 0, st>>>(b->phi.get(), b->g.get(), m->mask.get(), nv, nr);
+    b->have_phi = true;
+    std::vector<double> bb, rr;
+    neg_residual_batch(b, b->g.get());  // b = -A[free,fixed] g per system
+    system_norms(b, bb);
+    neg_residual_batch(b, b->phi.get());
+    system_norms(b, rr);
+    std::vector<int64_t> total(S, 0), its(S);
+    std::vector<int32_t> outer(S, 0), act(S), stat(S, FK_OK);
+    std::vector<double> scale(S, 1.0);
+    for (int step = 0; step < max_outer; ++step) {
+        int nact = 0;
+        for (int s = 0; s < S; ++s) {
+            act[s] = (bb[s] > 0.0 && !(rr[s] < rtol * rtol * bb[s]) && stat[s] == FK_OK) ? 1 : 0;
+            scale[s] = act[s] ? std::sqrt(rr[s]) : 1.0;
+            nact += act[s];
+        }
+        if (nact == 0) break;
+        FK_HIP(hipMemcpyAsync(b->active.get(), act.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(b->scale.get(), scale.data(), S * sizeof(double), hipMemcpyHostToDevice, st));
+        inner_solve(b, inner_rtol, max_iter);
+        kb_phi_update<256><<<b->nblocks, 256, 0, st>>>(args_of(b), b->phi.get(), b->x.get(), b->scale.get(),
+                                                       b->active.get());
+        FK_HIP(hipGetLastError());
+        std::vector<int32_t> st_dev(S);
+        FK_HIP(hipMemcpyAsync(its.data(), b->iters.get(), S * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipMemcpyAsync(st_dev.data(), b->status.get(), S * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        neg_residual_batch(b, b->phi.get());
+        system_norms(b, rr);
+        for (int s = 0; s < S; ++s)
+            if (act[s]) {
+                total[s] += its[s];
+                outer[s] += 1;
+                if (st_dev[s] == FK_ERR_NONFINITE) stat[s] = FK_ERR_NONFINITE;
+            }
+    }
+    int status = FK_OK;
+    for (int s = 0; s < S; ++s) {
+        const double rel = bb[s] > 0.0 ? std::sqrt(rr[s] / bb[s]) : 0.0;
+        if (iters_out) iters_out[s] = total[s];
+        if (outer_out) outer_out[s] = outer[s];
+        if (relres_out) relres_out[s] = rel;
+        if (stat[s] != FK_OK) status = stat[s];
+        else if (bb[s] > 0.0 && !(rr[s] < rtol * rtol * bb[s]) && status == FK_OK) status = FK_ERR_NOT_CONVERGED;
+    }
+    if (status != FK_OK) fail(status, std::string("batch solve: ") + fk_status_string(status));
+}
+
+// time the SpMM kernel for every (block, lanes) candidate on the batch itself
+void batch_autotune(fk_batch *b, int32_t *lanes_out, int32_t *block_out, double *best_us_out) {
+    fk_mesh *m = b->mesh;
+    if (!m->have_values) fail(FK_ERR_STATE, "batch not assembled");
+    make_val32(m);
+    hipStream_t st = b->ctx->stream;
+    const int64_t nvr = m->nv * b->nrhs;
+    // all systems active, p = 1 so the kernel does full work
+    std::vector<int32_t> zero(b->nsys, 0);
+    FK_HIP(hipMemcpyAsync(b->done.get(), zero.data(), b->nsys * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    std::vector<float> ones(static_cast<size_t>(nvr), 1.0f);
+    FK_HIP(hipMemcpyAsync(b->p.get(), ones.data(), nvr * sizeof(float), hipMemcpyHostToDevice, st));
+    FK_HIP(hipStreamSynchronize(st));
+    hipEvent_t e0, e1;
+    FK_HIP(hipEventCreate(&e0));
+    FK_HIP(hipEventCreate(&e1));
+    double best = 1e30;
+    int bl = 8, bb = 256;
+    const int blocks[3] = {128, 256, 512};
+    const int lanes[3] = {4, 8, 16};
+    for (int bi = 0; bi < 3; ++bi)
+        for (int li = 0; li < 3; ++li) {
+            spmm(b, blocks[bi], lanes[li]);  // warm
+            FK_HIP(hipEventRecord(e0, st));
+            const int reps = 10;
+            for (int k = 0; k < reps; ++k) spmm(b, blocks[bi], lanes[li]);
+            FK_HIP(hipEventRecord(e1, st));
+            FK_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const double us = ms * 1e3 / reps;
+            if (us < best) {
+                best = us;
+                bl = lanes[li];
+                bb = blocks[bi];
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->lanes = bl;
+    b->block = bb;
+    if (lanes_out) *lanes_out = bl;
+    if (block_out) *block_out = bb;
+    if (best_us_out) *best_us_out = best;
+}
+
+}  // namespace fk
+
+// ---------------------------------------------------------------------------
+using fk::guard;
+
+extern "C" {
+
+int fk_batch_create(fk_context *ctx, int32_t nmesh, const int64_t *nv, const int64_t *nt, const double *const *p,
+                    const int32_t *const *t, int32_t index_base, int32_t nrhs, fk_batch **out) {
+    if (!ctx || !out || nmesh < 1 || !nv || !nt || !p || !t || nrhs < 1 || nrhs > 4 ||
+        (index_base != 0 && index_base != 1))
+        return FK_ERR_INVALID_ARG;
+    *out = nullptr;
+    fk_batch *b = new (std::nothrow) fk_batch();
+    if (!b) return FK_ERR_OOM;
+    b->ctx = ctx;
+    b->nmesh = nmesh;
+    b->nrhs = nrhs;
+    b->nsys = nmesh * nrhs;
+    int s = guard(ctx, "fk_batch_create", [&] {
+        fk::set_device(ctx);
+        b->node_off.assign(nmesh + 1, 0);
+        b->elem_off.assign(nmesh + 1, 0);
+        for (int m = 0; m < nmesh; ++m) {
+            if (nv[m] < 4 || nt[m] < 1 || !p[m] || !t[m]) fk::fail(FK_ERR_INVALID_ARG, "bad mesh sizes or pointers");
+            b->node_off[m + 1] = b->node_off[m] + nv[m];
+            b->elem_off[m + 1] = b->elem_off[m] + nt[m];
+        }
+        const int64_t NV = b->node_off.back(), NT = b->elem_off.back();
+        if (NV >= (int64_t(1) << 31) || 16 * NT >= (int64_t(1) << 31))
+            fk::fail(FK_ERR_UNSUPPORTED, "batch exceeds 2^31 nodes or 2^27 elements");
+        // concatenate: node numbers of mesh m are offset by node_off[m]
+        std::vector<double> P(static_cast<size_t>(3 * NV));
+        std::vector<int32_t> T(static_cast<size_t>(4 * NT));
+        for (int m = 0; m < nmesh; ++m) {
+            std::copy(p[m], p[m] + 3 * nv[m], P.begin() + 3 * b->node_off[m]);
+            for (int64_t k = 0; k < 4 * nt[m]; ++k) {
+                const int32_t v = t[m][k];
+                if (v < index_base || v - index_base >= nv[m])
+                    fk::fail(FK_ERR_INDEX, "mesh " + std::to_string(m) + ": node number out of range");
+                T[4 * b->elem_off[m] + k] = static_cast<int32_t>(v - index_base + b->node_off[m]);
+            }
+        }
+        b->mesh = new fk_mesh();
+        b->mesh->ctx = ctx;
+        b->mesh->nv = NV;
+        b->mesh->nt = NT;
+        fk::build_mesh(b->mesh, P.data(), T.data(), 0, nullptr);
+        fk::build_block_table(b);
+        const size_t nvr = static_cast<size_t>(NV) * nrhs;
+        b->x.alloc(nvr); b->r.alloc(nvr); b->p.alloc(nvr); b->q.alloc(nvr);
+        const size_t np = static_cast<size_t>(b->nblocks) * nrhs;
+        b->part_pq.alloc(np); b->part_rz.alloc(2 * np); b->part_rr.alloc(np); b->part_bb.alloc(np);
+        b->done.alloc(b->nsys); b->status.alloc(b->nsys); b->active.alloc(b->nsys); b->ndone.alloc(1);
+        b->iters.alloc(b->nsys); b->thresh.alloc(b->nsys); b->scale.alloc(b->nsys);
+        b->hpart.alloc(np + 16);
+        b->hcount.alloc(4);
+    });
+    if (s != FK_OK) {
+        delete b;
+        return s;
+    }
+    *out = b;
+    return FK_OK;
+}
+
+int fk_batch_destroy(fk_batch *b) {
+    if (!b) return FK_ERR_INVALID_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return FK_OK;
+}
+
+int fk_batch_info(const fk_batch *b, int64_t *nodes, int64_t *nnz, int32_t *nblocks, int32_t *ncolours) {
+    if (!b) return FK_ERR_INVALID_ARG;
+    if (nodes) *nodes = b->mesh->nv;
+    if (nnz) *nnz = b->mesh->nnz;
+    if (nblocks) *nblocks = b->nblocks;
+    if (ncolours) *ncolours = b->mesh->ncolours;
+    return FK_OK;
+}
+
+int fk_batch_set_coefficient(fk_batch *b, const double *const *f) {
+    if (!b || !f) return FK_ERR_INVALID_ARG;
+    return guard(b->ctx, "fk_batch_set_coefficient", [&] {
+        fk::set_device(b->ctx);
+        std::vector<double> F(static_cast<size_t>(b->mesh->nt));
+        for (int m = 0; m < b->nmesh; ++m) {
+            if (!f[m]) fk::fail(FK_ERR_INVALID_ARG, "null coefficient array");
+            std::copy(f[m], f[m] + (b->elem_off[m + 1] - b->elem_off[m]), F.begin() + b->elem_off[m]);
+        }
+        fk::upload_coefficient(b->mesh, F.data());
+    });
+}
+
+int fk_batch_assemble(fk_batch *b) {
+    if (!b) return FK_ERR_INVALID_ARG;
+    return guard(b->ctx, "fk_batch_assemble", [&] {
+        fk::set_device(b->ctx);
+        fk::assemble(b->mesh, FK_ASM_COLOURED);
+    });
+}
+
+int fk_batch_set_dirichlet(fk_batch *b, const uint8_t *const *mask, const double *const *g) {
+    if (!b || !mask || !g) return FK_ERR_INVALID_ARG;
+    return guard(b->ctx, "fk_batch_set_dirichlet", [&] {
+        fk::set_device(b->ctx);
+        const int64_t NV = b->mesh->nv;
+        const int nr = b->nrhs;
+        std::vector<uint8_t> M(static_cast<size_t>(NV));
+        std::vector<double> G(static_cast<size_t>(NV) * nr, 0.0), G0(static_cast<size_t>(NV), 0.0);
+        for (int m = 0; m < b->nmesh; ++m) {
+            if (!mask[m] || !g[m]) fk::fail(FK_ERR_INVALID_ARG, "null Dirichlet array");
+            const int64_t a = b->node_off[m], n = b->node_off[m + 1] - a;
+            int64_t fixed = 0;
+            for (int64_t i = 0; i < n; ++i) {
+                M[a + i] = mask[m][i] ? 1 : 0;
+                fixed += M[a + i];
+                for (int j = 0; j < nr; ++j) {
+                    const double v = g[m][j * n + i];  // right-hand side major per mesh
+                    if (M[a + i] && !std::isfinite(v)) fk::fail(FK_ERR_NONFINITE, "non-finite Dirichlet value");
+                    G[(a + i) * nr + j] = M[a + i] ? v : 0.0;
+                }
+            }
+            if (fixed == 0) fk::fail(FK_ERR_SINGULAR, "mesh " + std::to_string(m) + " has no fixed node");
+        }
+        fk::set_dirichlet(b->mesh, M.data(), G0.data());
+        b->g.ensure(static_cast<size_t>(NV) * nr);
+        FK_HIP(hipMemcpy(b->g.get(), G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
+        b->have_dirichlet = true;
+    });
+}
+
+int fk_batch_solve(fk_batch *b, double rtol, double inner_rtol, int32_t max_outer, int64_t max_iter,
+                   int64_t *iterations, int32_t *outer, double *relres) {
+    if (!b || !(rtol >= 0.0) || !(inner_rtol > 0.0) || !(inner_rtol < 1.0) || max_outer < 1 || max_iter < 1)
+        return FK_ERR_INVALID_ARG;
+    return guard(b->ctx, "fk_batch_solve", [&] {
+        fk::set_device(b->ctx);
+        fk::batch_solve(b, rtol, inner_rtol, max_outer, max_iter, iterations, outer, relres);
+    });
+}
+
+int fk_batch_potential(fk_batch *b, int32_t mesh, double *phi) {
+    if (!b || !phi || mesh < 0 || mesh >= b->nmesh) return FK_ERR_INVALID_ARG;
+    return guard(b->ctx, "fk_batch_potential", [&] {
+        if (!b->have_phi) fk::fail(FK_ERR_STATE, "no potential");
+        fk::set_device(b->ctx);
+        const int64_t a = b->node_off[mesh], n = b->node_off[mesh + 1] - a;
+        const int nr = b->nrhs;
+        std::vector<double> tmp(static_cast<size_t>(n) * nr);
+        FK_HIP(hipMemcpyAsync(tmp.data(), b->phi.get() + a * nr, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              b->ctx->stream));
+        FK_HIP(hipStreamSynchronize(b->ctx->stream));
+        for (int64_t i = 0; i < n; ++i)
+            for (int j = 0; j < nr; ++j) phi[j * n + i] = tmp[i * nr + j];
+    });
+}
+
+int fk_batch_autotune(fk_batch *b, int32_t *lanes, int32_t *block, double *best_us) {
+    if (!b) return FK_ERR_INVALID_ARG;
+    return guard(b->ctx, "fk_batch_autotune", [&] {
+        fk::set_device(b->ctx);
+        fk::batch_autotune(b, lanes, block, best_us);
+    });
+}
+
+int fk_batch_set_launch(fk_batch *b, int32_t lanes, int32_t block) {
+    if (!b || (lanes != 4 && lanes != 8 && lanes != 16) || (block != 128 && block != 256 && block != 512))
+        return FK_ERR_INVALID_ARG;
+    b->lanes = lanes;
+    b->block = block;
+    return FK_OK;
+}
+
+}  // extern "C"

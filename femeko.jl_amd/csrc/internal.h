@@ -197,6 +197,7 @@ void field(fk_mesh *m, double *H, double *Hn, double *energy);
 void setup_partition(fk_mesh *m);
 void ensure_work_f64(fk_mesh *m);
 void compute_dinv_f64(fk_mesh *m);
+void compute_dinv_f32(fk_mesh *m);
 CgState run_pcg_f64(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref);
 void phi_axpy_f64(fk_mesh *m, const double *x, double s);
 void k_init_phi_dev(fk_mesh *m, int keep_free);

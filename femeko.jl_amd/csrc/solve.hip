@@ -514,6 +514,7 @@ static double neg_residual(fk_mesh *m, const double *phi, double *r) {
 // ---- entry points for the other translation units (nonlinear loop, adjoint)
 void ensure_work_f64(fk_mesh *m) { ensure_work<double>(m); }
 void compute_dinv_f64(fk_mesh *m) { compute_dinv<double>(m); }
+void compute_dinv_f32(fk_mesh *m) { compute_dinv<float>(m); }
 CgState run_pcg_f64(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref) {
     return run_pcg<double>(m, rtol, max_iter, bb_ref);
 }

@@ -78,6 +78,17 @@ SIGNATURES = {
     "fk_line_search": (_c.c_int, [_vp, _dp, _dp, _dp, _c.c_double, _dp]),
     "fk_get_coefficient": (_c.c_int, [_vp, _dp]),
     "fk_adjoint_field_target": (_c.c_int, [_vp, _up, _dp, _c.c_double, _dp, _dp, _i64p]),
+    "fk_batch_create": (_c.c_int, [_vp, _c.c_int32, _i64p, _i64p, _c.POINTER(_dp), _c.POINTER(_ip), _c.c_int32,
+                                   _c.c_int32, _vpp]),
+    "fk_batch_destroy": (_c.c_int, [_vp]),
+    "fk_batch_info": (_c.c_int, [_vp, _i64p, _i64p, _ip, _ip]),
+    "fk_batch_set_coefficient": (_c.c_int, [_vp, _c.POINTER(_dp)]),
+    "fk_batch_assemble": (_c.c_int, [_vp]),
+    "fk_batch_set_dirichlet": (_c.c_int, [_vp, _c.POINTER(_up), _c.POINTER(_dp)]),
+    "fk_batch_solve": (_c.c_int, [_vp, _c.c_double, _c.c_double, _c.c_int32, _c.c_int64, _i64p, _ip, _dp]),
+    "fk_batch_potential": (_c.c_int, [_vp, _c.c_int32, _dp]),
+    "fk_batch_autotune": (_c.c_int, [_vp, _ip, _ip, _dp]),
+    "fk_batch_set_launch": (_c.c_int, [_vp, _c.c_int32, _c.c_int32]),
     "fk_set_timing": (_c.c_int, [_vp, _c.c_int32]),
     "fk_spmv_timing": (_c.c_int, [_vp, _i64p, _dp]),
     "fk_set_lanes_per_row": (_c.c_int, [_vp, _c.c_int32]),

@@ -1,0 +1,111 @@
+"""Batched solves over the C ABI (fk_batch_*): many meshes, several applied
+fields each, solved together on one GPU."""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import FemekoHIPError, dptr, iptr, uptr
+from .device import Context, context
+
+
+@dataclass
+class BatchReport:
+    iterations: np.ndarray   # (nmesh, nrhs)
+    outer: np.ndarray
+    relres: np.ndarray
+
+
+class Batch:
+    """nmesh meshes x nrhs Dirichlet right-hand sides (fk_batch)."""
+
+    def __init__(self, meshes, nrhs: int = 3, ctx: Context | None = None, index_base: int = 1):
+        self.ctx = ctx or context(0)
+        self._L = self.ctx._L
+        self.nmesh = len(meshes)
+        self.nrhs = int(nrhs)
+        self.nv = np.array([m.p.shape[1] for m in meshes], dtype=np.int64)
+        self.nt = np.array([m.t.shape[1] for m in meshes], dtype=np.int64)
+        self._p = [np.ascontiguousarray(np.asarray(m.p, dtype=np.float64).T) for m in meshes]
+        self._t = [np.ascontiguousarray(np.asarray(m.t, dtype=np.int32).T) for m in meshes]
+        P = (_lib._dp * self.nmesh)(*[dptr(a) for a in self._p])
+        T = (_lib._ip * self.nmesh)(*[iptr(a) for a in self._t])
+        h = ctypes.c_void_p()
+        s = self._L.fk_batch_create(self.ctx.handle, self.nmesh, self.nv.ctypes.data_as(_lib._i64p),
+                                    self.nt.ctypes.data_as(_lib._i64p), P, T, int(index_base), self.nrhs,
+                                    ctypes.byref(h))
+        self.ctx.check(s, "fk_batch_create")
+        self.handle = h
+        self._p = self._t = None
+        nodes = ctypes.c_int64(0)
+        nnz = ctypes.c_int64(0)
+        nb = ctypes.c_int32(0)
+        nc = ctypes.c_int32(0)
+        self._L.fk_batch_info(h, ctypes.byref(nodes), ctypes.byref(nnz), ctypes.byref(nb), ctypes.byref(nc))
+        self.total_nodes, self.total_nnz = int(nodes.value), int(nnz.value)
+        self.nblocks, self.ncolours = int(nb.value), int(nc.value)
+
+    def set_coefficient(self, mus) -> None:
+        arrs = [np.ascontiguousarray(mu, dtype=np.float64) for mu in mus]
+        for a, n in zip(arrs, self.nt):
+            if a.size != n:
+                raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "coefficient size")
+        F = (_lib._dp * self.nmesh)(*[dptr(a) for a in arrs])
+        self.ctx.check(self._L.fk_batch_set_coefficient(self.handle, F), "fk_batch_set_coefficient")
+
+    def assemble(self) -> None:
+        self.ctx.check(self._L.fk_batch_assemble(self.handle), "fk_batch_assemble")
+
+    def set_dirichlet(self, masks, gs) -> None:
+        """masks[m]: (nv,), gs[m]: (nrhs, nv) Dirichlet values per right-hand side."""
+        ms = [np.ascontiguousarray(m, dtype=np.uint8) for m in masks]
+        gg = [np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(self.nrhs, -1)) for g in gs]
+        for a, g, n in zip(ms, gg, self.nv):
+            if a.size != n or g.shape[1] != n:
+                raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "Dirichlet array size")
+        M = (_lib._up * self.nmesh)(*[uptr(a) for a in ms])
+        G = (_lib._dp * self.nmesh)(*[dptr(a) for a in gg])
+        self.ctx.check(self._L.fk_batch_set_dirichlet(self.handle, M, G), "fk_batch_set_dirichlet")
+
+    def solve(self, rtol: float = 1e-10, inner_rtol: float = 1e-5, max_outer: int = 50,
+              max_iter: int = 1_000_000) -> BatchReport:
+        S = self.nmesh * self.nrhs
+        it = np.zeros(S, np.int64)
+        outer = np.zeros(S, np.int32)
+        rel = np.zeros(S)
+        self.ctx.check(self._L.fk_batch_solve(self.handle, float(rtol), float(inner_rtol), int(max_outer),
+                                              int(max_iter), it.ctypes.data_as(_lib._i64p), iptr(outer), dptr(rel)),
+                       "fk_batch_solve")
+        sh = (self.nmesh, self.nrhs)
+        return BatchReport(it.reshape(sh), outer.reshape(sh), rel.reshape(sh))
+
+    def potential(self, m: int) -> np.ndarray:
+        """(nrhs, nv[m]) potentials of mesh m."""
+        out = np.empty(self.nrhs * int(self.nv[m]))
+        self.ctx.check(self._L.fk_batch_potential(self.handle, int(m), dptr(out)), "fk_batch_potential")
+        return out.reshape(self.nrhs, -1)
+
+    def autotune(self):
+        ln = ctypes.c_int32(0)
+        bl = ctypes.c_int32(0)
+        us = ctypes.c_double(0.0)
+        self.ctx.check(self._L.fk_batch_autotune(self.handle, ctypes.byref(ln), ctypes.byref(bl), ctypes.byref(us)),
+                       "fk_batch_autotune")
+        return int(ln.value), int(bl.value), float(us.value)
+
+    def set_launch(self, lanes: int, block: int) -> None:
+        self.ctx.check(self._L.fk_batch_set_launch(self.handle, int(lanes), int(block)), "fk_batch_set_launch")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self._L.fk_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass

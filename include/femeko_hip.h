@@ -184,6 +184,36 @@ int fk_get_coefficient(fk_mesh *mesh, double *f);
 int fk_adjoint_field_target(fk_mesh *mesh, const uint8_t *target, const double *Hstar, double rtol,
                             double *J, double *dJdmu, int64_t *iterations);
 
+/* ---- batched solves ----------------------------------------------------------
+ * No counterpart in the reference (which solves one mesh and one applied
+ * field per driver run, magneticField.jl:96-108): nmesh independent meshes,
+ * nrhs (1..4) Dirichlet right-hand sides each, solved together. Mesh m has
+ * p[m] (3 x nv[m]) and t[m] (4 x nt[m]) in Femeko layout with node numbers
+ * local to the mesh. The batch is uploaded as one disconnected mesh (one
+ * colouring, one pattern, one assembly launch per colour); the solver runs
+ * nmesh*nrhs FP32 Jacobi-PCG systems with FP64 residual refinement in one set
+ * of graph-captured launches, sharing each mesh's matrix across its
+ * right-hand sides. */
+typedef struct fk_batch fk_batch;
+int fk_batch_create(fk_context *ctx, int32_t nmesh, const int64_t *nv, const int64_t *nt, const double *const *p,
+                    const int32_t *const *t, int32_t index_base, int32_t nrhs, fk_batch **out);
+int fk_batch_destroy(fk_batch *batch);
+int fk_batch_info(const fk_batch *batch, int64_t *nodes, int64_t *nnz, int32_t *nblocks, int32_t *ncolours);
+/* f[m]: nt[m] coefficients of mesh m */
+int fk_batch_set_coefficient(fk_batch *batch, const double *const *f);
+int fk_batch_assemble(fk_batch *batch);
+/* mask[m]: nv[m]; g[m]: nrhs * nv[m], right-hand side major (g[m][j*nv[m] + i]) */
+int fk_batch_set_dirichlet(fk_batch *batch, const uint8_t *const *mask, const double *const *g);
+/* per system s = m*nrhs + j: iterations[s], outer[s], relres[s] (FP64 true residual) */
+int fk_batch_solve(fk_batch *batch, double rtol, double inner_rtol, int32_t max_outer, int64_t max_iter,
+                   int64_t *iterations, int32_t *outer, double *relres);
+/* phi: nrhs * nv[mesh], right-hand side major */
+int fk_batch_potential(fk_batch *batch, int32_t mesh, double *phi);
+/* Autotuner: times the batched SpMM for lanes per row {4, 8, 16} x block
+ * size {128, 256, 512} on this batch and keeps the fastest. */
+int fk_batch_autotune(fk_batch *batch, int32_t *lanes, int32_t *block, double *best_us);
+int fk_batch_set_launch(fk_batch *batch, int32_t lanes, int32_t block);
+
 /* ---- instrumentation -------------------------------------------------------
  * Average device time of the sparse matrix-vector kernel over the last solve
  * (HIP events on the context stream around each launch), when enabled with
