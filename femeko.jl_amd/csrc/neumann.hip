@@ -1,0 +1,213 @@
+// Neumann flux load and the Lagrange-multiplier saddle solve of the
+// reference's linear driver, Magnetostatics/magneticField.jl:96-108:
+//   RHS = BoundaryIntegral(mesh, F, shell_id)      (src/FEM3D.jl:284-297)
+//   Lag = lagrange(mesh)                           (src/FEM3D.jl:274-281)
+//   [A Lag; Lag' 0] [u; m] = [b; 0]
+// A is singular with null space span{1} (A 1 = 0) and Lag' 1 = total volume
+// > 0, so the saddle system has the unique solution
+//   m = (1'b) / (1'Lag),   A u = b - m Lag  (consistent),   Lag' u = 0.
+// The GPU solves the consistent singular system with the same Jacobi-PCG
+// kernels (every node free) and projects the constant out at the end:
+//   u <- u - (Lag'u / Lag'1) 1.
+#include <algorithm>
+#include <cmath>
+
+#include "device.h"
+#include "internal.h"
+
+namespace fk {
+
+// F . n * area / 3 onto the 3 nodes of every tagged surface triangle
+// (FP64 atomics: a handful of triangles share a node)
+__global__ void k_boundary_integral(int64_t ns, const int32_t *__restrict__ sT, const double *__restrict__ nrm,
+                                    const double *__restrict__ AE, double fx, double fy, double fz,
+                                    const int32_t *__restrict__ ids, int nids, int32_t base,
+                                    double *__restrict__ rhs) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int32_t tag = sT[4 * s + 3];
+    bool in = false;
+    for (int k = 0; k < nids; ++k) in |= (tag == ids[k]);
+    if (!in) return;
+    const double v = (nrm[3 * s] * fx + nrm[3 * s + 1] * fy + nrm[3 * s + 2] * fz) * AE[s] / 3;
+    for (int i = 0; i < 3; ++i) unsafeAtomicAdd(&rhs[sT[4 * s + i] - base], v);
+}
+
+__global__ void k_dinv_free(const double *__restrict__ val, const int32_t *__restrict__ diag, int64_t nv,
+                            double *__restrict__ dinv, int32_t *__restrict__ bad) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const int32_t s = diag[i];
+    const double d = s >= 0 ? val[s] : 0.0;
+    if (!(d > 0.0) || !isfinite(d)) {
+        dinv[i] = 0.0;
+        atomicAdd(bad, 1);
+        return;
+    }
+    dinv[i] = 1.0 / d;
+}
+
+// partials of (sum a, sum c, sum a.c)
+__global__ __launch_bounds__(kBlock) void k_sums3(const double *__restrict__ a, const double *__restrict__ c,
+                                                  int64_t n, double *__restrict__ part) {
+    __shared__ double sh[kBlock / 64 + 1];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+        s0 += a[i];
+        s1 += c[i];
+        s2 += a[i] * c[i];
+    }
+    s0 = block_sum<kBlock>(s0, sh);
+    s1 = block_sum<kBlock>(s1, sh);
+    s2 = block_sum<kBlock>(s2, sh);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s0;
+        part[3 * blockIdx.x + 1] = s1;
+        part[3 * blockIdx.x + 2] = s2;
+    }
+}
+
+__global__ void k_axpy_inplace(double *__restrict__ y, const double *__restrict__ x, double a, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+__global__ void k_shift(double *__restrict__ y, double c, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] -= c;
+}
+
+__global__ __launch_bounds__(256) void k_lagrange_n(const int4 *__restrict__ tet, const double *__restrict__ X,
+                                                    const double *__restrict__ Y, const double *__restrict__ Z,
+                                                    double *__restrict__ C, int64_t e0, int64_t e1) {
+    const int64_t e = e0 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    const int4 nd = tet[e];
+    const Grad G = element_gradients(X, Y, Z, nd);
+    const double v = fabs(G.det) * (1.0 / 6.0) / 4;
+    C[nd.x] += v;
+    C[nd.y] += v;
+    C[nd.z] += v;
+    C[nd.w] += v;
+}
+
+void boundary_integral(fk_mesh *m, int64_t ns, const int32_t *sT, const double *nrm, const double *AE,
+                       const double *F, const int32_t *ids, int nids, int32_t base, double *rhs_host) {
+    hipStream_t st = m->ctx->stream;
+    for (int64_t s = 0; s < ns; ++s)
+        for (int i = 0; i < 3; ++i) {
+            const int64_t v = static_cast<int64_t>(sT[4 * s + i]) - base;
+            if (v < 0 || v >= m->nv) fail(FK_ERR_INDEX, "surface triangle node number out of range");
+        }
+    DevBuf<int32_t> dT(4 * std::max<int64_t>(ns, 1)), dids(std::max(nids, 1));
+    DevBuf<double> dn(3 * std::max<int64_t>(ns, 1)), dA(std::max<int64_t>(ns, 1)), rhs(m->nv);
+    if (ns > 0) {
+        FK_HIP(hipMemcpyAsync(dT.get(), sT, 4 * ns * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(dn.get(), nrm, 3 * ns * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(dA.get(), AE, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    if (nids > 0) FK_HIP(hipMemcpyAsync(dids.get(), ids, nids * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemsetAsync(rhs.get(), 0, m->nv * sizeof(double), st));
+    if (ns > 0)
+        k_boundary_integral<<<grid_for(ns, 256), 256, 0, st>>>(ns, dT.get(), dn.get(), dA.get(), F[0], F[1], F[2],
+                                                               dids.get(), nids, base, rhs.get());
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipMemcpyAsync(rhs_host, rhs.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+}
+
+static void sums3(fk_mesh *m, const double *a, const double *c, double out[3]) {
+    hipStream_t st = m->ctx->stream;
+    const int nb = std::max(1, m->nblocks);
+    DevBuf<double> part(3 * nb);
+    k_sums3<<<nb, kBlock, 0, st>>>(a, c, m->nv, part.get());
+    FK_HIP(hipGetLastError());
+    std::vector<double> h(3 * nb);
+    FK_HIP(hipMemcpyAsync(h.data(), part.get(), 3 * nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    out[0] = out[1] = out[2] = 0.0;
+    for (int i = 0; i < nb; ++i) {
+        out[0] += h[3 * i];
+        out[1] += h[3 * i + 1];
+        out[2] += h[3 * i + 2];
+    }
+}
+
+void solve_neumann(fk_mesh *m, const double *b_host, double rtol, int64_t max_iter, double *u_host,
+                   double *mult_out, int64_t *iters_out, double *relres_out) {
+    if (!m->have_values) fail(FK_ERR_STATE, "matrix not assembled");
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = m->nv;
+    ensure_work_f64(m);
+    PcgWork<double> &w = m->w64;
+    // every node free
+    {
+        DevBuf<int32_t> bad(1);
+        FK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), st));
+        k_dinv_free<<<grid_for(nv, 256), 256, 0, st>>>(m->val64.get(), m->diag.get(), nv, w.dinv.get(), bad.get());
+        FK_HIP(hipGetLastError());
+        int32_t h = 0;
+        FK_HIP(hipMemcpyAsync(&h, bad.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        m->dinv64_valid = false;  // the Dirichlet solves recompute theirs
+        if (h) fail(FK_ERR_SINGULAR, std::to_string(h) + " node(s) without a positive diagonal");
+    }
+    // Lag and b
+    DevBuf<double> lag(nv);
+    FK_HIP(hipMemsetAsync(lag.get(), 0, nv * sizeof(double), st));
+    for (int c = 0; c < m->ncolours; ++c) {
+        const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
+        if (e1 <= e0) continue;
+        k_lagrange_n<<<grid_for(e1 - e0, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
+                                                             lag.get(), e0, e1);
+    }
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipMemcpyAsync(w.r.get(), b_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
+    double s[3];
+    sums3(m, w.r.get(), lag.get(), s);  // sum b, sum Lag
+    const double mult = s[1] != 0.0 ? s[0] / s[1] : 0.0;
+    // b' = b - m Lag, consistent with the null space of A
+    k_axpy_inplace<<<grid_for(nv, 256), 256, 0, st>>>(w.r.get(), lag.get(), -mult, nv);
+    FK_HIP(hipGetLastError());
+    const CgState cs = run_pcg_f64(m, rtol, max_iter, -1.0);
+    if (cs.status != FK_OK) fail(cs.status, std::string("Neumann solve: ") + fk_status_string(cs.status));
+    // project: Lag' u = 0
+    sums3(m, w.x.get(), lag.get(), s);
+    k_shift<<<grid_for(nv, 256), 256, 0, st>>>(w.x.get(), s[2] / s[1], nv);
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipMemcpyAsync(u_host, w.x.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    if (mult_out) *mult_out = mult;
+    if (iters_out) *iters_out = cs.iters;
+    if (relres_out) *relres_out = cs.bb > 0.0 ? std::sqrt(cs.rr / cs.bb) : 0.0;
+}
+
+}  // namespace fk
+
+using fk::guard;
+
+extern "C" {
+
+int fk_boundary_integral(fk_mesh *m, int64_t ns, const int32_t *surfaceT, const double *normal, const double *AE,
+                         const double *F, const int32_t *shell_ids, int32_t nshell, int32_t index_base,
+                         double *RHS) {
+    if (!m || ns < 0 || (ns > 0 && (!surfaceT || !normal || !AE)) || !F || (nshell > 0 && !shell_ids) ||
+        nshell < 0 || !RHS || (index_base != 0 && index_base != 1))
+        return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_boundary_integral", [&] {
+        fk::set_device(m->ctx);
+        fk::boundary_integral(m, ns, surfaceT, normal, AE, F, shell_ids, nshell, index_base, RHS);
+    });
+}
+
+int fk_solve_neumann(fk_mesh *m, const double *b, double rtol, int64_t max_iter, double *u, double *multiplier,
+                     int64_t *iterations, double *relres) {
+    if (!m || !b || !u || !(rtol > 0.0) || max_iter < 1) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_solve_neumann", [&] {
+        fk::set_device(m->ctx);
+        fk::solve_neumann(m, b, rtol, max_iter, u, multiplier, iterations, relres);
+    });
+}
+
+}  // extern "C"

@@ -2,10 +2,12 @@
 
 Host-side mirror of the reference interface over the C ABI in
 include/femeko_hip.h (libfemeko_hip.so). See fem3d for the Femeko-named
-drop-ins, device for the handles, mesh/workloads for the synthetic meshes.
+drop-ins, device for the handles, batch for batched solves, adjoint for the
+field-target gradient, mesh/workloads for the synthetic meshes.
 """
 from ._lib import FemekoHIPError, LIB_PATH  # noqa: F401
 from .device import Context, DeviceMesh, SolveReport, context  # noqa: F401
-from .fem3d import (device_mesh, invalidate, lagrange, lineSearch, localStiffnessMatrix,  # noqa: F401
-                    magneticField, solveDirichlet, stiffnessMatrix, tangentialStiffnessMatrix)
+from .fem3d import (BoundaryIntegral, device_mesh, invalidate, lagrange, lineSearch,  # noqa: F401
+                    localStiffnessMatrix, magneticField, solveDirichlet, solveSaddle, stiffnessMatrix,
+                    tangentialStiffnessMatrix)
 from .mesh import MESH, MU0, kuhn_box_mesh  # noqa: F401

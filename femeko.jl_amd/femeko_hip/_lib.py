@@ -73,6 +73,8 @@ SIGNATURES = {
     "fk_potential": (_c.c_int, [_vp, _dp]),
     "fk_set_potential": (_c.c_int, [_vp, _dp]),
     "fk_field": (_c.c_int, [_vp, _dp, _dp, _dp]),
+    "fk_boundary_integral": (_c.c_int, [_vp, _c.c_int64, _ip, _dp, _dp, _dp, _ip, _c.c_int32, _c.c_int32, _dp]),
+    "fk_solve_neumann": (_c.c_int, [_vp, _dp, _c.c_double, _c.c_int64, _dp, _dp, _i64p, _dp]),
     "fk_nonlinear_froehlich": (_c.c_int, [_vp, _up, _c.c_double, _c.c_double, _c.c_double, _c.c_double,
                                           _c.c_int32, _c.c_double, _ip, _i64p, _dp]),
     "fk_line_search": (_c.c_int, [_vp, _dp, _dp, _dp, _c.c_double, _dp]),

@@ -217,6 +217,29 @@ class DeviceMesh:
                                         ctypes.byref(E) if want_energy else None), "fk_field")
         return (H.T if H is not None else None), Hn, (float(E.value) if want_energy else None)
 
+    def boundary_integral(self, surfaceT, normal, AE, F, shell_ids, index_base: int = 1) -> np.ndarray:
+        sT = np.ascontiguousarray(np.asarray(surfaceT, dtype=np.int32).T)   # (ns,4) == 4 x ns column-major
+        nr = np.ascontiguousarray(np.asarray(normal, dtype=np.float64).T)
+        ae = _f64(AE, sT.shape[0], "AE")
+        ids = np.ascontiguousarray(np.atleast_1d(shell_ids), dtype=np.int32)
+        rhs = np.empty(self.nv)
+        self.ctx.check(self._L.fk_boundary_integral(self.handle, sT.shape[0], iptr(sT), dptr(nr), dptr(ae),
+                                                    dptr(_f64(F, 3, "F")), iptr(ids), ids.size, int(index_base),
+                                                    dptr(rhs)), "fk_boundary_integral")
+        return rhs
+
+    def solve_neumann(self, b, rtol: float = 1e-10, max_iter: int = 1_000_000):
+        """Saddle solve [A Lag; Lag' 0][u; m] = [b; 0]: returns (u, m, iterations, relres)."""
+        b = _f64(b, self.nv, "b")
+        u = np.empty(self.nv)
+        mult = ctypes.c_double(0.0)
+        it = ctypes.c_int64(0)
+        rel = ctypes.c_double(0.0)
+        self.ctx.check(self._L.fk_solve_neumann(self.handle, dptr(b), float(rtol), int(max_iter), dptr(u),
+                                                ctypes.byref(mult), ctypes.byref(it), ctypes.byref(rel)),
+                       "fk_solve_neumann")
+        return u, float(mult.value), int(it.value), float(rel.value)
+
     def get_coefficient(self) -> np.ndarray:
         f = np.empty(self.nt)
         self.ctx.check(self._L.fk_get_coefficient(self.handle, dptr(f)), "fk_get_coefficient")

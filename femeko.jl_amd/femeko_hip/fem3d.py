@@ -86,6 +86,21 @@ def lagrange(mesh) -> np.ndarray:
     return device_mesh(mesh).lagrange()
 
 
+def BoundaryIntegral(mesh, F, shell_id) -> np.ndarray:
+    """RHS[nds] += dot(normal, F) * AE / 3 over the tagged surface triangles."""
+    return device_mesh(mesh).boundary_integral(mesh.surfaceT, mesh.normal, mesh.AE, F, shell_id)
+
+
+def solveSaddle(mesh, mu, b, rtol: float = 1e-10):
+    """u of [A Lag; Lag' 0] \\ [b; 0] (Magnetostatics/magneticField.jl:99-108)
+    with A = stiffnessMatrix(mesh, mu). Returns (u, multiplier)."""
+    dm = device_mesh(mesh)
+    dm.set_coefficient(mu)
+    dm.assemble()
+    u, mult, _it, _rel = dm.solve_neumann(b, rtol)
+    return u, mult
+
+
 def magneticField(mesh, u) -> np.ndarray:
     """H (3, nt) = -sum_nd u[nd] grad(lambda_nd) per element."""
     dm = device_mesh(mesh)

@@ -149,6 +149,21 @@ int fk_set_potential(fk_mesh *mesh, const double *phi);
  * NULL; src/magneticProperties.jl:268-273 with the element permeability). */
 int fk_field(fk_mesh *mesh, double *H, double *Hn, double *energy);
 
+/* ---- Neumann flux load and Lagrange saddle solve ----------------------------
+ * BoundaryIntegral(mesh, F, shell_id) (src/FEM3D.jl:284-297): for surface
+ * triangles whose tag (row 4 of surfaceT, 4 x ns) is one of shell_ids,
+ * RHS[nodes] += dot(normal, F) * AE / 3. RHS: nv (overwritten). */
+int fk_boundary_integral(fk_mesh *mesh, int64_t ns, const int32_t *surfaceT, const double *normal,
+                         const double *AE, const double *F, const int32_t *shell_ids, int32_t nshell,
+                         int32_t index_base, double *RHS);
+/* The saddle solve [A Lag; Lag' 0][u; m] = [b; 0] of
+ * Magnetostatics/magneticField.jl:99-108 with A the assembled matrix and Lag
+ * = lagrange(mesh): m = sum(b)/sum(Lag), then Jacobi-PCG on the consistent
+ * singular system A u = b - m Lag (no fixed node) to rtol, and the constant
+ * projected out so that Lag' u = 0. u: nv. */
+int fk_solve_neumann(fk_mesh *mesh, const double *b, double rtol, int64_t max_iter, double *u,
+                     double *multiplier, int64_t *iterations, double *relres);
+
 /* ---- nonlinear loop and line search ----------------------------------------
  * Picard/Newton mu(|H|) loop of Magnetostatics/NewtonRaphson.jl:148-212 with
  * the Froehlich law mu = mu0 (1 + Ms/(a+|H|)) on elements with region != 0

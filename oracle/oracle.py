@@ -364,3 +364,16 @@ def objective(mesh, mask, g, rho, target, Hstar, rtol: float = 1e-13,
     H, _Hn = field(mesh, phi)
     d = H.T - np.asarray(Hstar)[None, :]
     return float(np.sum(np.where(target, mesh.VE, 0.0) * np.sum(d * d, axis=1)))
+
+
+def saddle_solve(A: CSC, Lag: np.ndarray, b: np.ndarray):
+    """[A Lag; Lag' 0] \\ [b; 0] (Magnetostatics/magneticField.jl:107) by a
+    direct sparse LU of the bordered matrix (scipy SuperLU in place of Julia's
+    UMFPACK). Returns (u, multiplier)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    S = A.to_scipy()
+    L = sp.csc_matrix(Lag.reshape(-1, 1))
+    K = sp.bmat([[S, L], [L.T, None]], format="csc")
+    sol = sla.spsolve(K, np.concatenate([b, [0.0]]))
+    return sol[:-1], sol[-1]
