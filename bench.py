@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=160)
     ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--format", default="csr", choices=["csr", "sell64"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -109,6 +110,7 @@ def main():
     t0 = time.perf_counter()
     dm = DeviceMesh(m, ctx)
     dm.set_lanes(args.lanes)
+    dm.set_format(args.format)
     log(f"device mesh: nnz={dm.nnz} colours={dm.ncolours} ({time.perf_counter()-t0:.1f} s)")
     dm.set_coefficient(pr.mu)
     dm.set_dirichlet(pr.mask, pr.g)
@@ -192,7 +194,7 @@ def main():
                          f"[1,1e3], Dirichlet phi=-x; step = value assembly + FP32 Jacobi-PCG (inner rtol 1e-5) "
                          f"with FP64 refinement to rtol 1e-10"),
             "nodes": m.nv, "tets": m.nt, "nnz": dm.nnz, "free_dofs": free, "colours": dm.ncolours,
-            "lanes_per_row": args.lanes,
+            "lanes_per_row": args.lanes, "spmv_format": args.format,
         },
         "iterations_f32": rep.iterations,
         "outer_refinements": rep.outer,
@@ -204,7 +206,7 @@ def main():
         "error_vs_fp64_rtol1e-13": {"phi_rel_l2": err_phi, "Hn_rel_l2_VE": err_H, "energy_rel": err_E},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_cg_spmv<float,%d>" % args.lanes,
+            "kernel": ("k_cg_spmv<float,%d>" % args.lanes) if args.format == "csr" else "k_cg_spmv_sell<float>",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

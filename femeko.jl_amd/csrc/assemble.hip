@@ -181,10 +181,7 @@ void assemble(fk_mesh *m, int method) {
         }
     }
     FK_HIP(hipGetLastError());
-    m->have_values = true;
-    m->have_val32 = false;
-    m->dinv64_valid = false;
-    m->dinv32_valid = false;
+    values_changed(m);
 }
 
 void make_val32(fk_mesh *m) {

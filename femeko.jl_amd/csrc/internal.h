@@ -168,6 +168,15 @@ struct fk_mesh {
     fk::HostBuf<fk::CgState> hstate;  // pinned copies of the CG state
     fk::HostBuf<double> hpart;        // pinned copies of partials
 
+    // SELL-64 copy of the matrix for the solver (CSR stays the storage format)
+    int format = 0;                   // 0: CSR kernels, 1: SELL-64 kernels
+    bool have_sell = false, sell64_valid = false, sell32_valid = false;
+    int64_t nslices = 0, sell_len = 0, sell_chunk = 0;
+    fk::DevBuf<int64_t> sell_ptr;
+    fk::DevBuf<int32_t> sell_width, sell_col, sell_map;
+    fk::DevBuf<double> sell_val64;
+    fk::DevBuf<float> sell_val32;
+
     int lanes = 16;                   // lanes per CSR row in SpMV
     int nblocks = 0;                  // persistent grid of the solver kernels
     int64_t chunk = 0;                // rows per logical block
@@ -198,6 +207,15 @@ void setup_partition(fk_mesh *m);
 void ensure_work_f64(fk_mesh *m);
 void compute_dinv_f64(fk_mesh *m);
 void compute_dinv_f32(fk_mesh *m);
+// call whenever val64 changed: drops every derived copy (FP32, SELL, D^-1)
+inline void values_changed(fk_mesh *m) {
+    m->have_values = true;
+    m->have_val32 = false;
+    m->dinv64_valid = false;
+    m->dinv32_valid = false;
+    m->sell64_valid = false;
+    m->sell32_valid = false;
+}
 CgState run_pcg_f64(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref);
 void phi_axpy_f64(fk_mesh *m, const double *x, double s);
 void k_init_phi_dev(fk_mesh *m, int keep_free);

@@ -226,10 +226,7 @@ void nonlinear_froehlich(fk_mesh *m, const uint8_t *region_host, double Ms, doub
                 m->slot.get(), m->val64.get(), w.r.get(), newton, e0, e1);
         }
         FK_HIP(hipGetLastError());
-        m->have_values = true;
-        m->have_val32 = false;
-        m->dinv64_valid = false;
-        m->dinv32_valid = false;
+        values_changed(m);
         compute_dinv_f64(m);
         m->dinv64_valid = true;
         const CgState s = run_pcg_f64(m, rtol, 1000000, -1.0);

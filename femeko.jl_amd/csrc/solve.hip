@@ -170,6 +170,96 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
     for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
+// ---- SELL-64: slices of 64 rows (one wavefront), entry k of lane l of slice
+// s at sell_ptr[s] + 64 k + l; padding entries carry value 0 and column =
+// their own row. Every matrix load of a wave is 64 consecutive elements.
+template <typename T>
+__global__ void k_sell_gather(const T *__restrict__ src, const int32_t *__restrict__ map, int64_t n,
+                              T *__restrict__ dst) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t k = map[i];
+    dst[i] = k >= 0 ? src[k] : T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ T sell_row(const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
+                                      const int32_t *__restrict__ col, const T *__restrict__ val,
+                                      const T *__restrict__ x, int64_t s, int lane) {
+    const int64_t base = sptr[s] + lane;
+    const int W = width[s];
+    T acc0 = T(0), acc1 = T(0);
+    int k = 0;
+    for (; k + 1 < W; k += 2) {
+        const int64_t o0 = base + 64 * static_cast<int64_t>(k);
+        const int64_t o1 = o0 + 64;
+        acc0 += val[o0] * x[col[o0]];
+        acc1 += val[o1] * x[col[o1]];
+    }
+    if (k < W) {
+        const int64_t o0 = base + 64 * static_cast<int64_t>(k);
+        acc0 += val[o0] * x[col[o0]];
+    }
+    return acc0 + acc1;
+}
+
+// K1 in SELL form: q = A p, partial p.q; each wave owns whole slices
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(const int64_t *__restrict__ sptr,
+                                                         const int32_t *__restrict__ width,
+                                                         const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                         const T *__restrict__ p, T *__restrict__ q,
+                                                         double *__restrict__ part, int64_t n, int64_t nslices,
+                                                         int64_t chunk_slices, int nblocks,
+                                                         const CgState *__restrict__ st) {
+    if (st->done) return;
+    __shared__ double sh[kBlock / 64 + 1];
+    const int lb = logical_block(nblocks);
+    const int64_t s0 = static_cast<int64_t>(lb) * chunk_slices;
+    const int64_t s1 = min(nslices, s0 + chunk_slices);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int64_t s = s0 + wv; s < s1; s += kBlock / 64) {
+        const T y = sell_row<T>(sptr, width, col, val, p, s, lane);
+        const int64_t row = s * 64 + lane;
+        if (row < n) {
+            q[row] = y;
+            acc += static_cast<double>(p[row]) * static_cast<double>(y);
+        }
+    }
+    const double tot = block_sum<kBlock>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// FP64 r = -mask(A phi), SELL form
+__global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const int64_t *__restrict__ sptr,
+                                                              const int32_t *__restrict__ width,
+                                                              const int32_t *__restrict__ col,
+                                                              const double *__restrict__ val,
+                                                              const double *__restrict__ phi,
+                                                              const uint8_t *__restrict__ mask,
+                                                              double *__restrict__ r, double *__restrict__ part,
+                                                              int64_t n, int64_t nslices, int64_t chunk_slices,
+                                                              int nblocks) {
+    __shared__ double sh[kBlock / 64 + 1];
+    const int lb = logical_block(nblocks);
+    const int64_t s0 = static_cast<int64_t>(lb) * chunk_slices;
+    const int64_t s1 = min(nslices, s0 + chunk_slices);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int64_t s = s0 + wv; s < s1; s += kBlock / 64) {
+        const double y = sell_row<double>(sptr, width, col, val, phi, s, lane);
+        const int64_t row = s * 64 + lane;
+        if (row < n) {
+            const double v = mask[row] ? 0.0 : -y;
+            r[row] = v;
+            acc += v * v;
+        }
+    }
+    const double tot = block_sum<kBlock>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
 // x = 0, p = z = D^-1 r, partials of r.z and r.r (initial residual r given)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_init(T *__restrict__ x, T *__restrict__ r, T *__restrict__ p,
@@ -354,6 +444,96 @@ static void compute_dinv(fk_mesh *m) {
     if (h) fail(FK_ERR_SINGULAR, std::to_string(h) + " free node(s) without a positive diagonal");
 }
 
+// fill the SELL-64 columns and the SELL -> CSR position map
+__global__ void k_sell_fill(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
+                            const int64_t *__restrict__ sptr, const int32_t *__restrict__ width, int64_t nv,
+                            int64_t nrows_padded, int32_t *__restrict__ scol, int32_t *__restrict__ smap) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= nrows_padded) return;
+    const int64_t s = r >> 6;
+    const int lane = static_cast<int>(r & 63);
+    const int W = width[s];
+    const int64_t base = sptr[s] + lane;
+    const int32_t a = r < nv ? rowptr[r] : 0;
+    const int32_t len = r < nv ? rowptr[r + 1] - a : 0;
+    const int32_t self = static_cast<int32_t>(r < nv ? r : nv - 1);
+    for (int k = 0; k < W; ++k) {
+        const int64_t o = base + 64 * static_cast<int64_t>(k);
+        if (k < len) {
+            scol[o] = colidx[a + k];
+            smap[o] = a + k;
+        } else {
+            scol[o] = self;
+            smap[o] = -1;
+        }
+    }
+}
+
+static void build_sell(fk_mesh *m) {
+    if (m->have_sell) return;
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = m->nv;
+    std::vector<int32_t> rp(static_cast<size_t>(nv + 1));
+    FK_HIP(hipMemcpyAsync(rp.data(), m->rowptr.get(), (nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    const int64_t ns = (nv + 63) / 64;
+    std::vector<int32_t> width(static_cast<size_t>(ns));
+    std::vector<int64_t> sptr(static_cast<size_t>(ns + 1), 0);
+    for (int64_t s = 0; s < ns; ++s) {
+        int32_t w = 0;
+        for (int64_t r = s * 64; r < std::min(nv, s * 64 + 64); ++r) w = std::max(w, rp[r + 1] - rp[r]);
+        width[s] = w;
+        sptr[s + 1] = sptr[s] + 64 * static_cast<int64_t>(w);
+    }
+    m->nslices = ns;
+    m->sell_len = sptr[ns];
+    m->sell_chunk = (ns + m->nblocks - 1) / m->nblocks;
+    m->sell_ptr.alloc(ns + 1);
+    m->sell_width.alloc(ns);
+    m->sell_col.alloc(m->sell_len);
+    m->sell_map.alloc(m->sell_len);
+    FK_HIP(hipMemcpyAsync(m->sell_ptr.get(), sptr.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(m->sell_width.get(), width.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    k_sell_fill<<<grid_for(ns * 64, 256), 256, 0, st>>>(m->rowptr.get(), m->colidx.get(), m->sell_ptr.get(),
+                                                        m->sell_width.get(), nv, ns * 64, m->sell_col.get(),
+                                                        m->sell_map.get());
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipStreamSynchronize(st));
+    m->have_sell = true;
+}
+
+// bring the solver's copy of the values up to date with val64 / val32
+template <typename T>
+static void sync_solver_values(fk_mesh *m);
+template <>
+void sync_solver_values<double>(fk_mesh *m) {
+    if (m->format != 1 || m->sell64_valid) return;
+    build_sell(m);
+    m->sell_val64.ensure(m->sell_len);
+    k_sell_gather<double><<<grid_for(m->sell_len, 256), 256, 0, m->ctx->stream>>>(
+        m->val64.get(), m->sell_map.get(), m->sell_len, m->sell_val64.get());
+    FK_HIP(hipGetLastError());
+    m->sell64_valid = true;
+}
+template <>
+void sync_solver_values<float>(fk_mesh *m) {
+    if (m->format != 1 || m->sell32_valid) return;
+    build_sell(m);
+    make_val32(m);
+    m->sell_val32.ensure(m->sell_len);
+    k_sell_gather<float><<<grid_for(m->sell_len, 256), 256, 0, m->ctx->stream>>>(
+        m->val32.get(), m->sell_map.get(), m->sell_len, m->sell_val32.get());
+    FK_HIP(hipGetLastError());
+    m->sell32_valid = true;
+}
+
+template <typename T>
+static const T *sell_values(fk_mesh *m);
+template <>
+const double *sell_values<double>(fk_mesh *m) { return m->sell_val64.get(); }
+template <>
+const float *sell_values<float>(fk_mesh *m) { return m->sell_val32.get(); }
+
 template <typename T, int LPR>
 static void launch_spmv(fk_mesh *m, PcgWork<T> &w) {
     k_cg_spmv<T, LPR><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(m->rowptr.get(), m->colidx.get(), values<T>(m),
@@ -363,6 +543,12 @@ static void launch_spmv(fk_mesh *m, PcgWork<T> &w) {
 
 template <typename T>
 static void launch_spmv_lanes(fk_mesh *m, PcgWork<T> &w) {
+    if (m->format == 1) {
+        k_cg_spmv_sell<T><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(
+            m->sell_ptr.get(), m->sell_width.get(), m->sell_col.get(), sell_values<T>(m), w.p.get(), w.q.get(),
+            w.part_pq.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks, w.state.get());
+        return;
+    }
     switch (m->lanes) {
         case 4: launch_spmv<T, 4>(m, w); break;
         case 8: launch_spmv<T, 8>(m, w); break;
@@ -422,6 +608,7 @@ static CgState run_pcg(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref)
     hipStream_t st = m->ctx->stream;
     PcgWork<T> &w = work<T>(m);
     const int nb = m->nblocks;
+    sync_solver_values<T>(m);
     k_cg_init<T><<<nb, kBlock, 0, st>>>(w.x.get(), w.r.get(), w.p.get(), w.dinv.get(), w.part_rz.get(),
                                         w.part_bb.get(), m->nv, m->chunk, nb);
     k_cg_start<<<1, kBlock, 0, st>>>(w.part_bb.get(), nb, bb_ref, rtol, max_iter, w.state.get());
@@ -443,7 +630,8 @@ static CgState run_pcg(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref)
     }
     // graph of G iterations (G even keeps the parity sequence)
     const int G = m->nv > 200000 ? 32 : 16;
-    if (!w.graph || w.graph_iters != G || w.graph_lanes != m->lanes) {
+    const int cfg = m->lanes + 1000 * m->format;
+    if (!w.graph || w.graph_iters != G || w.graph_lanes != cfg) {
         w.reset_graph();
         hipGraph_t graph;
         FK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -453,7 +641,7 @@ static CgState run_pcg(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref)
         (void)hipGraphDestroy(graph);
         FK_HIP(e);
         w.graph_iters = G;
-        w.graph_lanes = m->lanes;
+        w.graph_lanes = cfg;
     }
     // launch chunk k+1 before waiting for the state copy after chunk k
     FK_HIP(hipGraphLaunch(w.graph, st));
@@ -496,7 +684,13 @@ static void launch_neg_residual(fk_mesh *m, const double *phi, double *r, double
 static double neg_residual(fk_mesh *m, const double *phi, double *r) {
     hipStream_t st = m->ctx->stream;
     m->part_misc.ensure(m->nblocks);
-    switch (m->lanes) {
+    if (m->format == 1) {
+        sync_solver_values<double>(m);
+        k_neg_residual_sell<<<m->nblocks, kBlock, 0, st>>>(m->sell_ptr.get(), m->sell_width.get(), m->sell_col.get(),
+                                                           m->sell_val64.get(), phi, m->mask.get(), r,
+                                                           m->part_misc.get(), m->nv, m->nslices, m->sell_chunk,
+                                                           m->nblocks);
+    } else switch (m->lanes) {
         case 4: launch_neg_residual<4>(m, phi, r, m->part_misc.get()); break;
         case 8: launch_neg_residual<8>(m, phi, r, m->part_misc.get()); break;
         case 32: launch_neg_residual<32>(m, phi, r, m->part_misc.get()); break;
@@ -755,6 +949,12 @@ int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms) {
     if (!ctx) return FK_ERR_INVALID_ARG;
     if (launches) *launches = ctx->timing.launches;
     if (total_ms) *total_ms = ctx->timing.total_ms;
+    return FK_OK;
+}
+
+int fk_set_spmv_format(fk_mesh *m, int32_t format) {
+    if (!m || (format != FK_SPMV_CSR && format != FK_SPMV_SELL64)) return FK_ERR_INVALID_ARG;
+    m->format = format;
     return FK_OK;
 }
 

@@ -285,6 +285,11 @@ class DeviceMesh:
                                               ctypes.byref(res)), "fk_line_search")
         return float(res.value)
 
+    def set_format(self, fmt: str) -> None:
+        """Solver SpMV format: 'csr' or 'sell64'."""
+        code = {"csr": 0, "sell64": 1, "sell": 1}[fmt]
+        self.ctx.check(self._L.fk_set_spmv_format(self.handle, code), "fk_set_spmv_format")
+
     def set_lanes(self, lanes: int) -> None:
         self.ctx.check(self._L.fk_set_lanes_per_row(self.handle, int(lanes)), "fk_set_lanes_per_row")
 

@@ -237,6 +237,13 @@ int fk_set_timing(fk_context *ctx, int32_t enable);
 int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms);
 /* SpMV tuning: lanes per row (4, 8, 16, 32, 64) of the CSR kernels. */
 int fk_set_lanes_per_row(fk_mesh *mesh, int32_t lanes);
+/* Matrix format of the solver's SpMV: FK_SPMV_CSR (the assembled CSR) or
+ * FK_SPMV_SELL64 (a sliced-ELL copy, 64 rows per slice = one wavefront,
+ * refreshed from the CSR values after each assembly). Results agree to
+ * rounding; the exported matrix is always CSR. */
+#define FK_SPMV_CSR 0
+#define FK_SPMV_SELL64 1
+int fk_set_spmv_format(fk_mesh *mesh, int32_t format);
 
 #ifdef __cplusplus
 }

@@ -174,6 +174,23 @@ def test_solve_f32_refined_linear20(lin20):
     assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) < 1e-9
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_sell64_format_solves_agree(lin20, prec):
+    pr, A, dm = lin20
+    golden = np.load(GOLDEN)
+    try:
+        dm.set_format("sell64")
+        rep = dm.solve(prec, rtol=1e-10)
+        phi = dm.potential()
+        assert rep.relres < 1e-10
+        assert np.linalg.norm(phi - golden["phi"]) / np.linalg.norm(golden["phi"]) < 1e-9
+        it_ref = int(golden["pcg_iters_1e10"])
+        if prec == "f64":
+            assert abs(rep.iterations - it_ref) <= max(3, 0.05 * it_ref)
+    finally:
+        dm.set_format("csr")
+
+
 def test_true_residual_of_gpu_solution(lin20):
     pr, A, dm = lin20
     dm.solve("f64", rtol=1e-10)
