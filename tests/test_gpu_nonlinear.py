@@ -60,11 +60,10 @@ def test_femeko_named_dropins(oracle):
     A = F.stiffnessMatrix(m, pr.mu)
     Ar = oracle.stiffnessMatrix(m, pr.mu)
     assert np.array_equal(A.indptr + 1, Ar.colptr) and np.array_equal(A.indices + 1, Ar.rowval)
-    scale = oracle.sparse_from_local(m, np.abs(oracle.localStiffnessMatrix(m, pr.mu))).nzval
+    scale = oracle.term_scale(m, pr.mu)
     assert np.all(np.abs(A.data - Ar.nzval) <= 1e-12 * scale)
     A1 = F.stiffnessMatrix(m)  # f defaults to ones
-    assert np.all(np.abs(A1.data - oracle.stiffnessMatrix(m).nzval)
-                  <= 1e-12 * oracle.sparse_from_local(m, np.abs(oracle.localStiffnessMatrix(m))).nzval)
+    assert np.all(np.abs(A1.data - oracle.stiffnessMatrix(m).nzval) <= 1e-12 * oracle.term_scale(m))
     Ak = F.localStiffnessMatrix(m, pr.mu)
     assert Ak.shape == (16, m.nt)
     np.testing.assert_allclose(F.lagrange(m), oracle.lagrange(m), rtol=1e-13)
@@ -79,6 +78,6 @@ def test_femeko_named_dropins(oracle):
     dmu = -rng.random(m.nt) * 1e-9
     At = F.tangentialStiffnessMatrix(m, Hv, dmu)
     Atr = oracle.tangentialStiffnessMatrix(m, Hv, dmu)
-    sc = oracle.sparse_from_local(m, np.abs(oracle.tangential_local(m, Hv, dmu))).nzval
-    assert np.all(np.abs(At.data - Atr.nzval) <= 1e-12 * sc + 1e-300)
+    sc = oracle.tangent_scale(m, Hv, dmu)
+    assert np.all(np.abs(At.data - Atr.nzval) <= 1e-12 * sc)
     F.invalidate(m)
