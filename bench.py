@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=160)
     ap.add_argument("--lanes", type=int, default=16)
-    ap.add_argument("--format", default="csr", choices=["csr", "sell64"])
+    ap.add_argument("--format", default="sell64", choices=["csr", "sell64"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

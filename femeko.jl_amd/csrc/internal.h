@@ -169,7 +169,7 @@ struct fk_mesh {
     fk::HostBuf<double> hpart;        // pinned copies of partials
 
     // SELL-64 copy of the matrix for the solver (CSR stays the storage format)
-    int format = 0;                   // 0: CSR kernels, 1: SELL-64 kernels
+    int format = 1;                   // 0: CSR kernels, 1: SELL-64 kernels (default)
     bool have_sell = false, sell64_valid = false, sell32_valid = false;
     int64_t nslices = 0, sell_len = 0, sell_chunk = 0;
     fk::DevBuf<int64_t> sell_ptr;

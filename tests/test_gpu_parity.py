@@ -175,12 +175,14 @@ def test_solve_f32_refined_linear20(lin20):
     assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) < 1e-9
 
 
+@pytest.mark.parametrize("fmt", ["csr", "sell64"])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_sell64_format_solves_agree(lin20, prec):
+def test_spmv_formats_solve_agree(lin20, prec, fmt):
+    # the solver's CSR kernels and its SELL-64 copy give the same solution
     pr, A, dm = lin20
     golden = np.load(GOLDEN)
     try:
-        dm.set_format("sell64")
+        dm.set_format(fmt)
         rep = dm.solve(prec, rtol=1e-10)
         phi = dm.potential()
         assert rep.relres < 1e-10
@@ -189,7 +191,7 @@ def test_sell64_format_solves_agree(lin20, prec):
         if prec == "f64":
             assert abs(rep.iterations - it_ref) <= max(3, 0.05 * it_ref)
     finally:
-        dm.set_format("csr")
+        dm.set_format("sell64")
 
 
 def test_true_residual_of_gpu_solution(lin20):
