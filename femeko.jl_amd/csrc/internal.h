@@ -82,7 +82,10 @@ struct HostBuf {
 // ---------------------------------------------------------------------------
 // state of one PCG run, read by every kernel of the iteration
 struct CgState {
-    int32_t done;       // 1 once converged / stopped: later kernels return at once
+    // stop flags by iteration parity: the kernels of an iteration with parity
+    // c read done_at[c] on entry and only ever write done_at[c ^ 1], so no
+    // kernel reads a flag that a block of the same launch may be writing
+    int32_t done_at[2];
     int32_t status;     // FK_OK, FK_ERR_NOT_CONVERGED, FK_ERR_SINGULAR
     int64_t iters;      // completed iterations
     double thresh;      // rtol^2 * ||b||^2
@@ -126,6 +129,7 @@ struct fk_context {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 256;
+    bool graphs = true;  // capture PCG iteration chunks in hipGraphs (FEMEKO_HIP_GRAPHS=0 disables)
     std::string err;
     fk::TimingEvents timing;
 };

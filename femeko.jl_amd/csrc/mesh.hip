@@ -315,6 +315,8 @@ int fk_context_create(int device, fk_context **out) {
         hipDeviceProp_t prop;
         FK_HIP(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        const char *g = std::getenv("FEMEKO_HIP_GRAPHS");
+        ctx->graphs = !(g && g[0] == '0');
         FK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     } catch (const fk::Error &e) {
         delete ctx;

@@ -62,14 +62,23 @@ __device__ __forceinline__ T csr_row(const int32_t *__restrict__ rowptr, const i
     return s;
 }
 
+// Entry test of an iteration's first kernel: stop if the previous iteration
+// stopped, and carry the flag to the other parity so the iteration after
+// this one sees it too.
+__device__ __forceinline__ bool stopped_entry(CgState *__restrict__ st, int parity) {
+    if (!st->done_at[parity]) return false;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->done_at[parity ^ 1] = 1;
+    return true;
+}
+
 // K1: q = A p, partial p.q
 template <typename T, int LPR>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(const int32_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ col, const T *__restrict__ val,
                                                     const T *__restrict__ p, T *__restrict__ q,
                                                     double *__restrict__ part, int64_t n, int64_t chunk,
-                                                    int nblocks, const CgState *__restrict__ st) {
-    if (st->done) return;
+                                                    int nblocks, CgState *__restrict__ st, int parity) {
+    if (stopped_entry(st, parity)) return;
     __shared__ double sh[kBlock / 64 + 1];
     const int lb = logical_block(nblocks);
     const int64_t r0 = static_cast<int64_t>(lb) * chunk;
@@ -99,16 +108,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *_
                                                          const double *__restrict__ part_rz_cur,
                                                          double *__restrict__ part_rz_next,
                                                          double *__restrict__ part_rr, int64_t n, int64_t chunk,
-                                                         int nblocks, CgState *__restrict__ st) {
-    if (st->done) return;
+                                                         int nblocks, CgState *__restrict__ st, int parity) {
+    if (st->done_at[parity]) return;
     __shared__ double sh[kBlock / 64 + 1];
     const double rho = sum_partials<kBlock>(part_rz_cur, nblocks, sh);
     const double pq = sum_partials<kBlock>(part_pq, nblocks, sh);
     if (!(pq > 0.0) || !isfinite(pq)) {
-        // breakdown: A not positive definite on the Krylov space
+        // breakdown: A not positive definite on the Krylov space (every
+        // block sees the same pq and skips the update)
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->status = FK_ERR_SINGULAR;
-            st->done = 1;
+            st->done_at[parity ^ 1] = 1;
         }
         return;
     }
@@ -141,8 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
                                                         const double *__restrict__ part_rz_next,
                                                         const double *__restrict__ part_rz_cur,
                                                         const double *__restrict__ part_rr, int64_t n,
-                                                        int64_t chunk, int nblocks, CgState *__restrict__ st) {
-    if (st->done) return;
+                                                        int64_t chunk, int nblocks, CgState *__restrict__ st,
+                                                        int parity) {
+    if (st->done_at[parity]) return;
     __shared__ double sh[kBlock / 64 + 1];
     const double rho_new = sum_partials<kBlock>(part_rz_next, nblocks, sh);
     const double rho = sum_partials<kBlock>(part_rz_cur, nblocks, sh);
@@ -152,15 +163,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
             const int64_t it = st->iters + 1;
             st->iters = it;
             st->rr = rr;
+            int stop = 0;
             if (rr < st->thresh) {
-                st->done = 1;
+                stop = 1;
             } else if (!isfinite(rr)) {
                 st->status = FK_ERR_NONFINITE;
-                st->done = 1;
+                stop = 1;
             } else if (it >= st->max_iter) {
                 st->status = FK_ERR_NOT_CONVERGED;
-                st->done = 1;
+                stop = 1;
             }
+            // seen from the next iteration on; every block of this launch
+            // still completes the p update below
+            if (stop) st->done_at[parity ^ 1] = 1;
         }
     }
     const T beta = static_cast<T>(rho_new / rho);
@@ -211,8 +226,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(const int64_t *__restri
                                                          const T *__restrict__ p, T *__restrict__ q,
                                                          double *__restrict__ part, int64_t n, int64_t nslices,
                                                          int64_t chunk_slices, int nblocks,
-                                                         const CgState *__restrict__ st) {
-    if (st->done) return;
+                                                         CgState *__restrict__ st, int parity) {
+    if (stopped_entry(st, parity)) return;
     __shared__ double sh[kBlock / 64 + 1];
     const int lb = logical_block(nblocks);
     const int64_t s0 = static_cast<int64_t>(lb) * chunk_slices;
@@ -302,18 +317,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(const double *__restrict__ 
         st->rr = rr;
         st->status = FK_OK;
         st->max_iter = max_iter;
-        st->done = 0;
+        int stop = 0;
         if (!isfinite(rr)) {
             st->status = FK_ERR_NONFINITE;
-            st->done = 1;
+            stop = 1;
         } else if (rr == 0.0 || rr < st->thresh) {
-            st->done = 1;
+            stop = 1;
         } else if (max_iter <= 0) {
             st->status = FK_ERR_NOT_CONVERGED;
-            st->done = 1;
+            stop = 1;
         }
+        st->done_at[0] = stop;
+        st->done_at[1] = stop;
     }
 }
+
+__host__ __device__ inline bool cg_stopped(const CgState &s) { return (s.done_at[0] | s.done_at[1]) != 0; }
 
 // r = -(A phi) on free rows, 0 on fixed rows; partials of r.r. FP64.
 template <int LPR>
@@ -535,26 +554,26 @@ template <>
 const float *sell_values<float>(fk_mesh *m) { return m->sell_val32.get(); }
 
 template <typename T, int LPR>
-static void launch_spmv(fk_mesh *m, PcgWork<T> &w) {
+static void launch_spmv(fk_mesh *m, PcgWork<T> &w, int parity) {
     k_cg_spmv<T, LPR><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(m->rowptr.get(), m->colidx.get(), values<T>(m),
                                                                  w.p.get(), w.q.get(), w.part_pq.get(), m->nv,
-                                                                 m->chunk, m->nblocks, w.state.get());
+                                                                 m->chunk, m->nblocks, w.state.get(), parity);
 }
 
 template <typename T>
-static void launch_spmv_lanes(fk_mesh *m, PcgWork<T> &w) {
+static void launch_spmv_lanes(fk_mesh *m, PcgWork<T> &w, int parity) {
     if (m->format == 1) {
         k_cg_spmv_sell<T><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(
             m->sell_ptr.get(), m->sell_width.get(), m->sell_col.get(), sell_values<T>(m), w.p.get(), w.q.get(),
-            w.part_pq.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks, w.state.get());
+            w.part_pq.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks, w.state.get(), parity);
         return;
     }
     switch (m->lanes) {
-        case 4: launch_spmv<T, 4>(m, w); break;
-        case 8: launch_spmv<T, 8>(m, w); break;
-        case 32: launch_spmv<T, 32>(m, w); break;
-        case 64: launch_spmv<T, 64>(m, w); break;
-        default: launch_spmv<T, 16>(m, w); break;
+        case 4: launch_spmv<T, 4>(m, w, parity); break;
+        case 8: launch_spmv<T, 8>(m, w, parity); break;
+        case 32: launch_spmv<T, 32>(m, w, parity); break;
+        case 64: launch_spmv<T, 64>(m, w, parity); break;
+        default: launch_spmv<T, 16>(m, w, parity); break;
     }
 }
 
@@ -575,16 +594,16 @@ static void enqueue_iteration(fk_mesh *m, PcgWork<T> &w, int parity, bool timed)
         }
         FK_HIP(hipEventRecord(te.ev[te.used], st));
     }
-    launch_spmv_lanes<T>(m, w);
+    launch_spmv_lanes<T>(m, w, parity);
     if (timed) {
         FK_HIP(hipEventRecord(te.ev[te.used + 1], st));
         te.used += 2;
     }
     k_cg_update_xr<T><<<nb, kBlock, 0, st>>>(w.x.get(), w.r.get(), w.p.get(), w.q.get(), w.dinv.get(),
                                              w.part_pq.get(), rz_cur, rz_next, w.part_rr.get(), m->nv, m->chunk, nb,
-                                             w.state.get());
+                                             w.state.get(), parity);
     k_cg_update_p<T><<<nb, kBlock, 0, st>>>(w.p.get(), w.r.get(), w.dinv.get(), rz_next, rz_cur, w.part_rr.get(),
-                                            m->nv, m->chunk, nb, w.state.get());
+                                            m->nv, m->chunk, nb, w.state.get(), parity);
 }
 
 // collect the first `valid` event pairs recorded since the last harvest
@@ -624,7 +643,18 @@ static CgState run_pcg(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref)
             FK_HIP(hipStreamSynchronize(st));
             harvest_timing(m->ctx, hs->iters - before);
             before = hs->iters;
-            if (hs->done) break;
+            if (cg_stopped(*hs)) break;
+        }
+        return *hs;
+    }
+    if (!m->ctx->graphs) {
+        // direct launches, poll every 32 iterations
+        int64_t it = 0;
+        while (true) {
+            for (int k = 0; k < 32; ++k, ++it) enqueue_iteration<T>(m, w, static_cast<int>(it & 1), false);
+            FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
+            FK_HIP(hipStreamSynchronize(st));
+            if (cg_stopped(*hs)) break;
         }
         return *hs;
     }
@@ -658,7 +688,7 @@ static CgState run_pcg(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref)
             FK_HIP(hipMemcpyAsync(&hs[1 - cur], w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
             FK_HIP(hipEventRecord(ev[1 - cur], st));
             FK_HIP(hipEventSynchronize(ev[cur]));
-            if (hs[cur].done) break;
+            if (cg_stopped(hs[cur])) break;
             cur = 1 - cur;
         }
         FK_HIP(hipStreamSynchronize(st));
@@ -671,6 +701,143 @@ static CgState run_pcg(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref)
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     return out;
+}
+
+// Run exactly G iterations from parity 0 (kernels after a stop return at
+// once), then return the state (host sync). G must be even.
+template <typename T>
+static CgState run_chunk(fk_mesh *m, PcgWork<T> &w, int G) {
+    hipStream_t st = m->ctx->stream;
+    CgState *hs = m->hstate.ptr;
+    const bool timed = m->ctx->timing.enabled;
+    if (timed || !m->ctx->graphs) {
+        FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        const int64_t before = hs->iters;
+        for (int k = 0; k < G; ++k) enqueue_iteration<T>(m, w, k & 1, timed);
+        FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        if (timed) harvest_timing(m->ctx, hs->iters - before);
+        return *hs;
+    }
+    const int cfg = m->lanes + 1000 * m->format;
+    if (!w.graph || w.graph_iters != G || w.graph_lanes != cfg) {
+        w.reset_graph();
+        hipGraph_t graph;
+        FK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < G; ++k) enqueue_iteration<T>(m, w, k & 1, false);
+        FK_HIP(hipStreamEndCapture(st, &graph));
+        hipError_t e = hipGraphInstantiate(&w.graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        FK_HIP(e);
+        w.graph_iters = G;
+        w.graph_lanes = cfg;
+    }
+    FK_HIP(hipGraphLaunch(w.graph, st));
+    FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    return *hs;
+}
+
+static double neg_residual(fk_mesh *m, const double *phi, double *r);
+
+// phi += x / s; x = 0
+template <typename T>
+__global__ void k_rr_fold(double *__restrict__ phi, T *__restrict__ x, double inv_s, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    phi[i] += inv_s * static_cast<double>(x[i]);
+    x[i] = T(0);
+}
+
+// residual replacement: r = s r64 (free rows), partials of r.D^-1.r into the
+// parity-0 slot the next iteration reads as rho
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rr_replace(T *__restrict__ r, const double *__restrict__ r64, double s,
+                                                       const T *__restrict__ dinv, double *__restrict__ part_rz,
+                                                       int64_t n, int64_t chunk, int nblocks) {
+    __shared__ double sh[kBlock / 64 + 1];
+    const int lb = logical_block(nblocks);
+    const int64_t r0 = static_cast<int64_t>(lb) * chunk;
+    const int64_t r1 = min(n, r0 + chunk);
+    double rz = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+        const T d = dinv[i];
+        const T ri = d != T(0) ? static_cast<T>(s * r64[i]) : T(0);
+        r[i] = ri;
+        rz += static_cast<double>(ri) * static_cast<double>(d * ri);
+    }
+    const double t = block_sum<kBlock>(rz, sh);
+    if (threadIdx.x == 0) part_rz[blockIdx.x] = t;
+}
+
+__global__ void k_rr_resume(CgState *__restrict__ st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->done_at[0] = 0;
+        st->done_at[1] = 0;
+        st->status = FK_OK;
+    }
+}
+
+// FP32 Jacobi-PCG whose residual is replaced by the FP64 true residual
+// -mask(A64 phi) every G iterations (phi accumulating the FP32 increments in
+// FP64). The search direction is kept across replacements, so the Krylov
+// space is not restarted and the iteration count stays that of the FP64
+// solve; the replacement bounds the drift of the FP32 recursion. Stops when
+// the FP64 true residual is below rtol ||b||. Returns (iterations,
+// replacements, final ||r||^2).
+static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double bb, int64_t *iters, int32_t *nrep,
+                             double *rr_out) {
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = m->nv;
+    PcgWork<float> &w = m->w32;
+    const int nb = m->nblocks;
+    const double thresh = rtol * rtol * bb;
+    double rr = neg_residual(m, m->phi.get(), m->r64.get());
+    *iters = 0;
+    *nrep = 0;
+    *rr_out = rr;
+    if (rr < thresh) return FK_OK;
+    const double s = 1.0 / std::sqrt(rr);
+    sync_solver_values<float>(m);
+    k_scale_to_f32<<<grid_for(nv, 256), 256, 0, st>>>(m->r64.get(), w.r.get(), s, nv);
+    k_cg_init<float><<<nb, kBlock, 0, st>>>(w.x.get(), w.r.get(), w.p.get(), w.dinv.get(), w.part_rz.get(),
+                                            w.part_bb.get(), nv, m->chunk, nb);
+    // the FP32 recursion alone must not stop the solve: its own threshold is
+    // 10x below the target; convergence is decided on the FP64 residual
+    k_cg_start<<<1, kBlock, 0, st>>>(w.part_bb.get(), nb, -1.0, 0.1 * rtol * std::sqrt(bb) * s, max_iter,
+                                     w.state.get());
+    FK_HIP(hipGetLastError());
+    const int G = nv > 200000 ? 32 : 16;
+    int status = FK_OK;
+    while (true) {
+        const CgState cs = run_chunk<float>(m, w, G);
+        *iters = cs.iters;
+        k_rr_fold<float><<<grid_for(nv, 256), 256, 0, st>>>(m->phi.get(), w.x.get(), 1.0 / s, nv);
+        FK_HIP(hipGetLastError());
+        rr = neg_residual(m, m->phi.get(), m->r64.get());
+        *nrep += 1;
+        *rr_out = rr;
+        if (rr < thresh) break;
+        if (!std::isfinite(rr)) {
+            status = FK_ERR_NONFINITE;
+            break;
+        }
+        if (cs.status == FK_ERR_SINGULAR || cs.status == FK_ERR_NONFINITE) {
+            status = cs.status;
+            break;
+        }
+        if (cs.iters >= max_iter) {
+            status = FK_ERR_NOT_CONVERGED;
+            break;
+        }
+        k_rr_replace<float><<<nb, kBlock, 0, st>>>(w.r.get(), m->r64.get(), s, w.dinv.get(), w.part_rz.get(), nv,
+                                                   m->chunk, nb);
+        k_rr_resume<<<1, 64, 0, st>>>(w.state.get());
+        FK_HIP(hipGetLastError());
+    }
+    FK_HIP(hipStreamSynchronize(st));
+    return status;
 }
 
 template <int LPR>
@@ -759,7 +926,8 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
     m->phi.ensure(nv);
     m->r64.ensure(nv);
     ensure_work<double>(m);
-    if (precision == FK_PREC_F32_REFINED) {
+    const bool mixed = precision == FK_PREC_F32_REFINED || precision == FK_PREC_F32_RESTART;
+    if (mixed) {
         ensure_work<float>(m);
         make_val32(m);
     }
@@ -767,7 +935,7 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
         compute_dinv<double>(m);
         m->dinv64_valid = true;
     }
-    if (precision == FK_PREC_F32_REFINED && !m->dinv32_valid) {
+    if (mixed && !m->dinv32_valid) {
         compute_dinv<float>(m);
         m->dinv32_valid = true;
     }
@@ -795,7 +963,12 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
         nout = 1;
         rel = std::sqrt(s.rr / bb);
         status = s.status;
+    } else if (precision == FK_PREC_F32_REFINED) {
+        double rr = 0.0;
+        status = solve_f32_replace(m, rtol, max_iter, bb, &total, &nout, &rr);
+        rel = std::sqrt(rr / bb);
     } else {
+        // restarted refinement: each FP32 PCG runs from scratch to inner_rtol
         PcgWork<float> &w = m->w32;
         double rr = neg_residual(m, m->phi.get(), m->r64.get());
         const double thresh = rtol * rtol * bb;
@@ -888,9 +1061,11 @@ int fk_set_dirichlet(fk_mesh *m, const uint8_t *mask, const double *g) {
 
 int fk_solve(fk_mesh *m, int32_t precision, double rtol, int64_t max_iter, double inner_rtol, int32_t max_outer,
              int32_t warm_start, int64_t *iterations, int32_t *outer, double *relres) {
-    if (!m || (precision != FK_PREC_F64 && precision != FK_PREC_F32_REFINED) || !(rtol >= 0.0) || max_iter < 0)
+    if (!m ||
+        (precision != FK_PREC_F64 && precision != FK_PREC_F32_REFINED && precision != FK_PREC_F32_RESTART) ||
+        !(rtol >= 0.0) || max_iter < 0)
         return FK_ERR_INVALID_ARG;
-    if (precision == FK_PREC_F32_REFINED && (!(inner_rtol > 0.0) || !(inner_rtol < 1.0) || max_outer < 1))
+    if (precision == FK_PREC_F32_RESTART && (!(inner_rtol > 0.0) || !(inner_rtol < 1.0) || max_outer < 1))
         return FK_ERR_INVALID_ARG;
     return guard(m->ctx, "fk_solve", [&] {
         fk::set_device(m->ctx);

@@ -26,6 +26,7 @@ FK_ERR_UNSUPPORTED = 10
 
 FK_PREC_F64 = 1
 FK_PREC_F32_REFINED = 2
+FK_PREC_F32_RESTART = 3
 FK_ASM_COLOURED = 0
 FK_ASM_ATOMIC = 1
 

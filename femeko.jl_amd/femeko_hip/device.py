@@ -12,6 +12,7 @@ from . import _lib
 from ._lib import FemekoHIPError, dptr, fptr, iptr, uptr
 
 PRECISIONS = {"f64": _lib.FK_PREC_F64, "f32": _lib.FK_PREC_F32_REFINED,
+              "f32-restart": _lib.FK_PREC_F32_RESTART,
               "fp64": _lib.FK_PREC_F64, "f32-refined": _lib.FK_PREC_F32_REFINED}
 METHODS = {"coloured": _lib.FK_ASM_COLOURED, "colored": _lib.FK_ASM_COLOURED, "atomic": _lib.FK_ASM_ATOMIC}
 

@@ -51,7 +51,10 @@ extern "C" {
 
 /* precision of the linear solve */
 #define FK_PREC_F64 1          /* FP64 matrix, FP64 Jacobi-PCG */
-#define FK_PREC_F32_REFINED 2  /* FP32 matrix + inner PCG, FP64 residual refinement */
+#define FK_PREC_F32_REFINED 2  /* FP32 matrix and PCG, FP64 true residual replaced in every
+                                  iteration chunk, FP64 solution (Krylov space kept) */
+#define FK_PREC_F32_RESTART 3  /* FP32 PCG to inner_rtol restarted inside an FP64
+                                  residual-correction loop (classic iterative refinement) */
 
 /* scatter method of the element assembly */
 #define FK_ASM_COLOURED 0      /* mesh-coloured conflict-free adds (deterministic) */
@@ -124,12 +127,16 @@ int fk_set_dirichlet(fk_mesh *mesh, const uint8_t *mask, const double *g);
 /* Jacobi-preconditioned CG (the solver of magnetostatics.cpp:26-38 and
  * run.jl:141) on the assembled matrix, stopping at ||r||_2 < rtol ||b||_2.
  * precision FK_PREC_F64: one FP64 PCG to rtol.
- * precision FK_PREC_F32_REFINED: FP32 PCG to inner_rtol inside an FP64
- * residual/update loop until the true FP64 residual meets rtol or max_outer.
+ * precision FK_PREC_F32_REFINED: FP32 values and PCG recursion; every 16-32
+ * iterations the FP32 increments are folded into the FP64 potential and the
+ * residual is replaced by the FP64 true residual -mask(A64 phi), keeping the
+ * search direction; stops on the FP64 residual (inner_rtol, max_outer unused).
+ * precision FK_PREC_F32_RESTART: FP32 PCG to inner_rtol restarted inside an
+ * FP64 residual/update loop (at most max_outer restarts).
  * warm_start != 0 starts from the resident potential. The potential stays
  * on the GPU (fk_potential copies it out).
- * iterations: total PCG iterations; outer: refinement steps; relres: final
- * ||r||/||b|| (FP64 true residual for the refined solve). */
+ * iterations: total PCG iterations; outer: residual replacements or restarts;
+ * relres: final ||r||/||b|| (FP64 true residual for the mixed solves). */
 int fk_solve(fk_mesh *mesh, int32_t precision, double rtol, int64_t max_iter, double inner_rtol,
              int32_t max_outer, int32_t warm_start, int64_t *iterations, int32_t *outer,
              double *relres);

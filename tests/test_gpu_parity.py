@@ -176,7 +176,7 @@ def test_solve_f32_refined_linear20(lin20):
 
 
 @pytest.mark.parametrize("fmt", ["csr", "sell64"])
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32-restart"])
 def test_spmv_formats_solve_agree(lin20, prec, fmt):
     # the solver's CSR kernels and its SELL-64 copy give the same solution
     pr, A, dm = lin20
