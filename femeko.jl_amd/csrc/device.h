@@ -40,6 +40,12 @@ __device__ __forceinline__ Grad element_gradients(const double *__restrict__ X, 
     return G;
 }
 
+// 16 bytes of T for one vector load / store
+template <typename T>
+struct alignas(16) Vec16 {
+    T v[16 / sizeof(T)];
+};
+
 __device__ __forceinline__ double dot3(const double *a, const double *b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }

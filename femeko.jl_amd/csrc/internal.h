@@ -178,6 +178,9 @@ struct fk_mesh {
     int64_t nslices = 0, sell_len = 0, sell_chunk = 0;
     fk::DevBuf<int64_t> sell_ptr;
     fk::DevBuf<int32_t> sell_width, sell_col, sell_map;
+    fk::DevBuf<int16_t> sell_dcol;    // int16 column offsets from the row (when sell_short)
+    bool sell_short = false, allow_short = true;
+    int32_t bandwidth = 0;
     fk::DevBuf<double> sell_val64;
     fk::DevBuf<float> sell_val32;
 

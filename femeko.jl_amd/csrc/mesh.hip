@@ -129,7 +129,8 @@ void setup_partition(fk_mesh *m) {
     int64_t nb = std::min<int64_t>(want, 4 * static_cast<int64_t>(m->ctx->num_cus));
     if (nb >= kNumXcd) nb = (nb / kNumXcd) * kNumXcd;
     m->nblocks = static_cast<int>(nb);
-    m->chunk = (m->nv + nb - 1) / nb;
+    // bands of whole 64-row groups: 16-byte aligned vector access in every band
+    m->chunk = ((m->nv + nb - 1) / nb + 63) / 64 * 64;
 }
 
 void build_mesh(fk_mesh *m, const double *p, const int32_t *t, int32_t base, const double *VE) {

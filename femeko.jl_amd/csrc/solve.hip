@@ -124,10 +124,31 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *_
     }
     const T alpha = static_cast<T>(rho / pq);
     const int lb = logical_block(nblocks);
-    const int64_t r0 = static_cast<int64_t>(lb) * chunk;
+    const int64_t r0 = static_cast<int64_t>(lb) * chunk;  // chunk is a multiple of 64
     const int64_t r1 = min(n, r0 + chunk);
     double rz = 0.0, rr = 0.0;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+    // 16-byte vector loads and stores over the band, scalar tail at n
+    constexpr int VW = 16 / sizeof(T);
+    using V = Vec16<T>;
+    const int64_t v1 = r1 > r0 ? r1 / VW : r0 / VW;
+    for (int64_t j = r0 / VW + threadIdx.x; j < v1; j += kBlock) {
+        const V D = reinterpret_cast<const V *>(dinv)[j];
+        const V P = reinterpret_cast<const V *>(p)[j];
+        const V Q = reinterpret_cast<const V *>(q)[j];
+        V X = reinterpret_cast<const V *>(x)[j];
+        V R = reinterpret_cast<const V *>(r)[j];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            X.v[e] += alpha * P.v[e];
+            const T ri = D.v[e] != T(0) ? R.v[e] - alpha * Q.v[e] : T(0);
+            R.v[e] = ri;
+            rz += static_cast<double>(ri) * static_cast<double>(D.v[e] * ri);
+            rr += static_cast<double>(ri) * static_cast<double>(ri);
+        }
+        reinterpret_cast<V *>(x)[j] = X;
+        reinterpret_cast<V *>(r)[j] = R;
+    }
+    for (int64_t i = max(r0, v1 * VW) + threadIdx.x; i < r1; i += kBlock) {
         const T d = dinv[i];
         x[i] += alpha * p[i];
         const T ri = d != T(0) ? r[i] - alpha * q[i] : T(0);
@@ -180,14 +201,31 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
     }
     const T beta = static_cast<T>(rho_new / rho);
     const int lb = logical_block(nblocks);
-    const int64_t r0 = static_cast<int64_t>(lb) * chunk;
+    const int64_t r0 = static_cast<int64_t>(lb) * chunk;  // chunk is a multiple of 64
     const int64_t r1 = min(n, r0 + chunk);
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) p[i] = dinv[i] * r[i] + beta * p[i];
+    constexpr int VW = 16 / sizeof(T);
+    using V = Vec16<T>;
+    const int64_t v1 = r1 > r0 ? r1 / VW : r0 / VW;
+    for (int64_t j = r0 / VW + threadIdx.x; j < v1; j += kBlock) {
+        const V D = reinterpret_cast<const V *>(dinv)[j];
+        const V R = reinterpret_cast<const V *>(r)[j];
+        V P = reinterpret_cast<const V *>(p)[j];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) P.v[e] = D.v[e] * R.v[e] + beta * P.v[e];
+        reinterpret_cast<V *>(p)[j] = P;
+    }
+    for (int64_t i = max(r0, v1 * VW) + threadIdx.x; i < r1; i += kBlock) p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
-// ---- SELL-64: slices of 64 rows (one wavefront), entry k of lane l of slice
-// s at sell_ptr[s] + 64 k + l; padding entries carry value 0 and column =
-// their own row. Every matrix load of a wave is 64 consecutive elements.
+// ---- SELL-64x4: slices of 64 rows (one wavefront). Slice s with width W
+// (its longest row) holds 64 W entries from sell_ptr[s]: the first
+// 4 floor(W/4) entries of every row as groups of 4 consecutive entries per
+// lane (group g of lane l at 4 (64 g + l)), so each lane reads 4 values and 4
+// column indices with one 16-byte load, then the remaining W mod 4 entries
+// one per lane (entry t of lane l at 256 floor(W/4) + 64 t + l). Padding
+// entries carry value 0 and a column inside the row's own slice. Column
+// indices are int32, or int16 offsets from the row when the matrix bandwidth
+// allows (every column within 32767 of its row).
 template <typename T>
 __global__ void k_sell_gather(const T *__restrict__ src, const int32_t *__restrict__ map, int64_t n,
                               T *__restrict__ dst) {
@@ -198,31 +236,44 @@ __global__ void k_sell_gather(const T *__restrict__ src, const int32_t *__restri
 }
 
 template <typename T>
+struct alignas(4 * sizeof(T)) Quad {
+    T v[4];
+};
+
+__device__ __forceinline__ int64_t col_of(int32_t c, int64_t) { return c; }
+__device__ __forceinline__ int64_t col_of(int16_t d, int64_t row) { return row + d; }
+
+template <typename T, typename IDX>
 __device__ __forceinline__ T sell_row(const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
-                                      const int32_t *__restrict__ col, const T *__restrict__ val,
-                                      const T *__restrict__ x, int64_t s, int lane) {
-    const int64_t base = sptr[s] + lane;
+                                      const IDX *__restrict__ col, const T *__restrict__ val,
+                                      const T *__restrict__ x, int64_t s, int lane, int64_t row) {
+    const int64_t base = sptr[s];
     const int W = width[s];
-    T acc0 = T(0), acc1 = T(0);
-    int k = 0;
-    for (; k + 1 < W; k += 2) {
-        const int64_t o0 = base + 64 * static_cast<int64_t>(k);
-        const int64_t o1 = o0 + 64;
-        acc0 += val[o0] * x[col[o0]];
-        acc1 += val[o1] * x[col[o1]];
+    const int ng = W >> 2;
+    const Quad<T> *v4 = reinterpret_cast<const Quad<T> *>(val + base) + lane;
+    const Quad<IDX> *c4 = reinterpret_cast<const Quad<IDX> *>(col + base) + lane;
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+    for (int g = 0; g < ng; ++g) {
+        const Quad<T> V = v4[64 * g];
+        const Quad<IDX> C = c4[64 * g];
+        a0 += V.v[0] * x[col_of(C.v[0], row)];
+        a1 += V.v[1] * x[col_of(C.v[1], row)];
+        a2 += V.v[2] * x[col_of(C.v[2], row)];
+        a3 += V.v[3] * x[col_of(C.v[3], row)];
     }
-    if (k < W) {
-        const int64_t o0 = base + 64 * static_cast<int64_t>(k);
-        acc0 += val[o0] * x[col[o0]];
-    }
-    return acc0 + acc1;
+    const int64_t tb = base + 256 * static_cast<int64_t>(ng) + lane;
+    const int tail = W - 4 * ng;
+    if (tail > 0) a0 += val[tb] * x[col_of(col[tb], row)];
+    if (tail > 1) a1 += val[tb + 64] * x[col_of(col[tb + 64], row)];
+    if (tail > 2) a2 += val[tb + 128] * x[col_of(col[tb + 128], row)];
+    return (a0 + a1) + (a2 + a3);
 }
 
 // K1 in SELL form: q = A p, partial p.q; each wave owns whole slices
-template <typename T>
+template <typename T, typename IDX>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(const int64_t *__restrict__ sptr,
                                                          const int32_t *__restrict__ width,
-                                                         const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                         const IDX *__restrict__ col, const T *__restrict__ val,
                                                          const T *__restrict__ p, T *__restrict__ q,
                                                          double *__restrict__ part, int64_t n, int64_t nslices,
                                                          int64_t chunk_slices, int nblocks,
@@ -235,8 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(const int64_t *__restri
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double acc = 0.0;
     for (int64_t s = s0 + wv; s < s1; s += kBlock / 64) {
-        const T y = sell_row<T>(sptr, width, col, val, p, s, lane);
         const int64_t row = s * 64 + lane;
+        const T y = sell_row<T, IDX>(sptr, width, col, val, p, s, lane, row);
         if (row < n) {
             q[row] = y;
             acc += static_cast<double>(p[row]) * static_cast<double>(y);
@@ -247,9 +298,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(const int64_t *__restri
 }
 
 // FP64 r = -mask(A phi), SELL form
+template <typename IDX>
 __global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const int64_t *__restrict__ sptr,
                                                               const int32_t *__restrict__ width,
-                                                              const int32_t *__restrict__ col,
+                                                              const IDX *__restrict__ col,
                                                               const double *__restrict__ val,
                                                               const double *__restrict__ phi,
                                                               const uint8_t *__restrict__ mask,
@@ -263,8 +315,8 @@ __global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const int64_t *__r
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double acc = 0.0;
     for (int64_t s = s0 + wv; s < s1; s += kBlock / 64) {
-        const double y = sell_row<double>(sptr, width, col, val, phi, s, lane);
         const int64_t row = s * 64 + lane;
+        const double y = sell_row<double, IDX>(sptr, width, col, val, phi, s, lane, row);
         if (row < n) {
             const double v = mask[row] ? 0.0 : -y;
             r[row] = v;
@@ -463,29 +515,40 @@ static void compute_dinv(fk_mesh *m) {
     if (h) fail(FK_ERR_SINGULAR, std::to_string(h) + " free node(s) without a positive diagonal");
 }
 
-// fill the SELL-64 columns and the SELL -> CSR position map
+// fill the SELL-64x4 columns (int32 or int16 row offsets) and the SELL -> CSR
+// position map
 __global__ void k_sell_fill(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
                             const int64_t *__restrict__ sptr, const int32_t *__restrict__ width, int64_t nv,
-                            int64_t nrows_padded, int32_t *__restrict__ scol, int32_t *__restrict__ smap) {
+                            int64_t nrows_padded, int32_t *__restrict__ scol, int16_t *__restrict__ sdcol,
+                            int32_t *__restrict__ smap) {
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= nrows_padded) return;
     const int64_t s = r >> 6;
     const int lane = static_cast<int>(r & 63);
     const int W = width[s];
-    const int64_t base = sptr[s] + lane;
+    const int ng = W >> 2;
+    const int64_t base = sptr[s];
     const int32_t a = r < nv ? rowptr[r] : 0;
     const int32_t len = r < nv ? rowptr[r + 1] - a : 0;
     const int32_t self = static_cast<int32_t>(r < nv ? r : nv - 1);
     for (int k = 0; k < W; ++k) {
-        const int64_t o = base + 64 * static_cast<int64_t>(k);
-        if (k < len) {
-            scol[o] = colidx[a + k];
-            smap[o] = a + k;
-        } else {
-            scol[o] = self;
-            smap[o] = -1;
-        }
+        const int64_t o = k < 4 * ng ? base + 4 * (64 * static_cast<int64_t>(k >> 2) + lane) + (k & 3)
+                                     : base + 256 * static_cast<int64_t>(ng) + 64 * (k - 4 * ng) + lane;
+        const int32_t c = k < len ? colidx[a + k] : self;
+        if (scol) scol[o] = c;
+        if (sdcol) sdcol[o] = static_cast<int16_t>(c - r);
+        smap[o] = k < len ? a + k : -1;
     }
+}
+
+// largest |col - row| over the CSR pattern
+__global__ void k_bandwidth(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, int64_t nv,
+                            int32_t *__restrict__ out) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= nv) return;
+    int32_t b = 0;
+    for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) b = max(b, abs(colidx[k] - static_cast<int32_t>(r)));
+    atomicMax(out, b);
 }
 
 static void build_sell(fk_mesh *m) {
@@ -507,15 +570,32 @@ static void build_sell(fk_mesh *m) {
     m->nslices = ns;
     m->sell_len = sptr[ns];
     m->sell_chunk = (ns + m->nblocks - 1) / m->nblocks;
+    // int16 row offsets when every column lies within 32767 of its row
+    {
+        DevBuf<int32_t> bw(1);
+        FK_HIP(hipMemsetAsync(bw.get(), 0, sizeof(int32_t), st));
+        k_bandwidth<<<grid_for(nv, 256), 256, 0, st>>>(m->rowptr.get(), m->colidx.get(), nv, bw.get());
+        FK_HIP(hipGetLastError());
+        int32_t h = 0;
+        FK_HIP(hipMemcpyAsync(&h, bw.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        m->bandwidth = h;
+        m->sell_short = m->allow_short && h + 64 <= 32767;
+    }
     m->sell_ptr.alloc(ns + 1);
     m->sell_width.alloc(ns);
-    m->sell_col.alloc(m->sell_len);
+    m->sell_col.release();
+    m->sell_dcol.release();
+    if (m->sell_short)
+        m->sell_dcol.alloc(m->sell_len);
+    else
+        m->sell_col.alloc(m->sell_len);
     m->sell_map.alloc(m->sell_len);
     FK_HIP(hipMemcpyAsync(m->sell_ptr.get(), sptr.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
     FK_HIP(hipMemcpyAsync(m->sell_width.get(), width.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, st));
     k_sell_fill<<<grid_for(ns * 64, 256), 256, 0, st>>>(m->rowptr.get(), m->colidx.get(), m->sell_ptr.get(),
                                                         m->sell_width.get(), nv, ns * 64, m->sell_col.get(),
-                                                        m->sell_map.get());
+                                                        m->sell_dcol.get(), m->sell_map.get());
     FK_HIP(hipGetLastError());
     FK_HIP(hipStreamSynchronize(st));
     m->have_sell = true;
@@ -563,9 +643,14 @@ static void launch_spmv(fk_mesh *m, PcgWork<T> &w, int parity) {
 template <typename T>
 static void launch_spmv_lanes(fk_mesh *m, PcgWork<T> &w, int parity) {
     if (m->format == 1) {
-        k_cg_spmv_sell<T><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(
-            m->sell_ptr.get(), m->sell_width.get(), m->sell_col.get(), sell_values<T>(m), w.p.get(), w.q.get(),
-            w.part_pq.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks, w.state.get(), parity);
+        if (m->sell_short)
+            k_cg_spmv_sell<T, int16_t><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(
+                m->sell_ptr.get(), m->sell_width.get(), m->sell_dcol.get(), sell_values<T>(m), w.p.get(), w.q.get(),
+                w.part_pq.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks, w.state.get(), parity);
+        else
+            k_cg_spmv_sell<T, int32_t><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(
+                m->sell_ptr.get(), m->sell_width.get(), m->sell_col.get(), sell_values<T>(m), w.p.get(), w.q.get(),
+                w.part_pq.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks, w.state.get(), parity);
         return;
     }
     switch (m->lanes) {
@@ -780,14 +865,15 @@ __global__ void k_rr_resume(CgState *__restrict__ st) {
 }
 
 // FP32 Jacobi-PCG whose residual is replaced by the FP64 true residual
-// -mask(A64 phi) every G iterations (phi accumulating the FP32 increments in
-// FP64). The search direction is kept across replacements, so the Krylov
-// space is not restarted and the iteration count stays that of the FP64
-// solve; the replacement bounds the drift of the FP32 recursion. Stops when
-// the FP64 true residual is below rtol ||b||. Returns (iterations,
-// replacements, final ||r||^2).
-static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double bb, int64_t *iters, int32_t *nrep,
-                             double *rr_out) {
+// -mask(A64 phi), phi accumulating the FP32 increments in FP64 ("reliable
+// updates"). A replacement happens at the end of an iteration chunk once the
+// FP32 recursive residual has fallen by the factor `delta` since the last
+// replacement, or once it claims convergence; the search direction is kept,
+// so the Krylov space is not restarted. Convergence is decided on the FP64
+// true residual: ||r|| < rtol ||b||. Returns (iterations, replacements,
+// final ||r||^2).
+static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double bb, double delta, int64_t *iters,
+                             int32_t *nrep, double *rr_out) {
     hipStream_t st = m->ctx->stream;
     const int64_t nv = m->nv;
     PcgWork<float> &w = m->w32;
@@ -809,15 +895,21 @@ static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double b
                                      w.state.get());
     FK_HIP(hipGetLastError());
     const int G = nv > 200000 ? 32 : 16;
+    const double target2 = rtol * rtol * bb * s * s;  // the FP64 target in scaled units
+    double rr_last = 1.0;                              // scaled ||r||^2 at the last replacement
     int status = FK_OK;
     while (true) {
         const CgState cs = run_chunk<float>(m, w, G);
         *iters = cs.iters;
+        const bool need = cg_stopped(cs) || cs.rr < delta * delta * rr_last || cs.rr < target2 ||
+                          cs.iters >= max_iter;
+        if (!need) continue;
         k_rr_fold<float><<<grid_for(nv, 256), 256, 0, st>>>(m->phi.get(), w.x.get(), 1.0 / s, nv);
         FK_HIP(hipGetLastError());
         rr = neg_residual(m, m->phi.get(), m->r64.get());
         *nrep += 1;
         *rr_out = rr;
+        rr_last = rr * s * s;
         if (rr < thresh) break;
         if (!std::isfinite(rr)) {
             status = FK_ERR_NONFINITE;
@@ -853,10 +945,14 @@ static double neg_residual(fk_mesh *m, const double *phi, double *r) {
     m->part_misc.ensure(m->nblocks);
     if (m->format == 1) {
         sync_solver_values<double>(m);
-        k_neg_residual_sell<<<m->nblocks, kBlock, 0, st>>>(m->sell_ptr.get(), m->sell_width.get(), m->sell_col.get(),
-                                                           m->sell_val64.get(), phi, m->mask.get(), r,
-                                                           m->part_misc.get(), m->nv, m->nslices, m->sell_chunk,
-                                                           m->nblocks);
+        if (m->sell_short)
+            k_neg_residual_sell<int16_t><<<m->nblocks, kBlock, 0, st>>>(
+                m->sell_ptr.get(), m->sell_width.get(), m->sell_dcol.get(), m->sell_val64.get(), phi, m->mask.get(),
+                r, m->part_misc.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks);
+        else
+            k_neg_residual_sell<int32_t><<<m->nblocks, kBlock, 0, st>>>(
+                m->sell_ptr.get(), m->sell_width.get(), m->sell_col.get(), m->sell_val64.get(), phi, m->mask.get(),
+                r, m->part_misc.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks);
     } else switch (m->lanes) {
         case 4: launch_neg_residual<4>(m, phi, r, m->part_misc.get()); break;
         case 8: launch_neg_residual<8>(m, phi, r, m->part_misc.get()); break;
@@ -965,7 +1061,10 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
         status = s.status;
     } else if (precision == FK_PREC_F32_REFINED) {
         double rr = 0.0;
-        status = solve_f32_replace(m, rtol, max_iter, bb, &total, &nout, &rr);
+        // inner_rtol is the reliable-update trigger: replace once the FP32
+        // residual fell by this factor (values outside (0,1) mean 0.1)
+        const double delta = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 0.1;
+        status = solve_f32_replace(m, rtol, max_iter, bb, delta, &total, &nout, &rr);
         rel = std::sqrt(rr / bb);
     } else {
         // restarted refinement: each FP32 PCG runs from scratch to inner_rtol

@@ -127,10 +127,12 @@ int fk_set_dirichlet(fk_mesh *mesh, const uint8_t *mask, const double *g);
 /* Jacobi-preconditioned CG (the solver of magnetostatics.cpp:26-38 and
  * run.jl:141) on the assembled matrix, stopping at ||r||_2 < rtol ||b||_2.
  * precision FK_PREC_F64: one FP64 PCG to rtol.
- * precision FK_PREC_F32_REFINED: FP32 values and PCG recursion; every 16-32
- * iterations the FP32 increments are folded into the FP64 potential and the
- * residual is replaced by the FP64 true residual -mask(A64 phi), keeping the
- * search direction; stops on the FP64 residual (inner_rtol, max_outer unused).
+ * precision FK_PREC_F32_REFINED: FP32 values and PCG recursion; once the
+ * FP32 recursive residual has fallen by the factor inner_rtol (0 < inner_rtol
+ * < 1, e.g. 0.1) since the last update, the FP32 increments are folded into
+ * the FP64 potential and the residual is replaced by the FP64 true residual
+ * -mask(A64 phi), keeping the search direction ("reliable updates"); stops
+ * on the FP64 residual (max_outer unused).
  * precision FK_PREC_F32_RESTART: FP32 PCG to inner_rtol restarted inside an
  * FP64 residual/update loop (at most max_outer restarts).
  * warm_start != 0 starts from the resident potential. The potential stays
