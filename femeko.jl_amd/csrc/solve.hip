@@ -20,6 +20,8 @@
 // return at once.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "device.h"
 #include "internal.h"
@@ -889,42 +891,84 @@ static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double b
     k_scale_to_f32<<<grid_for(nv, 256), 256, 0, st>>>(m->r64.get(), w.r.get(), s, nv);
     k_cg_init<float><<<nb, kBlock, 0, st>>>(w.x.get(), w.r.get(), w.p.get(), w.dinv.get(), w.part_rz.get(),
                                             w.part_bb.get(), nv, m->chunk, nb);
-    // the FP32 recursion alone must not stop the solve: its own threshold is
-    // 10x below the target; convergence is decided on the FP64 residual
-    k_cg_start<<<1, kBlock, 0, st>>>(w.part_bb.get(), nb, -1.0, 0.1 * rtol * std::sqrt(bb) * s, max_iter,
+    // the FP32 recursion stops (mid-chunk) once it reaches the target; the
+    // FP64 true residual then decides, and the solve resumes if it is short
+    k_cg_start<<<1, kBlock, 0, st>>>(w.part_bb.get(), nb, -1.0, rtol * std::sqrt(bb) * s, max_iter,
                                      w.state.get());
     FK_HIP(hipGetLastError());
     const int G = nv > 200000 ? 32 : 16;
     const double target2 = rtol * rtol * bb * s * s;  // the FP64 target in scaled units
     double rr_last = 1.0;                              // scaled ||r||^2 at the last replacement
     int status = FK_OK;
+    const char *tr = std::getenv("FEMEKO_HIP_TRACE");
+    const bool trace = tr && tr[0] == '1';
+    constexpr int kMaxRestarts = 32;
+    int restarts = 0;
+    // r32 = s r64, x32 = 0, p = D^-1 r32 (rho into the parity-0 slot)
+    auto restart_direction = [&] {
+        k_scale_to_f32<<<grid_for(nv, 256), 256, 0, st>>>(m->r64.get(), w.r.get(), s, nv);
+        k_cg_init<float><<<nb, kBlock, 0, st>>>(w.x.get(), w.r.get(), w.p.get(), w.dinv.get(), w.part_rz.get(),
+                                                w.part_bb.get(), nv, m->chunk, nb);
+        FK_HIP(hipGetLastError());
+    };
     while (true) {
         const CgState cs = run_chunk<float>(m, w, G);
         *iters = cs.iters;
         const bool need = cg_stopped(cs) || cs.rr < delta * delta * rr_last || cs.rr < target2 ||
                           cs.iters >= max_iter;
+        if (trace)
+            std::fprintf(stderr, "[fk trace] its %lld status %d stopped %d rr32 %.3e rr_last %.3e replace %d\n",
+                         static_cast<long long>(cs.iters), cs.status, cg_stopped(cs) ? 1 : 0, cs.rr, rr_last,
+                         need ? 1 : 0);
         if (!need) continue;
+        const bool broken = cs.status == FK_ERR_SINGULAR || cs.status == FK_ERR_NONFINITE || !std::isfinite(cs.rr);
+        if (broken) {
+            // the FP32 recursion broke down since the last update: drop its
+            // increment (phi and r64 still hold the last FP64 state) and
+            // restart the direction from that residual
+            if (++restarts > kMaxRestarts) {
+                status = cs.status != FK_OK ? cs.status : FK_ERR_NONFINITE;
+                break;
+            }
+            if (cs.iters >= max_iter) {
+                status = FK_ERR_NOT_CONVERGED;
+                break;
+            }
+            restart_direction();
+            if (trace) std::fprintf(stderr, "[fk trace] FP32 breakdown: increment dropped, direction restarted\n");
+            k_rr_resume<<<1, 64, 0, st>>>(w.state.get());
+            FK_HIP(hipGetLastError());
+            continue;
+        }
         k_rr_fold<float><<<grid_for(nv, 256), 256, 0, st>>>(m->phi.get(), w.x.get(), 1.0 / s, nv);
         FK_HIP(hipGetLastError());
         rr = neg_residual(m, m->phi.get(), m->r64.get());
         *nrep += 1;
         *rr_out = rr;
         rr_last = rr * s * s;
+        if (trace) std::fprintf(stderr, "[fk trace] replacement: true rr (scaled) %.3e\n", rr_last);
         if (rr < thresh) break;
         if (!std::isfinite(rr)) {
             status = FK_ERR_NONFINITE;
-            break;
-        }
-        if (cs.status == FK_ERR_SINGULAR || cs.status == FK_ERR_NONFINITE) {
-            status = cs.status;
             break;
         }
         if (cs.iters >= max_iter) {
             status = FK_ERR_NOT_CONVERGED;
             break;
         }
-        k_rr_replace<float><<<nb, kBlock, 0, st>>>(w.r.get(), m->r64.get(), s, w.dinv.get(), w.part_rz.get(), nv,
-                                                   m->chunk, nb);
+        if (rr_last > 4.0 * cs.rr) {
+            // the recursion drifted below the true residual: the search
+            // direction no longer matches it, restart from p = D^-1 r
+            if (++restarts > kMaxRestarts) {
+                status = FK_ERR_NOT_CONVERGED;
+                break;
+            }
+            restart_direction();
+            if (trace) std::fprintf(stderr, "[fk trace] direction restarted\n");
+        } else {
+            k_rr_replace<float><<<nb, kBlock, 0, st>>>(w.r.get(), m->r64.get(), s, w.dinv.get(), w.part_rz.get(),
+                                                       nv, m->chunk, nb);
+        }
         k_rr_resume<<<1, 64, 0, st>>>(w.state.get());
         FK_HIP(hipGetLastError());
     }
@@ -1062,8 +1106,8 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
     } else if (precision == FK_PREC_F32_REFINED) {
         double rr = 0.0;
         // inner_rtol is the reliable-update trigger: replace once the FP32
-        // residual fell by this factor (values outside (0,1) mean 0.1)
-        const double delta = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 0.1;
+        // residual fell by this factor (values outside (0,1) mean 0.5)
+        const double delta = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 0.5;
         status = solve_f32_replace(m, rtol, max_iter, bb, delta, &total, &nout, &rr);
         rel = std::sqrt(rr / bb);
     } else {

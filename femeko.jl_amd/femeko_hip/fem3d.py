@@ -110,7 +110,7 @@ def magneticField(mesh, u) -> np.ndarray:
 
 
 def solveDirichlet(mesh, mu, mask, g, rtol: float = 1e-10, precision: str = "f64",
-                   inner_rtol: float = 1e-5, max_iter: int = 1_000_000):
+                   inner_rtol: float | None = None, max_iter: int = 1_000_000):
     """Assemble A(mu), fix phi = g on mask, Jacobi-PCG for the rest.
     Returns (u, SolveReport)."""
     dm = device_mesh(mesh)

@@ -129,7 +129,7 @@ int fk_set_dirichlet(fk_mesh *mesh, const uint8_t *mask, const double *g);
  * precision FK_PREC_F64: one FP64 PCG to rtol.
  * precision FK_PREC_F32_REFINED: FP32 values and PCG recursion; once the
  * FP32 recursive residual has fallen by the factor inner_rtol (0 < inner_rtol
- * < 1, e.g. 0.1) since the last update, the FP32 increments are folded into
+ * < 1, e.g. 0.5) since the last update, the FP32 increments are folded into
  * the FP64 potential and the residual is replaced by the FP64 true residual
  * -mask(A64 phi), keeping the search direction ("reliable updates"); stops
  * on the FP64 residual (max_outer unused).

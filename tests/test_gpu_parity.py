@@ -165,10 +165,13 @@ def test_solve_f64_linear20(oracle, lin20):
     assert np.array_equal(phi[pr.mask == 1], pr.g[pr.mask == 1])
 
 
-def test_solve_f32_refined_linear20(lin20):
+@pytest.mark.parametrize("delta", [0.1, 1e-5])
+def test_solve_f32_refined_linear20(lin20, delta):
+    # delta = 1e-5 lets the FP32 recursion drift far below the true residual
+    # between updates; the direction restart keeps it convergent
     pr, A, dm = lin20
     golden = np.load(GOLDEN)
-    rep = dm.solve("f32", rtol=1e-10, inner_rtol=1e-5)
+    rep = dm.solve("f32", rtol=1e-10, inner_rtol=delta)
     phi = dm.potential()
     ref = golden["phi"]
     assert rep.relres < 1e-10 and rep.outer >= 2
