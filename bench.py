@@ -4,17 +4,20 @@ Workload (the mixed-precision configuration, the largest that the headline
 metric is quoted on): Kuhn box mesh n = 160 (24,576,000 tets, 4,173,281 nodes,
 61,979,041 CSR entries), interior jitter 0.15 h, 8 seeded boxes with
 log-uniform mu_r in [1, 1e3], Dirichlet phi = -x on the outer faces. A step is
-value-only assembly of A(mu) on the GPU plus the FP32 Jacobi-PCG (inner rtol
-1e-5) with FP64 residual refinement to rtol 1e-10; mesh, pattern, mu and the
-Dirichlet data are resident in HBM before timing starts.
+value-only assembly of A(mu) on the GPU plus the FP32 Jacobi-PCG whose
+recursive residual is replaced by the FP64 true residual b - A64 phi each time
+it has fallen by --delta (reliable updates), to FP64 relres 1e-10; mesh,
+pattern, mu and the Dirichlet data are resident in HBM before timing starts.
 
 Reported beside the headline: the plain FP64 GPU solve of the same system
-(speed-up), the error of the refined potential, |H| and energy against an
-FP64 solve at rtol 1e-13, the SpMV kernel's achieved bandwidth (HIP events
-around every SpMV launch of an instrumented solve), and the CPU oracle timed on
-a bounded sample of the same workload.
+(speed-up), the classic restarted refinement (FP32 PCG to 1e-5 inside an FP64
+residual-correction loop), the error of the refined potential, |H| and energy
+against an FP64 solve at rtol 1e-13, the SpMV kernel's achieved bandwidth (HIP
+events around every SpMV launch of an instrumented solve), and the CPU oracle
+timed on a bounded sample of the same workload.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160] [--no-cpu]
+                [--shape 1024,1] [--delta 0.5] [--format sell64|sell64-rowidx|csr] [--workload boxes|batch|froehlich|adjoint]
 """
 import argparse
 import json
@@ -82,6 +85,25 @@ def cpu_baseline(pr, iters_f64, nthreads, budget_s=20.0):
         "t_assembly_s": t_asm,
         "t_iteration_ms": t_it * 1e3,
     }
+
+
+def _pmc_traffic(args):
+    """HBM bytes per SELL SpMV launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by what the same counters read
+    for the streaming kernel k_rr_fold of known traffic), when one exists for
+    this workload and launch shape."""
+    path = os.path.join(ROOT, "profiles", "pmc_spmv_sell_f32_boxes160.txt")
+    if args.n != 160 or args.format != "sell64" or not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            if ":" in line and line.split(":")[0].strip() in ("traffic_bytes_per_launch", "shape"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    if vals.get("shape") != args.shape or "traffic_bytes_per_launch" not in vals:
+        return None
+    return float(vals["traffic_bytes_per_launch"])
 
 
 def _common(args, metric, value, unit, ms, dtype, config, higher=True):
@@ -202,9 +224,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=160)
     ap.add_argument("--lanes", type=int, default=16)
-    ap.add_argument("--format", default="sell64", choices=["csr", "sell64"])
+    ap.add_argument("--format", default="sell64", choices=["csr", "sell64", "sell64-rowidx"])
     ap.add_argument("--delta", type=float, default=0.5,
                     help="FP32 solve: residual drop that triggers an FP64 residual replacement")
+    ap.add_argument("--shape", default="1024,1",
+                    help="SELL SpMV launch shape: threads per block, slices in flight per wave")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="boxes", choices=["boxes", "batch", "froehlich", "adjoint"],
                     help="boxes: the headline n=160 mixed-precision line (default); the others measure the "
@@ -231,7 +255,12 @@ def main():
     dm = DeviceMesh(m, ctx)
     dm.set_lanes(args.lanes)
     dm.set_format(args.format)
+    shape = tuple(int(v) for v in args.shape.split(","))
+    dm.set_sell_shape(*shape)
     log(f"device mesh: nnz={dm.nnz} colours={dm.ncolours} ({time.perf_counter()-t0:.1f} s)")
+    sell = dm.sell_info() if args.format != "csr" else None
+    if sell:
+        log(f"SELL copy: {sell[0]} slices, {sell[1]} with shared offsets, {sell[2]} stored entries")
     dm.set_coefficient(pr.mu)
     dm.set_dirichlet(pr.mask, pr.g)
     free = int((pr.mask == 0).sum())
@@ -311,10 +340,14 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"boxes{args.n}: Kuhn box mesh n={args.n}, jitter 0.15h, 8 log-uniform mu_r boxes in "
-                         f"[1,1e3], Dirichlet phi=-x; step = value assembly + FP32 Jacobi-PCG (inner rtol 1e-5) "
-                         f"with FP64 refinement to rtol 1e-10"),
+                         f"[1,1e3], Dirichlet phi=-x; step = value assembly + FP32 Jacobi-PCG whose residual is "
+                         f"replaced by the FP64 true residual after every {args.delta:g}x drop, to FP64 "
+                         f"relres 1e-10"),
             "nodes": m.nv, "tets": m.nt, "nnz": dm.nnz, "free_dofs": free, "colours": dm.ncolours,
-            "lanes_per_row": args.lanes, "spmv_format": args.format,
+            "lanes_per_row": args.lanes, "spmv_format": args.format, "sell_shape": list(shape),
+            "sell_slices": sell[0] if sell else None, "sell_shared_slices": sell[1] if sell else None,
+            "sell_entries": sell[2] if sell else None,
+            "update_trigger": args.delta,
         },
         "iterations_f32": rep.iterations,
         "outer_refinements": rep.outer,
@@ -326,7 +359,8 @@ def main():
         "error_vs_fp64_rtol1e-13": {"phi_rel_l2": err_phi, "Hn_rel_l2_VE": err_H, "energy_rel": err_E},
         "roofline": {
             "bound": "hbm",
-            "kernel": ("k_cg_spmv<float,%d>" % args.lanes) if args.format == "csr" else "k_cg_spmv_sell<float>",
+            "kernel": ("k_cg_spmv<float,%d>" % args.lanes) if args.format == "csr" else
+                      "k_cg_spmv_sell<float,int16,%d,%d>" % shape,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -338,6 +372,19 @@ def main():
             "instrumented_solve_ms": t_instr * 1e3,
         },
     }
+    # ---- classic restarted refinement (FP32 PCG to inner rtol 1e-5, FP64 residual correction)
+    dm.assemble()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rep_rs = dm.solve("f32-restart", rtol=1e-10, inner_rtol=1e-5)
+    ctx.synchronize()
+    t_rs = time.perf_counter() - t0
+    log(f"f32-restart step {t_rs*1e3:.1f} ms, iterations {rep_rs.iterations}, outer {rep_rs.outer}")
+    out["f32_restart_inner1e-5"] = {"ms_per_step": t_rs * 1e3, "iterations": rep_rs.iterations,
+                                    "outer": rep_rs.outer, "relres": rep_rs.relres}
+    traffic = _pmc_traffic(args)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         log(f"CPU baseline with {nthreads} threads ...")

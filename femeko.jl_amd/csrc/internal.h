@@ -104,7 +104,9 @@ struct PcgWork {
     DevBuf<T> x, r, p, q, dinv;
     DevBuf<double> part_pq, part_rz, part_rr, part_bb;  // part_rz holds 2 x nblocks
     DevBuf<CgState> state;
-    hipGraphExec_t graph = nullptr;  // captured chunk of iterations
+    // captured chunk of iterations; it bakes in device addresses, so it is
+    // reset whenever a buffer its kernels read is reallocated
+    hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
     int graph_lanes = 0;
     void reset_graph() {
@@ -173,18 +175,22 @@ struct fk_mesh {
     fk::HostBuf<double> hpart;        // pinned copies of partials
 
     // SELL-64 copy of the matrix for the solver (CSR stays the storage format)
-    int format = 1;                   // 0: CSR kernels, 1: SELL-64 kernels (default)
+    int format = 1;                   // FK_SPMV_CSR, FK_SPMV_SELL64 (default), FK_SPMV_SELL64_ROWIDX
     bool have_sell = false, sell64_valid = false, sell32_valid = false;
     int64_t nslices = 0, sell_len = 0, sell_chunk = 0;
-    fk::DevBuf<int64_t> sell_ptr;
-    fk::DevBuf<int32_t> sell_width, sell_col, sell_map;
+    int64_t nshared = 0;              // slices in the shared-offset form
+    fk::DevBuf<int64_t> sell_ptr, sell_cptr;
+    fk::DevBuf<int32_t> sell_width, sell_col, sell_map, sell_soff;
     fk::DevBuf<int16_t> sell_dcol;    // int16 column offsets from the row (when sell_short)
+    fk::DevBuf<double> sell_diag64;   // diagonal of the FP64 standard-form SpMV
     bool sell_short = false, allow_short = true;
     int32_t bandwidth = 0;
     fk::DevBuf<double> sell_val64;
     fk::DevBuf<float> sell_val32;
 
     int lanes = 16;                   // lanes per CSR row in SpMV
+    int sell_block = 1024;            // threads per block of the SELL SpMV
+    int sell_unroll = 1;              // slices in flight per wave in the SELL SpMV
     int nblocks = 0;                  // persistent grid of the solver kernels
     int64_t chunk = 0;                // rows per logical block
 

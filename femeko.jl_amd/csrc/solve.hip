@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <thread>
 
 #include "device.h"
 #include "internal.h"
@@ -219,15 +220,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
     for (int64_t i = max(r0, v1 * VW) + threadIdx.x; i < r1; i += kBlock) p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
-// ---- SELL-64x4: slices of 64 rows (one wavefront). Slice s with width W
-// (its longest row) holds 64 W entries from sell_ptr[s]: the first
-// 4 floor(W/4) entries of every row as groups of 4 consecutive entries per
-// lane (group g of lane l at 4 (64 g + l)), so each lane reads 4 values and 4
-// column indices with one 16-byte load, then the remaining W mod 4 entries
-// one per lane (entry t of lane l at 256 floor(W/4) + 64 t + l). Padding
-// entries carry value 0 and a column inside the row's own slice. Column
-// indices are int32, or int16 offsets from the row when the matrix bandwidth
-// allows (every column within 32767 of its row).
+// ---- SELL-64x4 solver copy of the matrix: off-diagonal entries only (the
+// diagonal is a separate array), slices of 64 rows (one wavefront). Slice s
+// holds 64 W values from ptr[s] (W = |width[s]|): the first 4 floor(W/4)
+// entries of every row as groups of 4 consecutive entries per lane (group g
+// of lane l at 4 (64 g + l)), so each lane reads 4 values with one 16-byte
+// load, then the remaining W mod 4 entries one per lane (entry t of lane l at
+// 256 floor(W/4) + 64 t + l). Column indices come in one of two forms:
+//   per row (width[s] = W >= 0): 64 W indices from cptr[s], laid out like
+//     the values, as int32 columns or int16 offsets from the row (when every
+//     column lies within 32767 of its row); padding entries point at the row
+//     itself (or at n - 1 past the last row) and carry value 0;
+//   shared (width[s] = -W < 0): the off-diagonal columns of every row of the
+//     slice are among row + o_k for W offsets o_k stored once from
+//     soff[cptr[s]]; a row without column row + o_k holds value 0 there and
+//     its index is clamped into [0, n). Regions of a mesh with translation-
+//     invariant connectivity (structured blocks) take this form: the
+//     wavefront reads its offsets with scalar loads and streams values only.
+template <typename T, typename IDX>
+struct Sell {
+    const int64_t *ptr;    // value offset per slice
+    const int32_t *width;  // W (per-row indices) or -W (shared offsets)
+    const int64_t *cptr;   // index offset per slice (col or soff)
+    const IDX *col;        // per-row indices
+    const int32_t *soff;   // shared offsets
+    const T *val;          // off-diagonal values
+    const T *diag;         // diagonal (FP64 standard form only)
+    int64_t n;
+};
+
 template <typename T>
 __global__ void k_sell_gather(const T *__restrict__ src, const int32_t *__restrict__ map, int64_t n,
                               T *__restrict__ dst) {
@@ -244,72 +265,130 @@ struct alignas(4 * sizeof(T)) Quad {
 
 __device__ __forceinline__ int64_t col_of(int32_t c, int64_t) { return c; }
 __device__ __forceinline__ int64_t col_of(int16_t d, int64_t row) { return row + d; }
+__device__ __forceinline__ int64_t clamp_col(int64_t c, int64_t n) { return min(max(c, int64_t(0)), n - 1); }
 
-template <typename T, typename IDX>
-__device__ __forceinline__ T sell_row(const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
-                                      const IDX *__restrict__ col, const T *__restrict__ val,
-                                      const T *__restrict__ x, int64_t s, int lane, int64_t row) {
-    const int64_t base = sptr[s];
-    const int W = width[s];
+// One gathered term: a x_j (standard form) or a (x_j - x_i) (difference form).
+// P1 stiffness rows sum to zero (the basis gradients of an element sum to
+// zero), so A x = sum_{j != i} a_ij (x_j - x_i) exactly; in FP32 the
+// difference form keeps smooth x from cancelling (the terms are as small as
+// the result) and makes the rounded operator an exact weighted graph
+// Laplacian, spectrally within FP32 rounding of A.
+template <bool DIFF, typename T>
+__device__ __forceinline__ T term(T a, T xj, T xi) {
+    return DIFF ? a * (xj - xi) : a * xj;
+}
+
+// off-diagonal part of row `row` (lane `lane` of slice s)
+template <bool DIFF, typename T, typename IDX>
+__device__ __forceinline__ T sell_row(const Sell<T, IDX> &A, const T *__restrict__ x, int64_t s, int lane,
+                                      int64_t row, T xi) {
+    const int64_t base = A.ptr[s];
+    const int Wr = A.width[s];
+    const int64_t cb = A.cptr[s];
+    const int W = Wr < 0 ? -Wr : Wr;
     const int ng = W >> 2;
-    const Quad<T> *v4 = reinterpret_cast<const Quad<T> *>(val + base) + lane;
-    const Quad<IDX> *c4 = reinterpret_cast<const Quad<IDX> *>(col + base) + lane;
-    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
-    for (int g = 0; g < ng; ++g) {
-        const Quad<T> V = v4[64 * g];
-        const Quad<IDX> C = c4[64 * g];
-        a0 += V.v[0] * x[col_of(C.v[0], row)];
-        a1 += V.v[1] * x[col_of(C.v[1], row)];
-        a2 += V.v[2] * x[col_of(C.v[2], row)];
-        a3 += V.v[3] * x[col_of(C.v[3], row)];
-    }
-    const int64_t tb = base + 256 * static_cast<int64_t>(ng) + lane;
     const int tail = W - 4 * ng;
-    if (tail > 0) a0 += val[tb] * x[col_of(col[tb], row)];
-    if (tail > 1) a1 += val[tb + 64] * x[col_of(col[tb + 64], row)];
-    if (tail > 2) a2 += val[tb + 128] * x[col_of(col[tb + 128], row)];
+    const Quad<T> *v4 = reinterpret_cast<const Quad<T> *>(A.val + base) + lane;
+    const int64_t tb = base + 256 * static_cast<int64_t>(ng) + lane;
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+    if (Wr < 0) {
+        const int32_t *o = A.soff + cb;  // uniform over the wave
+        for (int g = 0; g < ng; ++g) {
+            const Quad<T> V = v4[64 * g];
+            a0 += term<DIFF>(V.v[0], x[clamp_col(row + o[4 * g], A.n)], xi);
+            a1 += term<DIFF>(V.v[1], x[clamp_col(row + o[4 * g + 1], A.n)], xi);
+            a2 += term<DIFF>(V.v[2], x[clamp_col(row + o[4 * g + 2], A.n)], xi);
+            a3 += term<DIFF>(V.v[3], x[clamp_col(row + o[4 * g + 3], A.n)], xi);
+        }
+        const int32_t *ot = o + 4 * ng;
+        if (tail > 0) a0 += term<DIFF>(A.val[tb], x[clamp_col(row + ot[0], A.n)], xi);
+        if (tail > 1) a1 += term<DIFF>(A.val[tb + 64], x[clamp_col(row + ot[1], A.n)], xi);
+        if (tail > 2) a2 += term<DIFF>(A.val[tb + 128], x[clamp_col(row + ot[2], A.n)], xi);
+    } else {
+        const Quad<IDX> *c4 = reinterpret_cast<const Quad<IDX> *>(A.col + cb) + lane;
+        for (int g = 0; g < ng; ++g) {
+            const Quad<T> V = v4[64 * g];
+            const Quad<IDX> C = c4[64 * g];
+            a0 += term<DIFF>(V.v[0], x[col_of(C.v[0], row)], xi);
+            a1 += term<DIFF>(V.v[1], x[col_of(C.v[1], row)], xi);
+            a2 += term<DIFF>(V.v[2], x[col_of(C.v[2], row)], xi);
+            a3 += term<DIFF>(V.v[3], x[col_of(C.v[3], row)], xi);
+        }
+        const int64_t ct = cb + 256 * static_cast<int64_t>(ng) + lane;
+        if (tail > 0) a0 += term<DIFF>(A.val[tb], x[col_of(A.col[ct], row)], xi);
+        if (tail > 1) a1 += term<DIFF>(A.val[tb + 64], x[col_of(A.col[ct + 64], row)], xi);
+        if (tail > 2) a2 += term<DIFF>(A.val[tb + 128], x[col_of(A.col[ct + 128], row)], xi);
+    }
     return (a0 + a1) + (a2 + a3);
 }
 
-// K1 in SELL form: q = A p, partial p.q; each wave owns whole slices
+// (A x)_row: FP32 in the difference form; FP64 in the standard form with the
+// diagonal term added last
 template <typename T, typename IDX>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(const int64_t *__restrict__ sptr,
-                                                         const int32_t *__restrict__ width,
-                                                         const IDX *__restrict__ col, const T *__restrict__ val,
-                                                         const T *__restrict__ p, T *__restrict__ q,
-                                                         double *__restrict__ part, int64_t n, int64_t nslices,
-                                                         int64_t chunk_slices, int nblocks,
-                                                         CgState *__restrict__ st, int parity) {
+__device__ __forceinline__ T sell_apply(const Sell<T, IDX> &A, const T *__restrict__ x, int64_t s, int lane,
+                                        int64_t row, T xi) {
+    constexpr bool DIFF = sizeof(T) == 4;
+    const T off = sell_row<DIFF>(A, x, s, lane, row, xi);
+    if (DIFF) return off;
+    return row < A.n ? off + A.diag[row] * xi : off;
+}
+
+// K1 in SELL form: q = A p, partial p.q; each wave owns whole slices, U
+// slices in flight per wave (independent loads interleaved)
+template <typename T, typename IDX, int BLOCK, int U>
+__global__ __launch_bounds__(BLOCK) void k_cg_spmv_sell(const Sell<T, IDX> A, const T *__restrict__ p,
+                                                        T *__restrict__ q, double *__restrict__ part,
+                                                        int64_t nslices, int64_t chunk_slices, int nblocks,
+                                                        CgState *__restrict__ st, int parity) {
     if (stopped_entry(st, parity)) return;
-    __shared__ double sh[kBlock / 64 + 1];
+    __shared__ double sh[BLOCK / 64 + 1];
+    constexpr int NW = BLOCK / 64;
     const int lb = logical_block(nblocks);
     const int64_t s0 = static_cast<int64_t>(lb) * chunk_slices;
     const int64_t s1 = min(nslices, s0 + chunk_slices);
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // wave index made uniform for the compiler: slice tables and shared
+    // offsets are then read with scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t n = A.n;
     double acc = 0.0;
-    for (int64_t s = s0 + wv; s < s1; s += kBlock / 64) {
-        const int64_t row = s * 64 + lane;
-        const T y = sell_row<T, IDX>(sptr, width, col, val, p, s, lane, row);
-        if (row < n) {
-            q[row] = y;
-            acc += static_cast<double>(p[row]) * static_cast<double>(y);
+    int64_t s = s0 + wv;
+    for (; s + (U - 1) * NW < s1; s += U * NW) {
+        T y[U], xi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t row = (s + u * NW) * 64 + lane;
+            xi[u] = row < n ? p[row] : T(0);
+            y[u] = sell_apply(A, p, s + u * NW, lane, row, xi[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t row = (s + u * NW) * 64 + lane;
+            if (row < n) {
+                q[row] = y[u];
+                acc += static_cast<double>(xi[u]) * static_cast<double>(y[u]);
+            }
         }
     }
-    const double tot = block_sum<kBlock>(acc, sh);
+    for (; s < s1; s += NW) {
+        const int64_t row = s * 64 + lane;
+        const T xi = row < n ? p[row] : T(0);
+        const T y = sell_apply(A, p, s, lane, row, xi);
+        if (row < n) {
+            q[row] = y;
+            acc += static_cast<double>(xi) * static_cast<double>(y);
+        }
+    }
+    const double tot = block_sum<BLOCK>(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
 // FP64 r = -mask(A phi), SELL form
 template <typename IDX>
-__global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const int64_t *__restrict__ sptr,
-                                                              const int32_t *__restrict__ width,
-                                                              const IDX *__restrict__ col,
-                                                              const double *__restrict__ val,
+__global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const Sell<double, IDX> A,
                                                               const double *__restrict__ phi,
                                                               const uint8_t *__restrict__ mask,
                                                               double *__restrict__ r, double *__restrict__ part,
-                                                              int64_t n, int64_t nslices, int64_t chunk_slices,
-                                                              int nblocks) {
+                                                              int64_t nslices, int64_t chunk_slices, int nblocks) {
     __shared__ double sh[kBlock / 64 + 1];
     const int lb = logical_block(nblocks);
     const int64_t s0 = static_cast<int64_t>(lb) * chunk_slices;
@@ -318,8 +397,9 @@ __global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const int64_t *__r
     double acc = 0.0;
     for (int64_t s = s0 + wv; s < s1; s += kBlock / 64) {
         const int64_t row = s * 64 + lane;
-        const double y = sell_row<double, IDX>(sptr, width, col, val, phi, s, lane, row);
-        if (row < n) {
+        const double xi = row < A.n ? phi[row] : 0.0;
+        const double y = sell_apply(A, phi, s, lane, row, xi);
+        if (row < A.n) {
             const double v = mask[row] ? 0.0 : -y;
             r[row] = v;
             acc += v * v;
@@ -493,6 +573,11 @@ const double *values<double>(fk_mesh *m) { return m->val64.get(); }
 template <>
 const float *values<float>(fk_mesh *m) { return m->val32.get(); }
 
+// key of the captured iteration graphs: every launch parameter they bake in
+static int solver_config(const fk_mesh *m) {
+    return m->lanes + 100 * m->format + 1000 * m->sell_unroll + 10000 * m->sell_block;
+}
+
 template <typename T>
 static void ensure_work(fk_mesh *m) {
     PcgWork<T> &w = work<T>(m);
@@ -517,89 +602,170 @@ static void compute_dinv(fk_mesh *m) {
     if (h) fail(FK_ERR_SINGULAR, std::to_string(h) + " free node(s) without a positive diagonal");
 }
 
-// fill the SELL-64x4 columns (int32 or int16 row offsets) and the SELL -> CSR
-// position map
-__global__ void k_sell_fill(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
-                            const int64_t *__restrict__ sptr, const int32_t *__restrict__ width, int64_t nv,
-                            int64_t nrows_padded, int32_t *__restrict__ scol, int16_t *__restrict__ sdcol,
-                            int32_t *__restrict__ smap) {
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= nrows_padded) return;
-    const int64_t s = r >> 6;
-    const int lane = static_cast<int>(r & 63);
-    const int W = width[s];
-    const int ng = W >> 2;
-    const int64_t base = sptr[s];
-    const int32_t a = r < nv ? rowptr[r] : 0;
-    const int32_t len = r < nv ? rowptr[r + 1] - a : 0;
-    const int32_t self = static_cast<int32_t>(r < nv ? r : nv - 1);
-    for (int k = 0; k < W; ++k) {
-        const int64_t o = k < 4 * ng ? base + 4 * (64 * static_cast<int64_t>(k >> 2) + lane) + (k & 3)
-                                     : base + 256 * static_cast<int64_t>(ng) + 64 * (k - 4 * ng) + lane;
-        const int32_t c = k < len ? colidx[a + k] : self;
-        if (scol) scol[o] = c;
-        if (sdcol) sdcol[o] = static_cast<int16_t>(c - r);
-        smap[o] = k < len ? a + k : -1;
+// run f(i) for i in [0, n) on up to 16 host threads (contiguous ranges)
+template <typename F>
+static void parallel_for(int64_t n, F f) {
+    int nt = static_cast<int>(std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())));
+    if (n < 4096) nt = 1;
+    if (nt == 1) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
     }
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&f, n, nt, t] {
+            for (int64_t i = n * t / nt; i < n * (t + 1) / nt; ++i) f(i);
+        });
+    for (auto &x : th) x.join();
 }
 
-// largest |col - row| over the CSR pattern
-__global__ void k_bandwidth(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, int64_t nv,
-                            int32_t *__restrict__ out) {
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= nv) return;
-    int32_t b = 0;
-    for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) b = max(b, abs(colidx[k] - static_cast<int32_t>(r)));
-    atomicMax(out, b);
-}
-
+// Build the SELL-64x4 copy (layout above) on the host from the CSR pattern:
+// per slice, the shared-offset form when the union of the rows' offsets is
+// at most 1.25x the longest row (each extra offset costs one padded value
+// per row, each per-row index it saves costs a quarter to half a value),
+// else per-row indices. Uploads the slice tables, the indices and the
+// SELL -> CSR position map (-1 on padding) that later value refreshes use.
 static void build_sell(fk_mesh *m) {
     if (m->have_sell) return;
+    // captured iteration graphs hold the addresses of the arrays rebuilt here
+    m->w64.reset_graph();
+    m->w32.reset_graph();
     hipStream_t st = m->ctx->stream;
-    const int64_t nv = m->nv;
-    std::vector<int32_t> rp(static_cast<size_t>(nv + 1));
+    const int64_t nv = m->nv, nnz = m->nnz;
+    std::vector<int32_t> rp(static_cast<size_t>(nv + 1)), ci(static_cast<size_t>(nnz));
     FK_HIP(hipMemcpyAsync(rp.data(), m->rowptr.get(), (nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(ci.data(), m->colidx.get(), nnz * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
     const int64_t ns = (nv + 63) / 64;
+    const bool allow_shared = m->format == FK_SPMV_SELL64;
+    // pass 1: widths, shared offset sets, bandwidth
     std::vector<int32_t> width(static_cast<size_t>(ns));
-    std::vector<int64_t> sptr(static_cast<size_t>(ns + 1), 0);
-    for (int64_t s = 0; s < ns; ++s) {
-        int32_t w = 0;
-        for (int64_t r = s * 64; r < std::min(nv, s * 64 + 64); ++r) w = std::max(w, rp[r + 1] - rp[r]);
+    std::vector<std::vector<int32_t>> shared(static_cast<size_t>(ns));
+    std::vector<int32_t> band(static_cast<size_t>(ns), 0);
+    parallel_for(ns, [&](int64_t s) {
+        const int64_t r0 = s * 64, r1 = std::min(nv, r0 + 64);
+        int32_t w = 0, b = 0;
+        std::vector<int32_t> off;
+        for (int64_t r = r0; r < r1; ++r) {
+            int32_t len = 0;
+            for (int32_t k = rp[r]; k < rp[r + 1]; ++k) {
+                const int32_t d = ci[k] - static_cast<int32_t>(r);
+                if (d == 0) continue;
+                ++len;
+                b = std::max(b, d < 0 ? -d : d);
+                if (allow_shared) off.push_back(d);
+            }
+            w = std::max(w, len);
+        }
+        band[s] = b;
+        if (allow_shared && w > 0) {
+            std::sort(off.begin(), off.end());
+            off.erase(std::unique(off.begin(), off.end()), off.end());
+            if (static_cast<int32_t>(off.size()) * 4 <= 5 * w) {
+                shared[s] = std::move(off);
+                width[s] = -static_cast<int32_t>(shared[s].size());
+                return;
+            }
+        }
         width[s] = w;
-        sptr[s + 1] = sptr[s] + 64 * static_cast<int64_t>(w);
+    });
+    int32_t bw = 0;
+    for (int64_t s = 0; s < ns; ++s) bw = std::max(bw, band[s]);
+    m->bandwidth = bw;
+    m->sell_short = m->allow_short && bw + 64 <= 32767;
+    // pass 2: offsets of values and indices
+    std::vector<int64_t> vptr(static_cast<size_t>(ns + 1), 0), cptr(static_cast<size_t>(ns), 0);
+    int64_t ncol = 0, nsoff = 0, nshared = 0;
+    for (int64_t s = 0; s < ns; ++s) {
+        const int32_t W = width[s] < 0 ? -width[s] : width[s];
+        vptr[s + 1] = vptr[s] + 64 * static_cast<int64_t>(W);
+        if (width[s] < 0) {
+            cptr[s] = nsoff;
+            nsoff += W;
+            ++nshared;
+        } else {
+            cptr[s] = ncol;
+            ncol += 64 * static_cast<int64_t>(W);
+        }
     }
+    const int64_t len = vptr[ns];
+    // pass 3: map and indices
+    std::vector<int32_t> map(static_cast<size_t>(len), -1), soff(static_cast<size_t>(nsoff));
+    std::vector<int32_t> col32(m->sell_short ? 0 : static_cast<size_t>(ncol));
+    std::vector<int16_t> col16(m->sell_short ? static_cast<size_t>(ncol) : 0);
+    parallel_for(ns, [&](int64_t s) {
+        const int32_t W = width[s] < 0 ? -width[s] : width[s];
+        const int32_t ng = W >> 2;
+        auto pos = [&](int lane, int32_t k) {
+            return k < 4 * ng ? 4 * (64 * static_cast<int64_t>(k >> 2) + lane) + (k & 3)
+                              : 256 * static_cast<int64_t>(ng) + 64 * static_cast<int64_t>(k - 4 * ng) + lane;
+        };
+        if (width[s] < 0) {
+            const std::vector<int32_t> &off = shared[s];
+            std::copy(off.begin(), off.end(), soff.begin() + cptr[s]);
+            for (int lane = 0; lane < 64; ++lane) {
+                const int64_t r = s * 64 + lane;
+                if (r >= nv) break;
+                const int32_t *b = ci.data() + rp[r], *e = ci.data() + rp[r + 1];
+                for (int32_t k = 0; k < W; ++k) {
+                    const int64_t c = r + off[k];
+                    if (c < 0 || c >= nv) continue;
+                    const int32_t *f = std::lower_bound(b, e, static_cast<int32_t>(c));
+                    if (f != e && *f == c) map[vptr[s] + pos(lane, k)] = static_cast<int32_t>(f - ci.data());
+                }
+            }
+            return;
+        }
+        for (int lane = 0; lane < 64; ++lane) {
+            const int64_t r = s * 64 + lane;
+            const int32_t self = static_cast<int32_t>(r < nv ? r : nv - 1);
+            int32_t k = 0;
+            if (r < nv)
+                for (int32_t q = rp[r]; q < rp[r + 1]; ++q) {
+                    if (ci[q] == r) continue;
+                    const int64_t o = pos(lane, k++);
+                    map[vptr[s] + o] = q;
+                    if (m->sell_short)
+                        col16[cptr[s] + o] = static_cast<int16_t>(ci[q] - r);
+                    else
+                        col32[cptr[s] + o] = ci[q];
+                }
+            for (; k < W; ++k) {
+                const int64_t o = pos(lane, k);
+                if (m->sell_short)
+                    col16[cptr[s] + o] = static_cast<int16_t>(self - r);
+                else
+                    col32[cptr[s] + o] = self;
+            }
+        }
+    });
     m->nslices = ns;
-    m->sell_len = sptr[ns];
+    m->nshared = nshared;
+    m->sell_len = len;
     m->sell_chunk = (ns + m->nblocks - 1) / m->nblocks;
-    // int16 row offsets when every column lies within 32767 of its row
-    {
-        DevBuf<int32_t> bw(1);
-        FK_HIP(hipMemsetAsync(bw.get(), 0, sizeof(int32_t), st));
-        k_bandwidth<<<grid_for(nv, 256), 256, 0, st>>>(m->rowptr.get(), m->colidx.get(), nv, bw.get());
-        FK_HIP(hipGetLastError());
-        int32_t h = 0;
-        FK_HIP(hipMemcpyAsync(&h, bw.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        FK_HIP(hipStreamSynchronize(st));
-        m->bandwidth = h;
-        m->sell_short = m->allow_short && h + 64 <= 32767;
-    }
     m->sell_ptr.alloc(ns + 1);
     m->sell_width.alloc(ns);
-    m->sell_col.release();
-    m->sell_dcol.release();
-    if (m->sell_short)
-        m->sell_dcol.alloc(m->sell_len);
-    else
-        m->sell_col.alloc(m->sell_len);
-    m->sell_map.alloc(m->sell_len);
-    FK_HIP(hipMemcpyAsync(m->sell_ptr.get(), sptr.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    m->sell_cptr.alloc(ns);
+    m->sell_col.alloc(col32.size());
+    m->sell_dcol.alloc(col16.size());
+    m->sell_soff.alloc(soff.size());
+    m->sell_map.alloc(len);
+    FK_HIP(hipMemcpyAsync(m->sell_ptr.get(), vptr.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
     FK_HIP(hipMemcpyAsync(m->sell_width.get(), width.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    k_sell_fill<<<grid_for(ns * 64, 256), 256, 0, st>>>(m->rowptr.get(), m->colidx.get(), m->sell_ptr.get(),
-                                                        m->sell_width.get(), nv, ns * 64, m->sell_col.get(),
-                                                        m->sell_dcol.get(), m->sell_map.get());
-    FK_HIP(hipGetLastError());
-    FK_HIP(hipStreamSynchronize(st));
+    FK_HIP(hipMemcpyAsync(m->sell_cptr.get(), cptr.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    if (!col32.empty())
+        FK_HIP(hipMemcpyAsync(m->sell_col.get(), col32.data(), col32.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              st));
+    if (!col16.empty())
+        FK_HIP(hipMemcpyAsync(m->sell_dcol.get(), col16.data(), col16.size() * sizeof(int16_t),
+                              hipMemcpyHostToDevice, st));
+    if (!soff.empty())
+        FK_HIP(hipMemcpyAsync(m->sell_soff.get(), soff.data(), soff.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              st));
+    if (len > 0)
+        FK_HIP(hipMemcpyAsync(m->sell_map.get(), map.data(), len * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
     m->have_sell = true;
 }
 
@@ -608,22 +774,27 @@ template <typename T>
 static void sync_solver_values(fk_mesh *m);
 template <>
 void sync_solver_values<double>(fk_mesh *m) {
-    if (m->format != 1 || m->sell64_valid) return;
+    if (m->format == FK_SPMV_CSR || m->sell64_valid) return;
     build_sell(m);
     m->sell_val64.ensure(m->sell_len);
-    k_sell_gather<double><<<grid_for(m->sell_len, 256), 256, 0, m->ctx->stream>>>(
-        m->val64.get(), m->sell_map.get(), m->sell_len, m->sell_val64.get());
+    m->sell_diag64.ensure(m->nv);
+    if (m->sell_len > 0)
+        k_sell_gather<double><<<grid_for(m->sell_len, 256), 256, 0, m->ctx->stream>>>(
+            m->val64.get(), m->sell_map.get(), m->sell_len, m->sell_val64.get());
+    k_sell_gather<double><<<grid_for(m->nv, 256), 256, 0, m->ctx->stream>>>(m->val64.get(), m->diag.get(), m->nv,
+                                                                            m->sell_diag64.get());
     FK_HIP(hipGetLastError());
     m->sell64_valid = true;
 }
 template <>
 void sync_solver_values<float>(fk_mesh *m) {
-    if (m->format != 1 || m->sell32_valid) return;
+    if (m->format == FK_SPMV_CSR || m->sell32_valid) return;
     build_sell(m);
     make_val32(m);
     m->sell_val32.ensure(m->sell_len);
-    k_sell_gather<float><<<grid_for(m->sell_len, 256), 256, 0, m->ctx->stream>>>(
-        m->val32.get(), m->sell_map.get(), m->sell_len, m->sell_val32.get());
+    if (m->sell_len > 0)
+        k_sell_gather<float><<<grid_for(m->sell_len, 256), 256, 0, m->ctx->stream>>>(
+            m->val32.get(), m->sell_map.get(), m->sell_len, m->sell_val32.get());
     FK_HIP(hipGetLastError());
     m->sell32_valid = true;
 }
@@ -643,16 +814,44 @@ static void launch_spmv(fk_mesh *m, PcgWork<T> &w, int parity) {
 }
 
 template <typename T>
+static const T *sell_diag(fk_mesh *m);
+template <>
+const double *sell_diag<double>(fk_mesh *m) { return m->sell_diag64.get(); }
+template <>
+const float *sell_diag<float>(fk_mesh *) { return nullptr; }
+
+template <typename T, typename IDX>
+static Sell<T, IDX> sell_view(fk_mesh *m, const IDX *col) {
+    return Sell<T, IDX>{m->sell_ptr.get(), m->sell_width.get(), m->sell_cptr.get(), col, m->sell_soff.get(),
+                        sell_values<T>(m), sell_diag<T>(m), m->nv};
+}
+
+template <typename T, typename IDX, int BLOCK, int U>
+static void launch_sell(fk_mesh *m, PcgWork<T> &w, const IDX *col, int parity) {
+    k_cg_spmv_sell<T, IDX, BLOCK, U><<<m->nblocks, BLOCK, 0, m->ctx->stream>>>(
+        sell_view<T, IDX>(m, col), w.p.get(), w.q.get(), w.part_pq.get(), m->nslices, m->sell_chunk, m->nblocks,
+        w.state.get(), parity);
+}
+
+template <typename T, typename IDX>
+static void launch_sell_shape(fk_mesh *m, PcgWork<T> &w, const IDX *col, int parity) {
+    const int shape = m->sell_block * 4 + m->sell_unroll;
+    switch (shape) {
+        case 256 * 4 + 2: launch_sell<T, IDX, 256, 2>(m, w, col, parity); break;
+        case 512 * 4 + 1: launch_sell<T, IDX, 512, 1>(m, w, col, parity); break;
+        case 512 * 4 + 2: launch_sell<T, IDX, 512, 2>(m, w, col, parity); break;
+        case 1024 * 4 + 1: launch_sell<T, IDX, 1024, 1>(m, w, col, parity); break;
+        default: launch_sell<T, IDX, 256, 1>(m, w, col, parity); break;
+    }
+}
+
+template <typename T>
 static void launch_spmv_lanes(fk_mesh *m, PcgWork<T> &w, int parity) {
-    if (m->format == 1) {
+    if (m->format != FK_SPMV_CSR) {
         if (m->sell_short)
-            k_cg_spmv_sell<T, int16_t><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(
-                m->sell_ptr.get(), m->sell_width.get(), m->sell_dcol.get(), sell_values<T>(m), w.p.get(), w.q.get(),
-                w.part_pq.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks, w.state.get(), parity);
+            launch_sell_shape<T, int16_t>(m, w, m->sell_dcol.get(), parity);
         else
-            k_cg_spmv_sell<T, int32_t><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(
-                m->sell_ptr.get(), m->sell_width.get(), m->sell_col.get(), sell_values<T>(m), w.p.get(), w.q.get(),
-                w.part_pq.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks, w.state.get(), parity);
+            launch_sell_shape<T, int32_t>(m, w, m->sell_col.get(), parity);
         return;
     }
     switch (m->lanes) {
@@ -747,7 +946,7 @@ static CgState run_pcg(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref)
     }
     // graph of G iterations (G even keeps the parity sequence)
     const int G = m->nv > 200000 ? 32 : 16;
-    const int cfg = m->lanes + 1000 * m->format;
+    const int cfg = solver_config(m);
     if (!w.graph || w.graph_iters != G || w.graph_lanes != cfg) {
         w.reset_graph();
         hipGraph_t graph;
@@ -807,7 +1006,7 @@ static CgState run_chunk(fk_mesh *m, PcgWork<T> &w, int G) {
         if (timed) harvest_timing(m->ctx, hs->iters - before);
         return *hs;
     }
-    const int cfg = m->lanes + 1000 * m->format;
+    const int cfg = solver_config(m);
     if (!w.graph || w.graph_iters != G || w.graph_lanes != cfg) {
         w.reset_graph();
         hipGraph_t graph;
@@ -987,16 +1186,16 @@ static void launch_neg_residual(fk_mesh *m, const double *phi, double *r, double
 static double neg_residual(fk_mesh *m, const double *phi, double *r) {
     hipStream_t st = m->ctx->stream;
     m->part_misc.ensure(m->nblocks);
-    if (m->format == 1) {
+    if (m->format != FK_SPMV_CSR) {
         sync_solver_values<double>(m);
         if (m->sell_short)
             k_neg_residual_sell<int16_t><<<m->nblocks, kBlock, 0, st>>>(
-                m->sell_ptr.get(), m->sell_width.get(), m->sell_dcol.get(), m->sell_val64.get(), phi, m->mask.get(),
-                r, m->part_misc.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks);
+                sell_view<double, int16_t>(m, m->sell_dcol.get()), phi, m->mask.get(), r, m->part_misc.get(),
+                m->nslices, m->sell_chunk, m->nblocks);
         else
             k_neg_residual_sell<int32_t><<<m->nblocks, kBlock, 0, st>>>(
-                m->sell_ptr.get(), m->sell_width.get(), m->sell_col.get(), m->sell_val64.get(), phi, m->mask.get(),
-                r, m->part_misc.get(), m->nv, m->nslices, m->sell_chunk, m->nblocks);
+                sell_view<double, int32_t>(m, m->sell_col.get()), phi, m->mask.get(), r, m->part_misc.get(),
+                m->nslices, m->sell_chunk, m->nblocks);
     } else switch (m->lanes) {
         case 4: launch_neg_residual<4>(m, phi, r, m->part_misc.get()); break;
         case 8: launch_neg_residual<8>(m, phi, r, m->part_misc.get()); break;
@@ -1271,8 +1470,35 @@ int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms) {
 }
 
 int fk_set_spmv_format(fk_mesh *m, int32_t format) {
-    if (!m || (format != FK_SPMV_CSR && format != FK_SPMV_SELL64)) return FK_ERR_INVALID_ARG;
+    if (!m || (format != FK_SPMV_CSR && format != FK_SPMV_SELL64 && format != FK_SPMV_SELL64_ROWIDX))
+        return FK_ERR_INVALID_ARG;
+    if (format != m->format) {
+        // the two SELL forms differ in layout: rebuild on next use
+        m->have_sell = false;
+        m->sell64_valid = false;
+        m->sell32_valid = false;
+    }
     m->format = format;
+    return FK_OK;
+}
+
+int fk_sell_info(fk_mesh *m, int64_t *nslices, int64_t *shared_slices, int64_t *entries) {
+    if (!m) return FK_ERR_INVALID_ARG;
+    return fk::guard(m->ctx, "fk_sell_info", [&] {
+        if (m->format == FK_SPMV_CSR) fk::fail(FK_ERR_STATE, "the solver format is CSR");
+        fk::build_sell(m);
+        if (nslices) *nslices = m->nslices;
+        if (shared_slices) *shared_slices = m->nshared;
+        if (entries) *entries = m->sell_len;
+    });
+}
+
+int fk_set_sell_shape(fk_mesh *m, int32_t block, int32_t unroll) {
+    if (!m) return FK_ERR_INVALID_ARG;
+    const bool ok = ((block == 256 || block == 512) && (unroll == 1 || unroll == 2)) || (block == 1024 && unroll == 1);
+    if (!ok) return FK_ERR_INVALID_ARG;
+    m->sell_block = block;
+    m->sell_unroll = unroll;
     return FK_OK;
 }
 

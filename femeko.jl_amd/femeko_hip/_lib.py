@@ -96,6 +96,8 @@ SIGNATURES = {
     "fk_spmv_timing": (_c.c_int, [_vp, _i64p, _dp]),
     "fk_set_lanes_per_row": (_c.c_int, [_vp, _c.c_int32]),
     "fk_set_spmv_format": (_c.c_int, [_vp, _c.c_int32]),
+    "fk_set_sell_shape": (_c.c_int, [_vp, _c.c_int32, _c.c_int32]),
+    "fk_sell_info": (_c.c_int, [_vp, _c.c_void_p, _c.c_void_p, _c.c_void_p]),
 }
 
 _lib = None

@@ -292,9 +292,21 @@ class DeviceMesh:
         return float(res.value)
 
     def set_format(self, fmt: str) -> None:
-        """Solver SpMV format: 'csr' or 'sell64'."""
-        code = {"csr": 0, "sell64": 1, "sell": 1}[fmt]
+        """Solver SpMV format: 'csr', 'sell64' (shared slice offsets where the
+        rows allow) or 'sell64-rowidx' (per-row indices only)."""
+        code = {"csr": 0, "sell64": 1, "sell": 1, "sell64-rowidx": 2}[fmt]
         self.ctx.check(self._L.fk_set_spmv_format(self.handle, code), "fk_set_spmv_format")
+
+    def sell_info(self):
+        """(slices, shared-offset slices, stored entries) of the SELL copy."""
+        a, b, c = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        self.ctx.check(self._L.fk_sell_info(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                       "fk_sell_info")
+        return int(a.value), int(b.value), int(c.value)
+
+    def set_sell_shape(self, block: int, unroll: int = 1) -> None:
+        """SELL SpMV launch shape: threads per block and slices in flight per wave."""
+        self.ctx.check(self._L.fk_set_sell_shape(self.handle, int(block), int(unroll)), "fk_set_sell_shape")
 
     def set_lanes(self, lanes: int) -> None:
         self.ctx.check(self._L.fk_set_lanes_per_row(self.handle, int(lanes)), "fk_set_lanes_per_row")

@@ -246,13 +246,29 @@ int fk_set_timing(fk_context *ctx, int32_t enable);
 int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms);
 /* SpMV tuning: lanes per row (4, 8, 16, 32, 64) of the CSR kernels. */
 int fk_set_lanes_per_row(fk_mesh *mesh, int32_t lanes);
-/* Matrix format of the solver's SpMV: FK_SPMV_CSR (the assembled CSR) or
- * FK_SPMV_SELL64 (a sliced-ELL copy, 64 rows per slice = one wavefront,
- * refreshed from the CSR values after each assembly). Results agree to
- * rounding; the exported matrix is always CSR. */
+/* Matrix format of the solver's SpMV: FK_SPMV_CSR (the assembled CSR) or a
+ * sliced-ELL copy of the off-diagonal entries (64 rows per slice = one
+ * wavefront, refreshed from the CSR values after each assembly, diagonal
+ * kept apart). FK_SPMV_SELL64 (default) stores the column offsets of a slice
+ * once when its rows share one offset set (structured mesh regions) and per
+ * row elsewhere; FK_SPMV_SELL64_ROWIDX always stores them per row. The SELL
+ * kernels compute the FP32 product in the difference form
+ * sum_j a_ij (x_j - x_i) (rows of a P1 stiffness matrix sum to zero), the
+ * FP64 product in the standard form. Results agree to rounding; the exported
+ * matrix is always CSR. */
 #define FK_SPMV_CSR 0
 #define FK_SPMV_SELL64 1
+#define FK_SPMV_SELL64_ROWIDX 2
 int fk_set_spmv_format(fk_mesh *mesh, int32_t format);
+/* SELL copy statistics (built on demand): slices, slices in the shared-offset
+ * form, stored entries (values including padding). Any pointer may be NULL. */
+int fk_sell_info(fk_mesh *mesh, int64_t *nslices, int64_t *shared_slices, int64_t *entries);
+/* Launch shape of the SELL SpMV on the persistent grid of the solver
+ * kernels: threads per block (256, 512, 1024; default 1024) and slices in
+ * flight per wavefront (1 or 2; 1024 threads only with 1). Rows are computed
+ * alike in every shape; the p.q partial sums group differently, so solves
+ * agree to rounding. */
+int fk_set_sell_shape(fk_mesh *mesh, int32_t block, int32_t unroll);
 
 #ifdef __cplusplus
 }

@@ -178,10 +178,11 @@ def test_solve_f32_refined_linear20(lin20, delta):
     assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) < 1e-9
 
 
-@pytest.mark.parametrize("fmt", ["csr", "sell64"])
+@pytest.mark.parametrize("fmt", ["csr", "sell64", "sell64-rowidx"])
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32-restart"])
 def test_spmv_formats_solve_agree(lin20, prec, fmt):
-    # the solver's CSR kernels and its SELL-64 copy give the same solution
+    # the solver's CSR kernels and its SELL-64 copies (shared slice offsets or
+    # per-row indices) give the same solution
     pr, A, dm = lin20
     golden = np.load(GOLDEN)
     try:
@@ -193,6 +194,70 @@ def test_spmv_formats_solve_agree(lin20, prec, fmt):
         it_ref = int(golden["pcg_iters_1e10"])
         if prec == "f64":
             assert abs(rep.iterations - it_ref) <= max(3, 0.05 * it_ref)
+    finally:
+        dm.set_format("sell64")
+
+
+@pytest.mark.parametrize("shape", [(256, 1), (256, 2), (512, 1), (512, 2), (1024, 1)])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_sell_launch_shapes_agree(lin20, prec, shape):
+    # every SELL SpMV launch shape (1 or 2 slices in flight per wave, 4 to 16
+    # waves per block) solves to the golden potential; rows are computed
+    # alike, only the grouping of the p.q partial sums differs
+    pr, A, dm = lin20
+    golden = np.load(GOLDEN)
+    try:
+        dm.set_sell_shape(*shape)
+        rep = dm.solve(prec, rtol=1e-10)
+        phi = dm.potential()
+        assert rep.relres < 1e-10
+        assert np.linalg.norm(phi - golden["phi"]) / np.linalg.norm(golden["phi"]) < 1e-9
+        if prec == "f64":
+            it_ref = int(golden["pcg_iters_1e10"])
+            assert abs(rep.iterations - it_ref) <= max(3, 0.05 * it_ref)
+        # fixed-order reductions: a repeat is bitwise equal
+        dm.solve(prec, rtol=1e-10)
+        assert np.array_equal(dm.potential(), phi)
+    finally:
+        dm.set_sell_shape(1024, 1)
+
+
+def test_sell_shape_rejects_unknown():
+    pr = W.linear(4)
+    dm = DeviceMesh(pr.mesh)
+    for shape in [(128, 1), (256, 3), (256, 0), (1024, 2), (512, -1)]:
+        with pytest.raises(FemekoHIPError):
+            dm.set_sell_shape(*shape)
+    dm.close()
+
+
+def test_sell_layout_statistics(lin20):
+    # the Kuhn mesh has translation-invariant connectivity: every slice of
+    # 64 rows shares one offset set (boundary rows store zeros for the
+    # neighbours they lack); per-row indices store exactly the padded
+    # off-diagonal rows
+    pr, A, dm = lin20
+    try:
+        ns, nsh, ent = dm.sell_info()
+        assert ns == (dm.nv + 63) // 64 and nsh == ns
+        dm.set_format("sell64-rowidx")
+        ns2, nsh2, ent2 = dm.sell_info()
+        assert ns2 == ns and nsh2 == 0
+        counts = np.diff(A.colptr) - 1  # off-diagonal entries per row
+        padded = sum(64 * counts[s * 64:(s + 1) * 64].max() for s in range(ns))
+        assert ent2 == padded
+        dm.set_format("csr")
+        with pytest.raises(FemekoHIPError):
+            dm.sell_info()
+        # back to the first layout with no solve in between: the rebuilt
+        # arrays live at new addresses and the captured iteration graphs
+        # must follow them
+        dm.set_format("sell64")
+        for prec in ("f64", "f32"):
+            rep = dm.solve(prec, rtol=1e-10)
+            golden = np.load(GOLDEN)
+            assert rep.relres < 1e-10
+            assert np.linalg.norm(dm.potential() - golden["phi"]) / np.linalg.norm(golden["phi"]) < 1e-9
     finally:
         dm.set_format("sell64")
 
