@@ -196,23 +196,33 @@ def bench_adjoint(args):
     for _ in range(args.steps):
         J, grad, its = ADJ.fieldObjectiveGradient(m, rho, tg, Hs, pr.mask, pr.g)
     t = (time.perf_counter() - t0) / args.steps
+    # 32 seeded elements: the 16 largest gradients and 16 uniform draws; the
+    # differences use FP64 solves to rtol 1e-13 so their noise stays far
+    # below the step's truncation error
     rng = np.random.default_rng(80)
-    idx = rng.choice(m.nt, 32, replace=False)
+    idx = np.concatenate([np.argsort(-np.abs(grad))[:16], rng.choice(m.nt, 16, replace=False)])
     h = 1e-6
-    errs = []
+    gmax = float(np.abs(grad).max())
+    err_scaled, err_rel_large = [], []
     for e in idx:
         rp = rho.copy()
         rp[e] += h
         rm = rho.copy()
         rm[e] -= h
-        fd = (ADJ.fieldObjective(m, rp, tg, Hs, pr.mask, pr.g) - ADJ.fieldObjective(m, rm, tg, Hs, pr.mask, pr.g)) / (2 * h)
-        errs.append(abs(fd - grad[e]) / max(abs(fd), 1e-6 * np.abs(grad).max()))
-    log(f"adjoint: {t*1e3:.1f} ms per state+adjoint+gradient; FD max rel err {max(errs):.2e}")
+        fd = (ADJ.fieldObjective(m, rp, tg, Hs, pr.mask, pr.g, rtol=1e-13)
+              - ADJ.fieldObjective(m, rm, tg, Hs, pr.mask, pr.g, rtol=1e-13)) / (2 * h)
+        err_scaled.append(abs(fd - grad[e]) / gmax)
+        if abs(grad[e]) >= 1e-3 * gmax:
+            err_rel_large.append(abs(fd - grad[e]) / abs(grad[e]))
+    log(f"adjoint: {t*1e3:.1f} ms per state+adjoint+gradient; FD max |fd-g|/max|g| {max(err_scaled):.2e}, "
+        f"max rel err where |g| >= 1e-3 max|g| {max(err_rel_large):.2e} ({len(err_rel_large)} elements)")
     res = _common(args, "state+adjoint+gradient DOF/s (FP64)", m.nv / t, "DOF/s", t * 1e3, "f64",
                   {"workload": "adjoint80: Kuhn n=80, mu_r = 1 + 999 rho^3, target [-0.1,0.1]^3, H* = (2,0,0)",
                    "nodes": m.nv, "tets": m.nt})
-    res.update({"J": J, "adjoint_iterations": its, "fd_elements": 32, "fd_step": h,
-                "fd_max_rel_err": float(max(errs)), "fd_median_rel_err": float(np.median(errs))})
+    res.update({"J": J, "adjoint_iterations": its, "fd_elements": int(len(idx)), "fd_step": h,
+                "fd_max_err_over_max_grad": float(max(err_scaled)),
+                "fd_max_rel_err_large_grad": float(max(err_rel_large)),
+                "fd_large_grad_elements": len(err_rel_large)})
     fem3d.invalidate(m)
     return res
 
@@ -311,6 +321,11 @@ def main():
     err_E = float(abs(E32 - E_ref) / abs(E_ref))
     log(f"errors vs FP64 rtol 1e-13: phi {err_phi:.2e} |H| {err_H:.2e} energy {err_E:.2e}")
 
+    # ---- practical streaming ceiling: plain 1 GiB device copy
+    copy_ms = ctx.copy_probe(1 << 30, 5)
+    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    log(f"copy probe: 1 GiB in {copy_ms:.3f} ms, {copy_gbs:.0f} GB/s read+write")
+
     # ---- SpMV kernel timing: HIP events around every SpMV launch of one instrumented solve
     ctx.set_timing(True)
     dm.assemble()
@@ -367,6 +382,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "bytes_per_launch": B32,
+            "copy_probe_GBs": copy_gbs,
             "avg_launch_us": spmv_us,
             "launches": launches,
             "instrumented_solve_ms": t_instr * 1e3,
@@ -384,7 +400,11 @@ def main():
                                     "outer": rep_rs.outer, "relres": rep_rs.relres}
     traffic = _pmc_traffic(args)
     if traffic is not None:
+        # HBM bytes per launch from the committed counter profile, and the
+        # bandwidth they imply at this run's launch time
         out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_GBs"] = traffic / (spmv_us * 1e-6) / 1e9
+        out["roofline"]["traffic_frac"] = out["roofline"]["traffic_GBs"] / HBM_PEAK_GBS
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         log(f"CPU baseline with {nthreads} threads ...")

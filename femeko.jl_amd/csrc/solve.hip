@@ -102,6 +102,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const int32_t *__restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
+// Vectors per thread that K2 and K3 load before they reduce the previous
+// kernel's partials: the loads are in flight while the block waits on the
+// reduction (a band of 4096 rows is 4 vectors per thread in FP32).
+constexpr int kPrefetch = 4;
+
 // K2: x += alpha p; r -= alpha q; partials of r.z and r.r
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *__restrict__ r,
@@ -114,6 +119,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *_
                                                          int nblocks, CgState *__restrict__ st, int parity) {
     if (st->done_at[parity]) return;
     __shared__ double sh[kBlock / 64 + 1];
+    const int lb = logical_block(nblocks);
+    const int64_t r0 = static_cast<int64_t>(lb) * chunk;  // chunk is a multiple of 64
+    const int64_t r1 = min(n, r0 + chunk);
+    // 16-byte vector loads and stores over the band, scalar tail at n
+    constexpr int VW = 16 / sizeof(T);
+    using V = Vec16<T>;
+    const int64_t v1 = r1 > r0 ? r1 / VW : r0 / VW;
+    const int64_t j0 = r0 / VW + threadIdx.x;
+    V D[kPrefetch], P[kPrefetch], Q[kPrefetch], X[kPrefetch], R[kPrefetch];
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int64_t j = j0 + u * kBlock;
+        if (j < v1) {
+            D[u] = reinterpret_cast<const V *>(dinv)[j];
+            P[u] = reinterpret_cast<const V *>(p)[j];
+            Q[u] = reinterpret_cast<const V *>(q)[j];
+            X[u] = reinterpret_cast<const V *>(x)[j];
+            R[u] = reinterpret_cast<const V *>(r)[j];
+        }
+    }
     const double rho = sum_partials<kBlock>(part_rz_cur, nblocks, sh);
     const double pq = sum_partials<kBlock>(part_pq, nblocks, sh);
     if (!(pq > 0.0) || !isfinite(pq)) {
@@ -126,30 +151,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *_
         return;
     }
     const T alpha = static_cast<T>(rho / pq);
-    const int lb = logical_block(nblocks);
-    const int64_t r0 = static_cast<int64_t>(lb) * chunk;  // chunk is a multiple of 64
-    const int64_t r1 = min(n, r0 + chunk);
     double rz = 0.0, rr = 0.0;
-    // 16-byte vector loads and stores over the band, scalar tail at n
-    constexpr int VW = 16 / sizeof(T);
-    using V = Vec16<T>;
-    const int64_t v1 = r1 > r0 ? r1 / VW : r0 / VW;
-    for (int64_t j = r0 / VW + threadIdx.x; j < v1; j += kBlock) {
-        const V D = reinterpret_cast<const V *>(dinv)[j];
-        const V P = reinterpret_cast<const V *>(p)[j];
-        const V Q = reinterpret_cast<const V *>(q)[j];
-        V X = reinterpret_cast<const V *>(x)[j];
-        V R = reinterpret_cast<const V *>(r)[j];
+    auto step = [&](V &Dv, const V &Pv, const V &Qv, V &Xv, V &Rv) {
 #pragma unroll
         for (int e = 0; e < VW; ++e) {
-            X.v[e] += alpha * P.v[e];
-            const T ri = D.v[e] != T(0) ? R.v[e] - alpha * Q.v[e] : T(0);
-            R.v[e] = ri;
-            rz += static_cast<double>(ri) * static_cast<double>(D.v[e] * ri);
+            Xv.v[e] += alpha * Pv.v[e];
+            const T ri = Dv.v[e] != T(0) ? Rv.v[e] - alpha * Qv.v[e] : T(0);
+            Rv.v[e] = ri;
+            rz += static_cast<double>(ri) * static_cast<double>(Dv.v[e] * ri);
             rr += static_cast<double>(ri) * static_cast<double>(ri);
         }
-        reinterpret_cast<V *>(x)[j] = X;
-        reinterpret_cast<V *>(r)[j] = R;
+    };
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int64_t j = j0 + u * kBlock;
+        if (j < v1) {
+            step(D[u], P[u], Q[u], X[u], R[u]);
+            reinterpret_cast<V *>(x)[j] = X[u];
+            reinterpret_cast<V *>(r)[j] = R[u];
+        }
+    }
+    for (int64_t j = j0 + kPrefetch * kBlock; j < v1; j += kBlock) {
+        V Dv = reinterpret_cast<const V *>(dinv)[j];
+        const V Pv = reinterpret_cast<const V *>(p)[j];
+        const V Qv = reinterpret_cast<const V *>(q)[j];
+        V Xv = reinterpret_cast<const V *>(x)[j];
+        V Rv = reinterpret_cast<const V *>(r)[j];
+        step(Dv, Pv, Qv, Xv, Rv);
+        reinterpret_cast<V *>(x)[j] = Xv;
+        reinterpret_cast<V *>(r)[j] = Rv;
     }
     for (int64_t i = max(r0, v1 * VW) + threadIdx.x; i < r1; i += kBlock) {
         const T d = dinv[i];
@@ -179,6 +209,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
                                                         int parity) {
     if (st->done_at[parity]) return;
     __shared__ double sh[kBlock / 64 + 1];
+    const int lb = logical_block(nblocks);
+    const int64_t r0 = static_cast<int64_t>(lb) * chunk;  // chunk is a multiple of 64
+    const int64_t r1 = min(n, r0 + chunk);
+    constexpr int VW = 16 / sizeof(T);
+    using V = Vec16<T>;
+    const int64_t v1 = r1 > r0 ? r1 / VW : r0 / VW;
+    const int64_t j0 = r0 / VW + threadIdx.x;
+    V D[kPrefetch], R[kPrefetch], P[kPrefetch];
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int64_t j = j0 + u * kBlock;
+        if (j < v1) {
+            D[u] = reinterpret_cast<const V *>(dinv)[j];
+            R[u] = reinterpret_cast<const V *>(r)[j];
+            P[u] = reinterpret_cast<const V *>(p)[j];
+        }
+    }
     const double rho_new = sum_partials<kBlock>(part_rz_next, nblocks, sh);
     const double rho = sum_partials<kBlock>(part_rz_cur, nblocks, sh);
     if (blockIdx.x == 0) {
@@ -203,19 +250,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
         }
     }
     const T beta = static_cast<T>(rho_new / rho);
-    const int lb = logical_block(nblocks);
-    const int64_t r0 = static_cast<int64_t>(lb) * chunk;  // chunk is a multiple of 64
-    const int64_t r1 = min(n, r0 + chunk);
-    constexpr int VW = 16 / sizeof(T);
-    using V = Vec16<T>;
-    const int64_t v1 = r1 > r0 ? r1 / VW : r0 / VW;
-    for (int64_t j = r0 / VW + threadIdx.x; j < v1; j += kBlock) {
-        const V D = reinterpret_cast<const V *>(dinv)[j];
-        const V R = reinterpret_cast<const V *>(r)[j];
-        V P = reinterpret_cast<const V *>(p)[j];
 #pragma unroll
-        for (int e = 0; e < VW; ++e) P.v[e] = D.v[e] * R.v[e] + beta * P.v[e];
-        reinterpret_cast<V *>(p)[j] = P;
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int64_t j = j0 + u * kBlock;
+        if (j < v1) {
+#pragma unroll
+            for (int e = 0; e < VW; ++e) P[u].v[e] = D[u].v[e] * R[u].v[e] + beta * P[u].v[e];
+            reinterpret_cast<V *>(p)[j] = P[u];
+        }
+    }
+    for (int64_t j = j0 + kPrefetch * kBlock; j < v1; j += kBlock) {
+        const V Dv = reinterpret_cast<const V *>(dinv)[j];
+        const V Rv = reinterpret_cast<const V *>(r)[j];
+        V Pv = reinterpret_cast<const V *>(p)[j];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) Pv.v[e] = Dv.v[e] * Rv.v[e] + beta * Pv.v[e];
+        reinterpret_cast<V *>(p)[j] = Pv;
     }
     for (int64_t i = max(r0, v1 * VW) + threadIdx.x; i < r1; i += kBlock) p[i] = dinv[i] * r[i] + beta * p[i];
 }
@@ -517,6 +567,12 @@ __global__ void k_masked_copy(double *__restrict__ dst, const double *__restrict
 __global__ void k_scale_to_f32(const double *__restrict__ a, float *__restrict__ b, double s, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) b[i] = static_cast<float>(a[i] * s);
+}
+
+// plain 16-byte streaming copy (bandwidth probe)
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 template <typename T>
@@ -1467,6 +1523,32 @@ int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms) {
     if (launches) *launches = ctx->timing.launches;
     if (total_ms) *total_ms = ctx->timing.total_ms;
     return FK_OK;
+}
+
+int fk_copy_probe(fk_context *ctx, int64_t bytes, int32_t reps, double *ms_per_copy) {
+    if (!ctx || bytes < 16 || reps < 1 || !ms_per_copy) return FK_ERR_INVALID_ARG;
+    return fk::guard(ctx, "fk_copy_probe", [&] {
+        hipStream_t st = ctx->stream;
+        const int64_t n = bytes / 16;
+        fk::DevBuf<uint4> src(static_cast<size_t>(n)), dst(static_cast<size_t>(n));
+        FK_HIP(hipMemsetAsync(src.get(), 0, n * 16, st));
+        const int64_t grid = std::min<int64_t>((n + 255) / 256, 16 * static_cast<int64_t>(ctx->num_cus));
+        hipEvent_t e0, e1;
+        FK_HIP(hipEventCreate(&e0));
+        FK_HIP(hipEventCreate(&e1));
+        fk::k_copy16<<<static_cast<unsigned>(grid), 256, 0, st>>>(src.get(), dst.get(), n);  // warm-up
+        FK_HIP(hipEventRecord(e0, st));
+        for (int i = 0; i < reps; ++i)
+            fk::k_copy16<<<static_cast<unsigned>(grid), 256, 0, st>>>(src.get(), dst.get(), n);
+        FK_HIP(hipEventRecord(e1, st));
+        FK_HIP(hipGetLastError());
+        FK_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        FK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_per_copy = static_cast<double>(ms) / reps;
+    });
 }
 
 int fk_set_spmv_format(fk_mesh *m, int32_t format) {

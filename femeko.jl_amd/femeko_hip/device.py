@@ -47,6 +47,13 @@ class Context:
         self.check(self._L.fk_spmv_timing(self.handle, ctypes.byref(n), ctypes.byref(ms)), "fk_spmv_timing")
         return int(n.value), float(ms.value)
 
+    def copy_probe(self, nbytes: int = 1 << 30, reps: int = 5) -> float:
+        """Average milliseconds of one device-to-device copy of nbytes
+        (read nbytes + write nbytes) with the library's streaming kernel."""
+        ms = ctypes.c_double(0.0)
+        self.check(self._L.fk_copy_probe(self.handle, int(nbytes), int(reps), ctypes.byref(ms)), "fk_copy_probe")
+        return float(ms.value)
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             self._L.fk_context_destroy(self.handle)

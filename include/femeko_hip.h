@@ -244,6 +244,12 @@ int fk_batch_set_launch(fk_batch *batch, int32_t lanes, int32_t block);
  * fk_set_timing(ctx, 1). Returns launches and total milliseconds. */
 int fk_set_timing(fk_context *ctx, int32_t enable);
 int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms);
+/* Streaming-bandwidth probe: copy `bytes` (a multiple of 16 is used) between
+ * two device buffers `reps` times with a plain 16-byte-per-lane kernel
+ * (k_copy16) and return the average milliseconds per copy. Reads and writes
+ * `bytes` each per copy: the practical HBM ceiling of the device, and the
+ * kernel of known traffic that calibrates hardware byte counters. */
+int fk_copy_probe(fk_context *ctx, int64_t bytes, int32_t reps, double *ms_per_copy);
 /* SpMV tuning: lanes per row (4, 8, 16, 32, 64) of the CSR kernels. */
 int fk_set_lanes_per_row(fk_mesh *mesh, int32_t lanes);
 /* Matrix format of the solver's SpMV: FK_SPMV_CSR (the assembled CSR) or a

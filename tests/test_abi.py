@@ -50,6 +50,11 @@ def test_null_arguments_rejected_without_gpu():
     assert L.fk_set_timing(None, 1) == _lib.FK_ERR_INVALID_ARG
     v = np.zeros(4)
     assert L.fk_values(None, _lib.dptr(v)) == _lib.FK_ERR_INVALID_ARG
+    ms = ctypes.c_double()
+    assert L.fk_copy_probe(None, 1 << 20, 1, ctypes.byref(ms)) == _lib.FK_ERR_INVALID_ARG
+    assert L.fk_sell_info(None, None, None, None) == _lib.FK_ERR_INVALID_ARG
+    assert L.fk_set_sell_shape(None, 256, 1) == _lib.FK_ERR_INVALID_ARG
+    assert L.fk_set_spmv_format(None, 1) == _lib.FK_ERR_INVALID_ARG
 
 
 def test_header_cites_reference_for_each_entry_point():

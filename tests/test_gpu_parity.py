@@ -372,3 +372,12 @@ def test_error_statuses():
         assert rep.relres < 1e-12
     finally:
         dm.close()
+
+
+def test_copy_probe_bandwidth_is_physical():
+    ctx = context(0)
+    ms = ctx.copy_probe(1 << 28, 3)
+    gbs = 2 * (1 << 28) / (ms * 1e-3) / 1e9
+    assert 100.0 < gbs < 8000.0  # below the HBM peak, far above PCIe
+    with pytest.raises(FemekoHIPError):
+        ctx.copy_probe(8, 1)
