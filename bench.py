@@ -17,7 +17,7 @@ events around every SpMV launch of an instrumented solve), and the CPU oracle
 timed on a bounded sample of the same workload.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160] [--no-cpu]
-                [--shape 1024,1] [--delta 0.5] [--format sell64|sell64-rowidx|csr] [--workload boxes|batch|froehlich|adjoint]
+                [--shape 1024,1] [--delta 0.9] [--format sell64|sell64-rowidx|csr] [--workload boxes|batch|froehlich|adjoint]
 """
 import argparse
 import json
@@ -235,7 +235,7 @@ def main():
     ap.add_argument("--n", type=int, default=160)
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--format", default="sell64", choices=["csr", "sell64", "sell64-rowidx"])
-    ap.add_argument("--delta", type=float, default=0.5,
+    ap.add_argument("--delta", type=float, default=0.9,
                     help="FP32 solve: residual drop that triggers an FP64 residual replacement")
     ap.add_argument("--shape", default="1024,1",
                     help="SELL SpMV launch shape: threads per block, slices in flight per wave")

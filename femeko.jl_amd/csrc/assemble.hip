@@ -10,6 +10,18 @@
 // conflict free (no atomics, deterministic, and entry (i,j) and (j,i) receive
 // identical sums, so the matrix is exactly symmetric). The atomic variant
 // covers all elements in one launch with hardware FP64 atomics.
+//
+// The gather variant turns the scatter around: one thread per matrix row
+// walks the elements incident to its node (in element order), recomputes each
+// element's gradients and adds the element's row of K into the row's values
+// held in LDS, then the block writes its rows' values once, coalesced. No
+// colour passes (each colour pass of the scatter touches nearly every cache
+// line of the value array), no atomics; entry (i,j) and (j,i) receive the
+// same terms in the same (element) order, so the matrix stays exactly
+// symmetric and runs are bitwise reproducible.
+#include <algorithm>
+#include <thread>
+
 #include "device.h"
 #include "internal.h"
 
@@ -80,6 +92,67 @@ __global__ __launch_bounds__(256) void k_assemble(const int4 *__restrict__ tet, 
     scatter16<ATOMIC>(val, slot, e, k);
 }
 
+// Row-gather assembly: thread = row i (one wavefront = one slice of 64
+// rows, kGatherRows rows per block). The incidences of slice s are stored
+// transposed from sptr[s] (entry k of lane l at sptr[s] + 64 k + l, width[s]
+// entries per lane, kNoInc padding) so each load is coalesced: w = element <<
+// 2 | local index of the row's node, elements in caller order, ascending per
+// node; off packs the positions (0..255, relative to rowptr[i]) of the
+// element's 4 nodes in row i. Elements are read in caller order, whose
+// spatial locality (neighbouring rows share most elements) keeps them in L2.
+// The block's value segment [rowptr[i0], rowptr[i0 + rows]) is accumulated
+// in LDS (dynamic size) and written once.
+constexpr uint32_t kNoInc = 0xffffffffu;
+
+__global__ __launch_bounds__(kGatherRows) void k_assemble_gather(
+    const int32_t *__restrict__ rowptr, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
+    const uint32_t *__restrict__ inc, const uint32_t *__restrict__ off, const int4 *__restrict__ tet,
+    const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+    const double *__restrict__ coef, int64_t nv, double *__restrict__ val) {
+    extern __shared__ double acc[];
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kGatherRows;
+    const int64_t i = i0 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t s = i >> 6;  // uniform over the wave
+    const int32_t base = rowptr[i0];
+    const int32_t seg = rowptr[min(i0 + kGatherRows, nv)] - base;
+    for (int32_t k = threadIdx.x; k < seg; k += kGatherRows) acc[k] = 0.0;
+    __syncthreads();
+    if (s * 64 < nv) {
+        double *a = acc + (rowptr[min(i, nv)] - base);
+        const int64_t sb = sptr[s] + lane;
+        const int W = width[s];
+        constexpr int U[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+        for (int k = 0; k < W; ++k) {
+            const uint32_t w = inc[sb + 64 * k];
+            if (w == kNoInc) continue;
+            const uint32_t o = off[sb + 64 * k];
+            const int64_t e = w >> 2;
+            const int la = static_cast<int>(w & 3u);
+            const Grad G = element_gradients(X, Y, Z, tet[e]);
+            const double sc = fabs(G.det) * (1.0 / 6.0) * coef[e];
+            // all 10 unique local values by one fixed instruction sequence;
+            // row i and row j then add the very same value for (i,j), (j,i)
+            double kv[10];
+            local_values(G, sc, kv);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const double v = la == 0 ? kv[U[0][b]] : la == 1 ? kv[U[1][b]] : la == 2 ? kv[U[2][b]] : kv[U[3][b]];
+                a[(o >> (8 * b)) & 255u] += v;
+            }
+        }
+    }
+    __syncthreads();
+    for (int32_t k = threadIdx.x; k < seg; k += kGatherRows) val[base + k] = acc[k];
+}
+
+// coef_nat[perm[e]] = coef[e] (colour order -> caller order)
+__global__ void k_coef_to_caller(const double *__restrict__ coef, const int32_t *__restrict__ perm, int64_t nt,
+                                 double *__restrict__ coef_nat) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < nt) coef_nat[perm[e]] = coef[e];
+}
+
 // Newton tangent of src/FEM3D.jl:79-119:
 // At_ij = VE dmu/|H| (H . grad_i)(H . grad_j), H and dmu in caller order.
 __global__ __launch_bounds__(256) void k_tangent(const int4 *__restrict__ tet, const double *__restrict__ X,
@@ -144,7 +217,8 @@ __global__ __launch_bounds__(256) void k_lagrange(const int4 *__restrict__ tet, 
 // ---------------------------------------------------------------------------
 void upload_coefficient(fk_mesh *m, const double *f) {
     hipStream_t st = m->ctx->stream;
-    DevBuf<double> df(m->nt);
+    m->coef_nat.ensure(m->nt);
+    DevBuf<double> &df = m->coef_nat;  // caller order, read by the row-gather assembly
     DevBuf<int32_t> bad(1);
     FK_HIP(hipMemcpyAsync(df.get(), f, m->nt * sizeof(double), hipMemcpyHostToDevice, st));
     FK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), st));
@@ -159,6 +233,105 @@ void upload_coefficient(fk_mesh *m, const double *f) {
         fail(FK_ERR_NONFINITE, std::to_string(h) + " non-finite coefficient value(s)");
     }
     m->have_coef = true;
+    m->coef_nat_valid = true;
+}
+
+// Incidence lists of the row-gather assembly (host build, once per mesh).
+// Unsupported (gather_ok = false) when a row has more than 256 entries or a
+// block's value segment exceeds kGatherLdsMax doubles.
+void build_gather(fk_mesh *m) {
+    if (m->gather_built) return;
+    m->gather_built = true;
+    m->gather_ok = false;
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = m->nv, nt = m->nt, nnz = m->nnz;
+    if (nt >= (int64_t(1) << 30)) return;
+    std::vector<int4> tc(static_cast<size_t>(nt)), tet(static_cast<size_t>(nt));
+    std::vector<int32_t> perm(static_cast<size_t>(nt));
+    std::vector<int32_t> rp(static_cast<size_t>(nv + 1)), ci(static_cast<size_t>(nnz));
+    FK_HIP(hipMemcpyAsync(tc.data(), m->tet.get(), nt * sizeof(int4), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(perm.data(), m->perm.get(), nt * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(rp.data(), m->rowptr.get(), (nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(ci.data(), m->colidx.get(), nnz * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < nv; ++i)
+        if (rp[i + 1] - rp[i] > 256) return;
+    int64_t seg_max = 0;
+    for (int64_t i0 = 0; i0 < nv; i0 += kGatherRows)
+        seg_max = std::max<int64_t>(seg_max, rp[std::min(i0 + kGatherRows, nv)] - rp[i0]);
+    if (seg_max > kGatherLdsMax) return;
+    for (int64_t e = 0; e < nt; ++e) tet[perm[e]] = tc[e];  // caller order
+    std::vector<int32_t>().swap(perm);
+    std::vector<int4>().swap(tc);
+    // node -> incidences, ascending caller element order
+    std::vector<int64_t> ptr(static_cast<size_t>(nv + 1), 0);
+    for (int64_t e = 0; e < nt; ++e) {
+        const int4 t = tet[e];
+        ++ptr[t.x + 1];
+        ++ptr[t.y + 1];
+        ++ptr[t.z + 1];
+        ++ptr[t.w + 1];
+    }
+    for (int64_t i = 0; i < nv; ++i) ptr[i + 1] += ptr[i];
+    std::vector<uint32_t> incl(static_cast<size_t>(4 * nt));
+    {
+        std::vector<int64_t> cur(ptr.begin(), ptr.end() - 1);
+        for (int64_t e = 0; e < nt; ++e) {
+            const int4 t = tet[e];
+            const int32_t nd[4] = {t.x, t.y, t.z, t.w};
+            for (int a = 0; a < 4; ++a)
+                incl[cur[nd[a]]++] = static_cast<uint32_t>(e << 2) | static_cast<uint32_t>(a);
+        }
+    }
+    // slices of 64 rows, transposed
+    const int64_t ns = (nv + 63) / 64;
+    std::vector<int32_t> width(static_cast<size_t>(ns));
+    std::vector<int64_t> sptr(static_cast<size_t>(ns + 1), 0);
+    for (int64_t s = 0; s < ns; ++s) {
+        int64_t w = 0;
+        for (int64_t r = s * 64; r < std::min(nv, s * 64 + 64); ++r) w = std::max(w, ptr[r + 1] - ptr[r]);
+        width[s] = static_cast<int32_t>(w);
+        sptr[s + 1] = sptr[s] + 64 * w;
+    }
+    const int64_t len = sptr[ns];
+    std::vector<uint32_t> inc(static_cast<size_t>(len), kNoInc), off(static_cast<size_t>(len), 0);
+    const int nth = static_cast<int>(std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> th;
+    for (int q = 0; q < nth; ++q)
+        th.emplace_back([&, q] {
+            for (int64_t s = ns * q / nth; s < ns * (q + 1) / nth; ++s)
+                for (int64_t r = s * 64; r < std::min(nv, s * 64 + 64); ++r) {
+                    const int lane = static_cast<int>(r - s * 64);
+                    const int32_t *b = ci.data() + rp[r], *en = ci.data() + rp[r + 1];
+                    for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k) {
+                        const uint32_t w = incl[k];
+                        const int4 t = tet[w >> 2];
+                        const int32_t nd[4] = {t.x, t.y, t.z, t.w};
+                        uint32_t o = 0;
+                        for (int c = 0; c < 4; ++c) {
+                            const int32_t *f = std::lower_bound(b, en, nd[c]);
+                            o |= static_cast<uint32_t>(f - b) << (8 * c);
+                        }
+                        const int64_t pos = sptr[s] + 64 * (k - ptr[r]) + lane;
+                        inc[pos] = w;
+                        off[pos] = o;
+                    }
+                }
+        });
+    for (auto &x : th) x.join();
+    m->tet_nat.alloc(nt);
+    m->inc_sptr.alloc(ns + 1);
+    m->inc_width.alloc(ns);
+    m->inc.alloc(len);
+    m->inc_off.alloc(len);
+    FK_HIP(hipMemcpyAsync(m->tet_nat.get(), tet.data(), nt * sizeof(int4), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(m->inc_sptr.get(), sptr.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(m->inc_width.get(), width.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(m->inc.get(), inc.data(), len * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(m->inc_off.get(), off.data(), len * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipStreamSynchronize(st));
+    m->gather_lds = static_cast<int32_t>(seg_max);
+    m->gather_ok = true;
 }
 
 // Enqueue the value assembly on the context stream (no host sync).
@@ -166,6 +339,26 @@ void assemble(fk_mesh *m, int method) {
     if (!m->have_coef) fail(FK_ERR_STATE, "no coefficient set");
     hipStream_t st = m->ctx->stream;
     m->val64.ensure(m->nnz);
+    if (method == FK_ASM_GATHER) {
+        build_gather(m);
+        if (m->gather_ok) {
+            if (!m->coef_nat_valid) {  // coefficient last written on the device in colour order
+                m->coef_nat.ensure(m->nt);
+                k_coef_to_caller<<<grid_for(m->nt, 256), 256, 0, st>>>(m->coef.get(), m->perm.get(), m->nt,
+                                                                      m->coef_nat.get());
+                m->coef_nat_valid = true;
+            }
+            const int64_t nb = (m->nv + kGatherRows - 1) / kGatherRows;
+            k_assemble_gather<<<static_cast<unsigned>(nb), kGatherRows,
+                                static_cast<size_t>(std::max(m->gather_lds, 1)) * sizeof(double), st>>>(
+                m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->inc_off.get(),
+                m->tet_nat.get(), m->X.get(), m->Y.get(), m->Z.get(), m->coef_nat.get(), m->nv, m->val64.get());
+            FK_HIP(hipGetLastError());
+            values_changed(m);
+            return;
+        }
+        method = FK_ASM_COLOURED;  // rows too long for the LDS segment
+    }
     k_fill_f64<<<grid_for(m->nnz, 256), 256, 0, st>>>(m->val64.get(), m->nnz, 0.0);
     if (method == FK_ASM_ATOMIC) {
         k_assemble<true><<<grid_for(m->nt, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
@@ -265,7 +458,8 @@ int fk_set_coefficient(fk_mesh *m, const double *f) {
 }
 
 int fk_assemble(fk_mesh *m, int32_t method) {
-    if (!m || (method != FK_ASM_COLOURED && method != FK_ASM_ATOMIC)) return FK_ERR_INVALID_ARG;
+    if (!m || (method != FK_ASM_COLOURED && method != FK_ASM_ATOMIC && method != FK_ASM_GATHER))
+        return FK_ERR_INVALID_ARG;
     return guard(m->ctx, "fk_assemble", [&] {
         fk::set_device(m->ctx);
         fk::assemble(m, method);
@@ -300,7 +494,7 @@ int fk_stiffness_matrix(fk_mesh *m, const double *f, int32_t base, int32_t *colp
     if (!m || !f || !colptr || !rowval || !nzval || (base != 0 && base != 1)) return FK_ERR_INVALID_ARG;
     int s = fk_set_coefficient(m, f);
     if (s) return s;
-    s = fk_assemble(m, FK_ASM_COLOURED);
+    s = fk_assemble(m, FK_ASM_GATHER);
     if (s) return s;
     s = fk_pattern(m, base, colptr, rowval);
     if (s) return s;

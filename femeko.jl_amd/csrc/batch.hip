@@ -738,7 +738,7 @@ int fk_batch_assemble(fk_batch *b) {
     if (!b) return FK_ERR_INVALID_ARG;
     return guard(b->ctx, "fk_batch_assemble", [&] {
         fk::set_device(b->ctx);
-        fk::assemble(b->mesh, FK_ASM_COLOURED);
+        fk::assemble(b->mesh, FK_ASM_GATHER);
     });
 }
 

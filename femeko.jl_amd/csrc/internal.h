@@ -95,7 +95,28 @@ struct CgState {
     double pad[2];
 };
 
+// State of the FP32 solve's residual replacements. The decisions are taken
+// on the device between iteration chunks, so the host only polls for the end.
+struct RuState {
+    int32_t act;        // this chunk: 0 none, 1 fold x and replace r, 2 drop x (FP32 breakdown)
+    int32_t mode;       // decided: 1 keep the direction p, 2 restart p = D^-1 r
+    int32_t finished;   // the solve is over (converged or failed)
+    int32_t status;     // its status
+    int32_t nrep;       // replacements done
+    int32_t restarts;   // direction restarts done
+    int64_t max_iter;   // FP32 iteration limit
+    double s;           // residual scale 1 / ||r0||
+    double rr64;        // last FP64 ||r||^2
+    double rr_last;     // scaled ||r||^2 at the last replacement
+    double thresh64;    // rtol^2 ||b||^2
+    double target2;     // the same in scaled units
+    double delta2;      // squared update trigger
+    double pad[2];
+};
+
 constexpr int kBlock = 256;      // threads per block of the solver kernels
+constexpr int kGatherRows = 128;      // rows per block of the row-gather assembly
+constexpr int kGatherLdsMax = 8192;   // its largest per-block value segment (doubles, 64 KB)
 constexpr int kNumXcd = 8;       // MI355X accelerator complex dies
 
 // Solver workspace of one precision.
@@ -104,15 +125,21 @@ struct PcgWork {
     DevBuf<T> x, r, p, q, dinv;
     DevBuf<double> part_pq, part_rz, part_rr, part_bb;  // part_rz holds 2 x nblocks
     DevBuf<CgState> state;
-    // captured chunk of iterations; it bakes in device addresses, so it is
-    // reset whenever a buffer its kernels read is reallocated
+    // captured chunks of iterations (plain PCG, and PCG + residual
+    // replacement); they bake in device addresses, so they are reset
+    // whenever a buffer their kernels read is reallocated
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
     int graph_lanes = 0;
+    hipGraphExec_t ru_graph = nullptr;
+    int ru_graph_cfg = -1;
     void reset_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
         graph_iters = 0;
+        if (ru_graph) (void)hipGraphExecDestroy(ru_graph);
+        ru_graph = nullptr;
+        ru_graph_cfg = -1;
     }
     ~PcgWork() { reset_graph(); }
 };
@@ -173,6 +200,8 @@ struct fk_mesh {
     fk::DevBuf<double> part_misc;     // block partials of field/energy/residual kernels
     fk::HostBuf<fk::CgState> hstate;  // pinned copies of the CG state
     fk::HostBuf<double> hpart;        // pinned copies of partials
+    fk::DevBuf<fk::RuState> ru;       // FP32 solve replacement state
+    fk::HostBuf<fk::RuState> hru;     // pinned copies (2, polled alternately)
 
     // SELL-64 copy of the matrix for the solver (CSR stays the storage format)
     int format = 1;                   // FK_SPMV_CSR, FK_SPMV_SELL64 (default), FK_SPMV_SELL64_ROWIDX
@@ -187,6 +216,18 @@ struct fk_mesh {
     int32_t bandwidth = 0;
     fk::DevBuf<double> sell_val64;
     fk::DevBuf<float> sell_val32;
+
+    // row-gather assembly: per 64-row slice, transposed incidence lists
+    // (element << 2 | local index, caller element order) and the element's 4
+    // positions in the row; connectivity and coefficient in caller order
+    bool gather_built = false, gather_ok = false;
+    int32_t gather_lds = 0;           // largest block value segment (doubles)
+    fk::DevBuf<int64_t> inc_sptr;
+    fk::DevBuf<int32_t> inc_width;
+    fk::DevBuf<uint32_t> inc, inc_off;
+    fk::DevBuf<int4> tet_nat;
+    fk::DevBuf<double> coef_nat;      // coefficient in caller order
+    bool coef_nat_valid = false;
 
     int lanes = 16;                   // lanes per CSR row in SpMV
     int sell_block = 1024;            // threads per block of the SELL SpMV

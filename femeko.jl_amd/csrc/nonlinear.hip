@@ -206,6 +206,7 @@ void nonlinear_froehlich(fk_mesh *m, const uint8_t *region_host, double Ms, doub
     k_fill_f64b<<<grid_for(nt, 256), 256, 0, st>>>(m->dmu.get(), nt, 0.0);
     k_init_phi_dev(m, 0);  // phi = g on fixed nodes, 0 elsewhere
     m->have_coef = true;
+    m->coef_nat_valid = false;  // the loop updates coef (colour order) on the device
     m->have_phi = true;
     const int nb = std::max(1, m->nblocks);
     m->part_misc.ensure(nb);

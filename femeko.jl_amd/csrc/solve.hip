@@ -433,12 +433,15 @@ __global__ __launch_bounds__(BLOCK) void k_cg_spmv_sell(const Sell<T, IDX> A, co
 }
 
 // FP64 r = -mask(A phi), SELL form
+// (gate: when given, the kernel runs only in a replacement step, act == 1)
 template <typename IDX>
 __global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const Sell<double, IDX> A,
                                                               const double *__restrict__ phi,
                                                               const uint8_t *__restrict__ mask,
                                                               double *__restrict__ r, double *__restrict__ part,
-                                                              int64_t nslices, int64_t chunk_slices, int nblocks) {
+                                                              int64_t nslices, int64_t chunk_slices, int nblocks,
+                                                              const RuState *__restrict__ gate) {
+    if (gate && gate->act != 1) return;
     __shared__ double sh[kBlock / 64 + 1];
     const int lb = logical_block(nblocks);
     const int64_t s0 = static_cast<int64_t>(lb) * chunk_slices;
@@ -526,7 +529,8 @@ __global__ __launch_bounds__(kBlock) void k_neg_residual(const int32_t *__restri
                                                          const double *__restrict__ phi,
                                                          const uint8_t *__restrict__ mask, double *__restrict__ r,
                                                          double *__restrict__ part, int64_t n, int64_t chunk,
-                                                         int nblocks) {
+                                                         int nblocks, const RuState *__restrict__ gate) {
+    if (gate && gate->act != 1) return;
     __shared__ double sh[kBlock / 64 + 1];
     const int lb = logical_block(nblocks);
     const int64_t r0 = static_cast<int64_t>(lb) * chunk;
@@ -1045,60 +1049,111 @@ static CgState run_pcg(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref)
     return out;
 }
 
-// Run exactly G iterations from parity 0 (kernels after a stop return at
-// once), then return the state (host sync). G must be even.
-template <typename T>
-static CgState run_chunk(fk_mesh *m, PcgWork<T> &w, int G) {
-    hipStream_t st = m->ctx->stream;
-    CgState *hs = m->hstate.ptr;
-    const bool timed = m->ctx->timing.enabled;
-    if (timed || !m->ctx->graphs) {
-        FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
-        FK_HIP(hipStreamSynchronize(st));
-        const int64_t before = hs->iters;
-        for (int k = 0; k < G; ++k) enqueue_iteration<T>(m, w, k & 1, timed);
-        FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
-        FK_HIP(hipStreamSynchronize(st));
-        if (timed) harvest_timing(m->ctx, hs->iters - before);
-        return *hs;
-    }
-    const int cfg = solver_config(m);
-    if (!w.graph || w.graph_iters != G || w.graph_lanes != cfg) {
-        w.reset_graph();
-        hipGraph_t graph;
-        FK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < G; ++k) enqueue_iteration<T>(m, w, k & 1, false);
-        FK_HIP(hipStreamEndCapture(st, &graph));
-        hipError_t e = hipGraphInstantiate(&w.graph, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        FK_HIP(e);
-        w.graph_iters = G;
-        w.graph_lanes = cfg;
-    }
-    FK_HIP(hipGraphLaunch(w.graph, st));
-    FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
-    FK_HIP(hipStreamSynchronize(st));
-    return *hs;
-}
-
 static double neg_residual(fk_mesh *m, const double *phi, double *r);
+static void enqueue_neg_residual(fk_mesh *m, const double *phi, double *r, const RuState *gate);
 
-// phi += x / s; x = 0
-template <typename T>
-__global__ void k_rr_fold(double *__restrict__ phi, T *__restrict__ x, double inv_s, int64_t n) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    phi[i] += inv_s * static_cast<double>(x[i]);
-    x[i] = T(0);
+constexpr int kMaxRestarts = 32;  // direction restarts before the FP32 solve gives up
+
+// ---- FP32 solve with FP64 reliable updates, decided on the device --------
+// After every chunk of G iterations, five small kernels run (no-ops unless
+// the check asks for a replacement):
+//   check   the recursion's residual fell by delta since the last replacement,
+//           or it stopped: fold (1) or, after an FP32 breakdown, drop (2) its
+//           increment
+//   fold    phi += x / s (act 1), x = 0
+//   resid   r64 = -mask(A64 phi) and its partials (act 1 only)
+//   decide  converged / failed / replace (keep p) / restart (p = D^-1 r), and
+//           the next stop threshold max(target, delta^2 ||r||^2)
+//   replace r = s r64, rho partials, and p on a restart; then resume
+__global__ void k_ru_check(const CgState *__restrict__ cg, RuState *__restrict__ ru) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ru->act = 0;
+    if (ru->finished) return;
+    const bool stopped = (cg->done_at[0] | cg->done_at[1]) != 0;
+    if (!stopped) {
+        // replace at the end of the chunk in which the residual fell by delta
+        if (cg->rr < ru->delta2 * ru->rr_last) ru->act = 1;
+        return;
+    }
+    const bool broken = cg->status == FK_ERR_SINGULAR || cg->status == FK_ERR_NONFINITE || !isfinite(cg->rr);
+    if (!broken) {
+        ru->act = 1;
+        return;
+    }
+    if (++ru->restarts > kMaxRestarts || cg->iters >= ru->max_iter) {
+        ru->finished = 1;
+        ru->status = cg->iters >= ru->max_iter ? FK_ERR_NOT_CONVERGED
+                                               : (cg->status != FK_OK ? cg->status : FK_ERR_NONFINITE);
+        return;
+    }
+    ru->act = 2;
 }
 
-// residual replacement: r = s r64 (free rows), partials of r.D^-1.r into the
-// parity-0 slot the next iteration reads as rho
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_rr_replace(T *__restrict__ r, const double *__restrict__ r64, double s,
-                                                       const T *__restrict__ dinv, double *__restrict__ part_rz,
-                                                       int64_t n, int64_t chunk, int nblocks) {
+__global__ void k_ru_fold(double *__restrict__ phi, T *__restrict__ x, const RuState *__restrict__ ru, int64_t n) {
+    const int act = ru->act;
+    if (act == 0) return;
+    const double inv_s = 1.0 / ru->s;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (act == 1) phi[i] += inv_s * static_cast<double>(x[i]);
+        x[i] = T(0);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ru_decide(const double *__restrict__ part, int nblocks,
+                                                      CgState *__restrict__ cg, RuState *__restrict__ ru) {
+    if (ru->act == 0) return;
     __shared__ double sh[kBlock / 64 + 1];
+    const double rr = ru->act == 1 ? sum_partials<kBlock>(part, nblocks, sh) : 0.0;
+    if (threadIdx.x != 0) return;
+    if (ru->act == 2) {
+        ru->mode = 2;  // r64 still holds the last true residual
+        return;
+    }
+    ru->rr64 = rr;
+    ru->nrep += 1;
+    const double s = ru->s;
+    if (rr < ru->thresh64) {
+        ru->finished = 1;
+        ru->status = FK_OK;
+        return;
+    }
+    if (!isfinite(rr)) {
+        ru->finished = 1;
+        ru->status = FK_ERR_NONFINITE;
+        return;
+    }
+    if (cg->iters >= ru->max_iter) {
+        ru->finished = 1;
+        ru->status = FK_ERR_NOT_CONVERGED;
+        return;
+    }
+    const double scaled = rr * s * s;
+    int mode = 1;
+    if (scaled > 4.0 * cg->rr) {
+        // the recursion drifted below the true residual: its direction no
+        // longer matches it, restart from p = D^-1 r
+        if (++ru->restarts > kMaxRestarts) {
+            ru->finished = 1;
+            ru->status = FK_ERR_NOT_CONVERGED;
+            return;
+        }
+        mode = 2;
+    }
+    ru->mode = mode;
+    ru->rr_last = scaled;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ru_replace(T *__restrict__ r, T *__restrict__ p,
+                                                       const double *__restrict__ r64, const T *__restrict__ dinv,
+                                                       double *__restrict__ part_rz, const RuState *__restrict__ ru,
+                                                       int64_t n, int64_t chunk, int nblocks) {
+    if (ru->act == 0 || ru->finished) return;
+    __shared__ double sh[kBlock / 64 + 1];
+    const double s = ru->s;
+    const bool restart = ru->mode == 2;
     const int lb = logical_block(nblocks);
     const int64_t r0 = static_cast<int64_t>(lb) * chunk;
     const int64_t r1 = min(n, r0 + chunk);
@@ -1107,28 +1162,47 @@ __global__ __launch_bounds__(kBlock) void k_rr_replace(T *__restrict__ r, const 
         const T d = dinv[i];
         const T ri = d != T(0) ? static_cast<T>(s * r64[i]) : T(0);
         r[i] = ri;
+        if (restart) p[i] = d * ri;
         rz += static_cast<double>(ri) * static_cast<double>(d * ri);
     }
     const double t = block_sum<kBlock>(rz, sh);
-    if (threadIdx.x == 0) part_rz[blockIdx.x] = t;
+    if (threadIdx.x == 0) part_rz[blockIdx.x] = t;  // parity-0 slot: rho of the next iteration
 }
 
-__global__ void k_rr_resume(CgState *__restrict__ st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->done_at[0] = 0;
-        st->done_at[1] = 0;
-        st->status = FK_OK;
+__global__ void k_ru_resume(CgState *__restrict__ cg, RuState *__restrict__ ru) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (ru->act != 0 && !ru->finished) {
+        cg->done_at[0] = 0;
+        cg->done_at[1] = 0;
+        cg->status = FK_OK;
     }
+    ru->act = 0;
+}
+
+// G iterations (G even: the chunk ends on parity 0) and the replacement step
+static void enqueue_ru_chunk(fk_mesh *m, PcgWork<float> &w, int G, bool timed) {
+    hipStream_t st = m->ctx->stream;
+    const int nb = m->nblocks;
+    for (int k = 0; k < G; ++k) enqueue_iteration<float>(m, w, k & 1, timed);
+    RuState *ru = m->ru.get();
+    k_ru_check<<<1, 64, 0, st>>>(w.state.get(), ru);
+    k_ru_fold<float><<<nb, kBlock, 0, st>>>(m->phi.get(), w.x.get(), ru, m->nv);
+    enqueue_neg_residual(m, m->phi.get(), m->r64.get(), ru);
+    k_ru_decide<<<1, kBlock, 0, st>>>(m->part_misc.get(), nb, w.state.get(), ru);
+    k_ru_replace<float><<<nb, kBlock, 0, st>>>(w.r.get(), w.p.get(), m->r64.get(), w.dinv.get(), w.part_rz.get(), ru,
+                                               m->nv, m->chunk, nb);
+    k_ru_resume<<<1, 64, 0, st>>>(w.state.get(), ru);
+    FK_HIP(hipGetLastError());
 }
 
 // FP32 Jacobi-PCG whose residual is replaced by the FP64 true residual
 // -mask(A64 phi), phi accumulating the FP32 increments in FP64 ("reliable
-// updates"). A replacement happens at the end of an iteration chunk once the
-// FP32 recursive residual has fallen by the factor `delta` since the last
-// replacement, or once it claims convergence; the search direction is kept,
-// so the Krylov space is not restarted. Convergence is decided on the FP64
-// true residual: ||r|| < rtol ||b||. Returns (iterations, replacements,
-// final ||r||^2).
+// updates"). A replacement happens at the end of the iteration chunk in which
+// the FP32 recursive residual has fallen by the factor `delta` since the last
+// replacement, or once the recursion stops (target reached, breakdown); it
+// keeps the search direction, so the Krylov space is not restarted.
+// Convergence is decided on the FP64 true residual: ||r|| < rtol ||b||.
+// Returns (iterations, replacements, final ||r||^2).
 static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double bb, double delta, int64_t *iters,
                              int32_t *nrep, double *rr_out) {
     hipStream_t st = m->ctx->stream;
@@ -1142,124 +1216,152 @@ static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double b
     *rr_out = rr;
     if (rr < thresh) return FK_OK;
     const double s = 1.0 / std::sqrt(rr);
+    const double target2 = thresh * s * s;
     sync_solver_values<float>(m);
     k_scale_to_f32<<<grid_for(nv, 256), 256, 0, st>>>(m->r64.get(), w.r.get(), s, nv);
     k_cg_init<float><<<nb, kBlock, 0, st>>>(w.x.get(), w.r.get(), w.p.get(), w.dinv.get(), w.part_rz.get(),
                                             w.part_bb.get(), nv, m->chunk, nb);
-    // the FP32 recursion stops (mid-chunk) once it reaches the target; the
-    // FP64 true residual then decides, and the solve resumes if it is short
-    k_cg_start<<<1, kBlock, 0, st>>>(w.part_bb.get(), nb, -1.0, rtol * std::sqrt(bb) * s, max_iter,
-                                     w.state.get());
+    // the recursion itself stops only at the target (scaled units)
+    k_cg_start<<<1, kBlock, 0, st>>>(w.part_bb.get(), nb, 1.0, std::sqrt(target2), max_iter, w.state.get());
     FK_HIP(hipGetLastError());
+    if (!m->ru) {
+        m->ru.alloc(1);
+        m->hru.alloc(2);
+    }
+    RuState init{};
+    init.max_iter = max_iter;
+    init.s = s;
+    init.rr64 = rr;
+    init.rr_last = 1.0;
+    init.thresh64 = thresh;
+    init.target2 = target2;
+    init.delta2 = delta * delta;
+    FK_HIP(hipMemcpyAsync(m->ru.get(), &init, sizeof(RuState), hipMemcpyHostToDevice, st));
+    m->part_misc.ensure(nb);
     const int G = nv > 200000 ? 32 : 16;
-    const double target2 = rtol * rtol * bb * s * s;  // the FP64 target in scaled units
-    double rr_last = 1.0;                              // scaled ||r||^2 at the last replacement
-    int status = FK_OK;
     const char *tr = std::getenv("FEMEKO_HIP_TRACE");
     const bool trace = tr && tr[0] == '1';
-    constexpr int kMaxRestarts = 32;
-    int restarts = 0;
-    // r32 = s r64, x32 = 0, p = D^-1 r32 (rho into the parity-0 slot)
-    auto restart_direction = [&] {
-        k_scale_to_f32<<<grid_for(nv, 256), 256, 0, st>>>(m->r64.get(), w.r.get(), s, nv);
-        k_cg_init<float><<<nb, kBlock, 0, st>>>(w.x.get(), w.r.get(), w.p.get(), w.dinv.get(), w.part_rz.get(),
-                                                w.part_bb.get(), nv, m->chunk, nb);
-        FK_HIP(hipGetLastError());
-    };
-    while (true) {
-        const CgState cs = run_chunk<float>(m, w, G);
-        *iters = cs.iters;
-        const bool need = cg_stopped(cs) || cs.rr < delta * delta * rr_last || cs.rr < target2 ||
-                          cs.iters >= max_iter;
+    const bool timed = m->ctx->timing.enabled;
+    RuState *hr = m->hru.ptr;
+    CgState *hs = m->hstate.ptr;
+    auto report = [&](const RuState &x) {
         if (trace)
-            std::fprintf(stderr, "[fk trace] its %lld status %d stopped %d rr32 %.3e rr_last %.3e replace %d\n",
-                         static_cast<long long>(cs.iters), cs.status, cg_stopped(cs) ? 1 : 0, cs.rr, rr_last,
-                         need ? 1 : 0);
-        if (!need) continue;
-        const bool broken = cs.status == FK_ERR_SINGULAR || cs.status == FK_ERR_NONFINITE || !std::isfinite(cs.rr);
-        if (broken) {
-            // the FP32 recursion broke down since the last update: drop its
-            // increment (phi and r64 still hold the last FP64 state) and
-            // restart the direction from that residual
-            if (++restarts > kMaxRestarts) {
-                status = cs.status != FK_OK ? cs.status : FK_ERR_NONFINITE;
-                break;
+            std::fprintf(stderr, "[fk trace] replacements %d restarts %d rr64 %.3e rr_last(scaled) %.3e%s\n", x.nrep,
+                         x.restarts, x.rr64, x.rr_last, x.finished ? " finished" : "");
+    };
+    if (timed) {
+        // synchronous chunks, events around every SpMV
+        int64_t before = 0;
+        while (true) {
+            enqueue_ru_chunk(m, w, G, true);
+            FK_HIP(hipMemcpyAsync(hr, m->ru.get(), sizeof(RuState), hipMemcpyDeviceToHost, st));
+            FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
+            FK_HIP(hipStreamSynchronize(st));
+            harvest_timing(m->ctx, hs->iters - before);
+            before = hs->iters;
+            report(hr[0]);
+            if (hr[0].finished) break;
+        }
+    } else {
+        hipGraphExec_t *gx = nullptr;
+        if (m->ctx->graphs) {
+            const int cfg = solver_config(m) * 64 + G;
+            if (!w.ru_graph || w.ru_graph_cfg != cfg) {
+                if (w.ru_graph) (void)hipGraphExecDestroy(w.ru_graph);
+                w.ru_graph = nullptr;
+                hipGraph_t graph;
+                FK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                enqueue_ru_chunk(m, w, G, false);
+                FK_HIP(hipStreamEndCapture(st, &graph));
+                hipError_t e = hipGraphInstantiate(&w.ru_graph, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                FK_HIP(e);
+                w.ru_graph_cfg = cfg;
             }
-            if (cs.iters >= max_iter) {
-                status = FK_ERR_NOT_CONVERGED;
-                break;
+            gx = &w.ru_graph;
+        }
+        auto launch = [&] {
+            if (gx)
+                FK_HIP(hipGraphLaunch(*gx, st));
+            else
+                enqueue_ru_chunk(m, w, G, false);
+        };
+        // one chunk in flight ahead of the state copy being examined
+        hipEvent_t ev[2];
+        FK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        FK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        try {
+            launch();
+            FK_HIP(hipMemcpyAsync(&hr[0], m->ru.get(), sizeof(RuState), hipMemcpyDeviceToHost, st));
+            FK_HIP(hipEventRecord(ev[0], st));
+            int cur = 0;
+            int last_rep = -1;
+            while (true) {
+                launch();
+                FK_HIP(hipMemcpyAsync(&hr[1 - cur], m->ru.get(), sizeof(RuState), hipMemcpyDeviceToHost, st));
+                FK_HIP(hipEventRecord(ev[1 - cur], st));
+                FK_HIP(hipEventSynchronize(ev[cur]));
+                if (hr[cur].nrep != last_rep) {
+                    report(hr[cur]);
+                    last_rep = hr[cur].nrep;
+                }
+                if (hr[cur].finished) break;
+                cur = 1 - cur;
             }
-            restart_direction();
-            if (trace) std::fprintf(stderr, "[fk trace] FP32 breakdown: increment dropped, direction restarted\n");
-            k_rr_resume<<<1, 64, 0, st>>>(w.state.get());
-            FK_HIP(hipGetLastError());
-            continue;
+            FK_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
+            throw;
         }
-        k_rr_fold<float><<<grid_for(nv, 256), 256, 0, st>>>(m->phi.get(), w.x.get(), 1.0 / s, nv);
-        FK_HIP(hipGetLastError());
-        rr = neg_residual(m, m->phi.get(), m->r64.get());
-        *nrep += 1;
-        *rr_out = rr;
-        rr_last = rr * s * s;
-        if (trace) std::fprintf(stderr, "[fk trace] replacement: true rr (scaled) %.3e\n", rr_last);
-        if (rr < thresh) break;
-        if (!std::isfinite(rr)) {
-            status = FK_ERR_NONFINITE;
-            break;
-        }
-        if (cs.iters >= max_iter) {
-            status = FK_ERR_NOT_CONVERGED;
-            break;
-        }
-        if (rr_last > 4.0 * cs.rr) {
-            // the recursion drifted below the true residual: the search
-            // direction no longer matches it, restart from p = D^-1 r
-            if (++restarts > kMaxRestarts) {
-                status = FK_ERR_NOT_CONVERGED;
-                break;
-            }
-            restart_direction();
-            if (trace) std::fprintf(stderr, "[fk trace] direction restarted\n");
-        } else {
-            k_rr_replace<float><<<nb, kBlock, 0, st>>>(w.r.get(), m->r64.get(), s, w.dinv.get(), w.part_rz.get(),
-                                                       nv, m->chunk, nb);
-        }
-        k_rr_resume<<<1, 64, 0, st>>>(w.state.get());
-        FK_HIP(hipGetLastError());
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
     }
+    FK_HIP(hipMemcpyAsync(&hr[0], m->ru.get(), sizeof(RuState), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
-    return status;
+    *iters = hs->iters;
+    *nrep = hr[0].nrep;
+    *rr_out = hr[0].rr64;
+    return hr[0].status;
 }
 
 template <int LPR>
-static void launch_neg_residual(fk_mesh *m, const double *phi, double *r, double *part) {
+static void launch_neg_residual(fk_mesh *m, const double *phi, double *r, double *part, const RuState *gate) {
     k_neg_residual<LPR><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(m->rowptr.get(), m->colidx.get(),
                                                                    m->val64.get(), phi, m->mask.get(), r, part,
-                                                                   m->nv, m->chunk, m->nblocks);
+                                                                   m->nv, m->chunk, m->nblocks, gate);
+}
+
+// enqueue r = -mask(A64 phi) and its block partials of r.r into part_misc
+// (no host sync); gate as in the kernels
+static void enqueue_neg_residual(fk_mesh *m, const double *phi, double *r, const RuState *gate) {
+    hipStream_t st = m->ctx->stream;
+    if (m->format != FK_SPMV_CSR) {
+        if (m->sell_short)
+            k_neg_residual_sell<int16_t><<<m->nblocks, kBlock, 0, st>>>(
+                sell_view<double, int16_t>(m, m->sell_dcol.get()), phi, m->mask.get(), r, m->part_misc.get(),
+                m->nslices, m->sell_chunk, m->nblocks, gate);
+        else
+            k_neg_residual_sell<int32_t><<<m->nblocks, kBlock, 0, st>>>(
+                sell_view<double, int32_t>(m, m->sell_col.get()), phi, m->mask.get(), r, m->part_misc.get(),
+                m->nslices, m->sell_chunk, m->nblocks, gate);
+    } else switch (m->lanes) {
+        case 4: launch_neg_residual<4>(m, phi, r, m->part_misc.get(), gate); break;
+        case 8: launch_neg_residual<8>(m, phi, r, m->part_misc.get(), gate); break;
+        case 32: launch_neg_residual<32>(m, phi, r, m->part_misc.get(), gate); break;
+        case 64: launch_neg_residual<64>(m, phi, r, m->part_misc.get(), gate); break;
+        default: launch_neg_residual<16>(m, phi, r, m->part_misc.get(), gate); break;
+    }
+    FK_HIP(hipGetLastError());
 }
 
 // r = -mask(A64 phi); returns ||r||^2 (host sync)
 static double neg_residual(fk_mesh *m, const double *phi, double *r) {
     hipStream_t st = m->ctx->stream;
     m->part_misc.ensure(m->nblocks);
-    if (m->format != FK_SPMV_CSR) {
-        sync_solver_values<double>(m);
-        if (m->sell_short)
-            k_neg_residual_sell<int16_t><<<m->nblocks, kBlock, 0, st>>>(
-                sell_view<double, int16_t>(m, m->sell_dcol.get()), phi, m->mask.get(), r, m->part_misc.get(),
-                m->nslices, m->sell_chunk, m->nblocks);
-        else
-            k_neg_residual_sell<int32_t><<<m->nblocks, kBlock, 0, st>>>(
-                sell_view<double, int32_t>(m, m->sell_col.get()), phi, m->mask.get(), r, m->part_misc.get(),
-                m->nslices, m->sell_chunk, m->nblocks);
-    } else switch (m->lanes) {
-        case 4: launch_neg_residual<4>(m, phi, r, m->part_misc.get()); break;
-        case 8: launch_neg_residual<8>(m, phi, r, m->part_misc.get()); break;
-        case 32: launch_neg_residual<32>(m, phi, r, m->part_misc.get()); break;
-        case 64: launch_neg_residual<64>(m, phi, r, m->part_misc.get()); break;
-        default: launch_neg_residual<16>(m, phi, r, m->part_misc.get()); break;
-    }
-    FK_HIP(hipGetLastError());
+    if (m->format != FK_SPMV_CSR) sync_solver_values<double>(m);
+    enqueue_neg_residual(m, phi, r, nullptr);
     FK_HIP(hipMemcpyAsync(m->hpart.ptr, m->part_misc.get(), m->nblocks * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
     double s = 0.0;
@@ -1361,8 +1463,8 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
     } else if (precision == FK_PREC_F32_REFINED) {
         double rr = 0.0;
         // inner_rtol is the reliable-update trigger: replace once the FP32
-        // residual fell by this factor (values outside (0,1) mean 0.5)
-        const double delta = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 0.5;
+        // residual fell by this factor (values outside (0,1) mean 0.9)
+        const double delta = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 0.9;
         status = solve_f32_replace(m, rtol, max_iter, bb, delta, &total, &nout, &rr);
         rel = std::sqrt(rr / bb);
     } else {

@@ -29,6 +29,7 @@ FK_PREC_F32_REFINED = 2
 FK_PREC_F32_RESTART = 3
 FK_ASM_COLOURED = 0
 FK_ASM_ATOMIC = 1
+FK_ASM_GATHER = 2
 
 
 class FemekoHIPError(RuntimeError):

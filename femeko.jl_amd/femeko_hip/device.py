@@ -14,7 +14,8 @@ from ._lib import FemekoHIPError, dptr, fptr, iptr, uptr
 PRECISIONS = {"f64": _lib.FK_PREC_F64, "f32": _lib.FK_PREC_F32_REFINED,
               "f32-restart": _lib.FK_PREC_F32_RESTART,
               "fp64": _lib.FK_PREC_F64, "f32-refined": _lib.FK_PREC_F32_REFINED}
-METHODS = {"coloured": _lib.FK_ASM_COLOURED, "colored": _lib.FK_ASM_COLOURED, "atomic": _lib.FK_ASM_ATOMIC}
+METHODS = {"coloured": _lib.FK_ASM_COLOURED, "colored": _lib.FK_ASM_COLOURED, "atomic": _lib.FK_ASM_ATOMIC,
+           "gather": _lib.FK_ASM_GATHER}
 
 
 class Context:
@@ -139,7 +140,7 @@ class DeviceMesh:
         f = _f64(f, self.nt, "f")
         self.ctx.check(self._L.fk_set_coefficient(self.handle, dptr(f)), "fk_set_coefficient")
 
-    def assemble(self, method: str = "coloured") -> None:
+    def assemble(self, method: str = "gather") -> None:
         self.ctx.check(self._L.fk_assemble(self.handle, METHODS[method]), "fk_assemble")
 
     def values(self) -> np.ndarray:
@@ -188,10 +189,10 @@ class DeviceMesh:
               inner_rtol: float | None = None, max_outer: int = 50, warm_start: bool = False,
               raise_on_error: bool = True) -> SolveReport:
         """precision 'f64'; 'f32' (FP32 PCG with FP64 reliable updates, inner_rtol =
-        residual drop that triggers an update, default 0.5); 'f32-restart'
+        residual drop that triggers an update, default 0.9); 'f32-restart'
         (restarted FP32 PCG to inner_rtol, default 1e-5, inside FP64 refinement)."""
         if inner_rtol is None:
-            inner_rtol = 1e-5 if precision == "f32-restart" else 0.5
+            inner_rtol = 1e-5 if precision == "f32-restart" else 0.9
         it = ctypes.c_int64(0)
         outer = ctypes.c_int32(0)
         rel = ctypes.c_double(0.0)

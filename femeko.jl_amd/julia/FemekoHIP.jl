@@ -106,7 +106,7 @@ function stiffnessMatrix(mesh, f::Vector{Float64} = ones(mesh.nt))
     dm = device_mesh(mesh)
     check(dm.ctx, ccall((:fk_set_coefficient, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), dm.handle, f),
           "fk_set_coefficient")
-    check(dm.ctx, ccall((:fk_assemble, LIB), Cint, (Ptr{Cvoid}, Int32), dm.handle, Int32(0)), "fk_assemble")
+    check(dm.ctx, ccall((:fk_assemble, LIB), Cint, (Ptr{Cvoid}, Int32), dm.handle, Int32(2)), "fk_assemble")
     nzval = Vector{Float64}(undef, dm.nnz)
     check(dm.ctx, ccall((:fk_values, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), dm.handle, nzval), "fk_values")
     return tosparse(dm, nzval)
@@ -138,11 +138,11 @@ end
 
 # Assemble A(mu), fix u = g where fixed[i], Jacobi-PCG for the free nodes.
 function solveDirichlet(mesh, mu::Vector{Float64}, fixed::AbstractVector{Bool}, g::Vector{Float64};
-                        rtol = 1e-10, precision = :f64, inner_rtol = 0.5, maxiter = 1_000_000)
+                        rtol = 1e-10, precision = :f64, inner_rtol = 0.9, maxiter = 1_000_000)
     dm = device_mesh(mesh)
     check(dm.ctx, ccall((:fk_set_coefficient, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), dm.handle, mu),
           "fk_set_coefficient")
-    check(dm.ctx, ccall((:fk_assemble, LIB), Cint, (Ptr{Cvoid}, Int32), dm.handle, Int32(0)), "fk_assemble")
+    check(dm.ctx, ccall((:fk_assemble, LIB), Cint, (Ptr{Cvoid}, Int32), dm.handle, Int32(2)), "fk_assemble")
     mask = UInt8.(fixed)
     check(dm.ctx, ccall((:fk_set_dirichlet, LIB), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Ptr{Float64}), dm.handle, mask, g),
           "fk_set_dirichlet")

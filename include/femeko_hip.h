@@ -56,9 +56,12 @@ extern "C" {
 #define FK_PREC_F32_RESTART 3  /* FP32 PCG to inner_rtol restarted inside an FP64
                                   residual-correction loop (classic iterative refinement) */
 
-/* scatter method of the element assembly */
+/* method of the element assembly */
 #define FK_ASM_COLOURED 0      /* mesh-coloured conflict-free adds (deterministic) */
 #define FK_ASM_ATOMIC 1        /* hardware FP64 atomics (measured alternative) */
+#define FK_ASM_GATHER 2        /* one thread per row over the node's elements, values summed in
+                                  LDS and written once (deterministic, exactly symmetric; falls
+                                  back to FK_ASM_COLOURED for rows beyond 256 entries) */
 
 typedef struct fk_context fk_context;
 typedef struct fk_mesh fk_mesh;
@@ -99,7 +102,9 @@ int fk_pattern(fk_mesh *mesh, int32_t index_base, int32_t *rowptr, int32_t *coli
  * Non-finite values return FK_ERR_NONFINITE. */
 int fk_set_coefficient(fk_mesh *mesh, const double *f);
 /* Assemble A(f) values on the GPU (FP64; FP32 copy made when a mixed solve
- * needs it). method: FK_ASM_COLOURED or FK_ASM_ATOMIC. */
+ * needs it). method: FK_ASM_GATHER (fastest: 3.1 ms vs 13.7 ms coloured at 24.6M
+ * tets), FK_ASM_COLOURED or FK_ASM_ATOMIC; all three agree to rounding. The
+ * first FK_ASM_GATHER call on a mesh builds its incidence lists on the host. */
 int fk_assemble(fk_mesh *mesh, int32_t method);
 /* Copy the assembled FP64 values (nnz, CSR order == CSC order). */
 int fk_values(fk_mesh *mesh, double *values);
@@ -127,12 +132,14 @@ int fk_set_dirichlet(fk_mesh *mesh, const uint8_t *mask, const double *g);
 /* Jacobi-preconditioned CG (the solver of magnetostatics.cpp:26-38 and
  * run.jl:141) on the assembled matrix, stopping at ||r||_2 < rtol ||b||_2.
  * precision FK_PREC_F64: one FP64 PCG to rtol.
- * precision FK_PREC_F32_REFINED: FP32 values and PCG recursion; once the
- * FP32 recursive residual has fallen by the factor inner_rtol (0 < inner_rtol
- * < 1, e.g. 0.5) since the last update, the FP32 increments are folded into
- * the FP64 potential and the residual is replaced by the FP64 true residual
- * -mask(A64 phi), keeping the search direction ("reliable updates"); stops
- * on the FP64 residual (max_outer unused).
+ * precision FK_PREC_F32_REFINED: FP32 values and PCG recursion (difference-
+ * form SpMV on the SELL formats); at the end of each chunk of iterations in
+ * which the FP32 recursive residual has fallen by the factor inner_rtol
+ * (0 < inner_rtol < 1; other values mean 0.9) since the last update, the
+ * FP32 increments are folded into the FP64 potential and the residual is
+ * replaced by the FP64 true residual -mask(A64 phi), keeping the search
+ * direction ("reliable updates"). The decisions are taken on the device;
+ * the solve stops on the FP64 residual (max_outer unused).
  * precision FK_PREC_F32_RESTART: FP32 PCG to inner_rtol restarted inside an
  * FP64 residual/update loop (at most max_outer restarts).
  * warm_start != 0 starts from the resident potential. The potential stays

@@ -81,11 +81,12 @@ def test_pattern_bit_exact_small_and_shuffled(oracle, n, jit, shuffle):
         assert np.array_equal(r0 + 1, rowptr) and np.array_equal(c0 + 1, colidx)
         f = np.linspace(0.5, 2.0, m.nt)
         dm.set_coefficient(f)
-        dm.assemble()
-        v = dm.values()
         Af = oracle.stiffnessMatrix(m, f)
         bound = 1e-12 * abs_terms(oracle, m, f)
-        assert np.all(np.abs(v - Af.nzval) <= bound)
+        for method in ("coloured", "gather", "atomic"):
+            dm.assemble(method)
+            v = dm.values()
+            assert np.all(np.abs(v - Af.nzval) <= bound), method
     finally:
         dm.close()
 
@@ -103,6 +104,7 @@ def test_values_fp64_metric(oracle, lin20):
 
 def test_values_symmetric_deterministic_and_atomic(oracle, lin20):
     pr, A, dm = lin20
+    dm.assemble("coloured")
     v1 = dm.values()
     dm.assemble("coloured")
     v2 = dm.values()
@@ -117,6 +119,31 @@ def test_values_symmetric_deterministic_and_atomic(oracle, lin20):
     dm.assemble("coloured")
     f32 = dm.values_f32()
     assert np.array_equal(f32, v1.astype(np.float32))
+    dm.assemble()
+
+
+def test_values_row_gather_assembly(oracle, lin20):
+    # the row-gather assembly: same metric against the oracle, exactly
+    # symmetric, bitwise reproducible
+    pr, A, dm = lin20
+    try:
+        dm.assemble("gather")
+        vg = dm.values()
+        bound = abs_terms(oracle, pr.mesh, pr.mu)
+        assert np.all(np.abs(vg - A.nzval) <= 1e-12 * bound)
+        rowptr, colidx = dm.pattern(base=0)
+        S = sp.csr_matrix((vg, colidx, rowptr), shape=(dm.nv, dm.nv))
+        assert (S != S.T).nnz == 0
+        dm.assemble("gather")
+        assert np.array_equal(dm.values().view(np.uint64), vg.view(np.uint64))
+        golden = np.load(GOLDEN)
+        rep = dm.solve("f64", rtol=1e-10)
+        assert np.linalg.norm(dm.potential() - golden["phi"]) / np.linalg.norm(golden["phi"]) < 1e-9
+        # against the coloured scatter: the same terms summed in another order
+        dm.assemble("coloured")
+        assert np.all(np.abs(dm.values() - vg) <= 1e-13 * bound)
+    finally:
+        dm.assemble()
 
 
 def test_local_stiffness_volumes_lagrange(oracle, lin20):

@@ -44,10 +44,11 @@ def test_delaunay_mesh_parity(oracle):
         rowptr, colidx = dm.pattern(base=1)
         assert np.array_equal(rowptr, A.colptr) and np.array_equal(colidx, A.rowval)
         dm.set_coefficient(mu)
-        dm.assemble()
-        v = dm.values()
         scale = oracle.term_scale(m, mu)
-        assert np.all(np.abs(v - A.nzval) <= 1e-12 * scale)
+        for method in ("gather", "atomic", "coloured"):
+            dm.assemble(method)
+            v = dm.values()
+            assert np.all(np.abs(v - A.nzval) <= 1e-12 * scale), method
         g = np.where(hull == 1, np.sin(3 * m.p[0]) + m.p[1] * m.p[2], 0.0)
         dm.set_dirichlet(hull, g)
         ref, _it, _rel, s = oracle.pcg_dirichlet(A, hull, g, 1e-13)
