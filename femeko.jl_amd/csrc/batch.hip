@@ -13,9 +13,13 @@
 //   - The right-hand sides of a mesh share the matrix: vectors are stored node
 //     major with the nrhs values of a node together, one gather of the column
 //     serves all of them (SpMM), matrix traffic is paid once per nrhs solves.
-//   - FP32 inner PCG with FP64 residual refinement per system; systems that
-//     have converged are skipped, blocks whose systems are all done return at
-//     once; chunks of iterations run from a captured hipGraph.
+//   - FP32 PCG per system with FP64 reliable updates decided on the device
+//     after every chunk of iterations (per system: fold the FP32 increment,
+//     replace the residual by the FP64 true residual, keep the direction),
+//     or, as the alternative, FP32 PCG to an inner tolerance restarted inside
+//     an FP64 residual-correction loop; systems that have finished are
+//     skipped, blocks whose systems are all done return at once; chunks of
+//     iterations run from a captured hipGraph.
 //   - Lanes per row and block size come from an autotuner that times the
 //     SpMM kernel on the batch itself.
 #include <algorithm>
@@ -52,8 +56,17 @@ struct fk_batch {
     hipGraphExec_t graph = nullptr;
     int graph_cfg = -1;
     int lanes = 8, block = 256;
+    // reliable updates, per system
+    int precision = FK_PREC_F32_REFINED;
+    fk::DevBuf<int32_t> ru_act, ru_mode, ru_fin, ru_nrep, ru_restarts, ru_status, nfin;
+    fk::DevBuf<double> rr32, rr64, rr_last, thresh64, part_res;
+    hipGraphExec_t ru_graph = nullptr;
+    int ru_graph_cfg = -1;
+    double ru_delta = -1.0;     // baked into the captured chunk
+    int64_t ru_max_iter = -1;
     ~fk_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (ru_graph) (void)hipGraphExecDestroy(ru_graph);
         delete mesh;
     }
 };
@@ -84,7 +97,10 @@ __device__ __forceinline__ double seg_sum(const double *__restrict__ part, int b
     return block_sum<BLOCK>(s, sh);
 }
 
-// K1: q = A p for all right-hand sides of the block's mesh, partials of p.q
+// K1: q = A p for all right-hand sides of the block's mesh, partials of p.q.
+// Difference form q_i = sum_j a_ij (p_j - p_i): the rows of every mesh's P1
+// stiffness matrix sum to zero, and in FP32 this form does not cancel for
+// smooth p (the diagonal term is zero and costs nothing but its load).
 template <int BLOCK, int LPR>
 __global__ __launch_bounds__(BLOCK) void kb_spmm(BatchArgs ba, const int32_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ col, const float *__restrict__ val,
@@ -104,13 +120,17 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm(BatchArgs ba, const int32_t *__
         const bool valid = row < r1;
         float s[4] = {0.f, 0.f, 0.f, 0.f};
         if (valid) {
+            float pi[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (j < nrhs) pi[j] = p[row * nrhs + j];
             const int32_t a = rowptr[row], b = rowptr[row + 1];
             for (int32_t k = a + lane; k < b; k += LPR) {
                 const float v = val[k];
                 const float *pc = p + static_cast<int64_t>(col[k]) * nrhs;
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (j < nrhs) s[j] += v * pc[j];
+                    if (j < nrhs) s[j] += v * (pc[j] - pi[j]);
             }
         }
 #pragma unroll
@@ -140,7 +160,8 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
                                                       const float *__restrict__ dinv,
                                                       const double *__restrict__ part_pq,
                                                       const double *__restrict__ rz_cur, double *__restrict__ rz_next,
-                                                      double *__restrict__ part_rr, const int32_t *__restrict__ done) {
+                                                      double *__restrict__ part_rr, int32_t *__restrict__ done,
+                                                      int32_t *__restrict__ status, int32_t *__restrict__ ndone) {
     __shared__ double sh[BLOCK / 64 + 1];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
@@ -152,8 +173,16 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
     for (int j = 0; j < nrhs; ++j) {
         const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, nrhs, sh);
         const double pq = seg_sum<BLOCK>(part_pq, b0, b1, j, nrhs, sh);
-        act[j] = !done[mesh * nrhs + j] && pq > 0.0 && isfinite(pq);
+        const bool live = !done[mesh * nrhs + j];
+        act[j] = live && pq > 0.0 && isfinite(pq);
         alpha[j] = act[j] ? static_cast<float>(rho / pq) : 0.f;
+        if (live && !act[j] && lb == b0 && threadIdx.x == 0) {
+            // breakdown (every block of the mesh sees the same pq): stop the
+            // system, its caller restarts or reports it
+            status[mesh * nrhs + j] = FK_ERR_SINGULAR;
+            done[mesh * nrhs + j] = 1;
+            atomicAdd(ndone, 1);
+        }
     }
     const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
     double rz[4] = {0.0, 0.0, 0.0, 0.0}, rr[4] = {0.0, 0.0, 0.0, 0.0};
@@ -193,7 +222,7 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
                                                      const double *__restrict__ part_rr, int32_t *__restrict__ done,
                                                      int32_t *__restrict__ status, int64_t *__restrict__ iters,
                                                      const double *__restrict__ thresh, int64_t max_iter,
-                                                     int32_t *__restrict__ ndone) {
+                                                     int32_t *__restrict__ ndone, double *__restrict__ rr32) {
     __shared__ double sh[BLOCK / 64 + 1];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
@@ -213,6 +242,7 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
             if (threadIdx.x == 0 && act[j]) {
                 const int64_t it = iters[s] + 1;
                 iters[s] = it;
+                rr32[s] = rr;
                 int fin = 0;
                 if (rr < thresh[s]) {
                     fin = 1;
@@ -249,11 +279,18 @@ __global__ __launch_bounds__(BLOCK) void kb_neg_residual(BatchArgs ba, const int
                                                          const double *__restrict__ val,
                                                          const double *__restrict__ phi,
                                                          const uint8_t *__restrict__ mask, double *__restrict__ r,
-                                                         double *__restrict__ part) {
+                                                         double *__restrict__ part, const int32_t *__restrict__ gate) {
     __shared__ double sh[BLOCK / 64 + 1];
     constexpr int LPR = 16, RPB = BLOCK / LPR;
     const int lb = batch_logical_block(ba.nblocks);
     const int nrhs = ba.nrhs;
+    if (gate) {
+        // replacement step: only meshes with a system that replaces (act 1)
+        const int mesh = ba.blk_mesh[lb];
+        bool any = false;
+        for (int j = 0; j < nrhs; ++j) any |= gate[mesh * nrhs + j] == 1;
+        if (!any) return;
+    }
     const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
     const int sub = threadIdx.x / LPR, lane = threadIdx.x % LPR;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -372,6 +409,181 @@ __global__ void kb_init_phi(double *__restrict__ phi, const double *__restrict__
     phi[o] = mask[o / nrhs] ? g[o] : 0.0;
 }
 
+// ---- reliable updates per system (after every chunk of iterations) ------
+constexpr int kBatchMaxRestarts = 32;
+
+struct BatchRu {
+    int32_t *act, *mode, *fin, *nrep, *restarts, *fstatus, *nfin;
+    double *rr32, *rr64, *rr_last, *thresh64;
+    const double *scale;
+    int32_t *done, *status;
+    const int64_t *iters;
+    double delta2;
+    int64_t max_iter;
+    int nsys;
+};
+
+// one thread per system: replace when the FP32 residual fell by delta since
+// the last replacement, or the recursion stopped (target, limit, breakdown)
+__global__ void kb_ru_check(BatchRu u) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= u.nsys) return;
+    u.act[s] = 0;
+    if (u.fin[s]) return;
+    if (!u.done[s]) {
+        if (u.rr32[s] < u.delta2 * u.rr_last[s] || u.iters[s] >= u.max_iter) u.act[s] = 1;
+        return;
+    }
+    const bool broken = u.status[s] == FK_ERR_SINGULAR || u.status[s] == FK_ERR_NONFINITE || !isfinite(u.rr32[s]);
+    if (!broken) {
+        u.act[s] = 1;
+        return;
+    }
+    if (++u.restarts[s] > kBatchMaxRestarts || u.iters[s] >= u.max_iter) {
+        u.fin[s] = 1;
+        u.fstatus[s] = u.iters[s] >= u.max_iter ? FK_ERR_NOT_CONVERGED : u.status[s];
+        atomicAdd(u.nfin, 1);
+        return;
+    }
+    u.act[s] = 2;  // drop the increment, restart from the last true residual
+}
+
+// phi += scale x (act 1), x = 0 (act 1 and 2)
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kb_ru_fold(BatchArgs ba, BatchRu u, double *__restrict__ phi,
+                                                    float *__restrict__ x) {
+    const int lb = batch_logical_block(ba.nblocks);
+    const int mesh = ba.blk_mesh[lb];
+    const int nrhs = ba.nrhs;
+    int a[4] = {0, 0, 0, 0};
+    bool any = false;
+    for (int j = 0; j < nrhs; ++j) {
+        a[j] = u.act[mesh * nrhs + j];
+        any |= a[j] != 0;
+    }
+    if (!any) return;
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK)
+        for (int j = 0; j < nrhs; ++j) {
+            if (!a[j]) continue;
+            const int64_t o = i * nrhs + j;
+            if (a[j] == 1) phi[o] += u.scale[mesh * nrhs + j] * static_cast<double>(x[o]);
+            x[o] = 0.f;
+        }
+}
+
+// one thread per system: the FP64 residual decides (block sums in order)
+__global__ void kb_ru_decide(BatchArgs ba, BatchRu u, const double *__restrict__ part) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= u.nsys || u.act[s] == 0) return;
+    if (u.act[s] == 2) {
+        u.mode[s] = 2;
+        return;
+    }
+    const int mesh = s / ba.nrhs, j = s % ba.nrhs;
+    double rr = 0.0;
+    for (int b = ba.mesh_b0[mesh]; b < ba.mesh_b1[mesh]; ++b) rr += part[b * ba.nrhs + j];
+    u.rr64[s] = rr;
+    u.nrep[s] += 1;
+    int fin = 0, st = FK_OK;
+    if (rr < u.thresh64[s]) {
+        fin = 1;
+    } else if (!isfinite(rr)) {
+        fin = 1;
+        st = FK_ERR_NONFINITE;
+    } else if (u.iters[s] >= u.max_iter) {
+        fin = 1;
+        st = FK_ERR_NOT_CONVERGED;
+    } else {
+        const double sc = u.scale[s];
+        const double scaled = rr / (sc * sc);
+        int mode = 1;
+        if (scaled > 4.0 * u.rr32[s]) {  // drifted: restart the direction
+            if (++u.restarts[s] > kBatchMaxRestarts) {
+                fin = 1;
+                st = FK_ERR_NOT_CONVERGED;
+            }
+            mode = 2;
+        }
+        u.mode[s] = mode;
+        u.rr_last[s] = scaled;
+    }
+    if (fin) {
+        u.fin[s] = 1;
+        u.fstatus[s] = st;
+        u.done[s] = 1;  // the FP32 kernels skip it from now on
+        atomicAdd(u.nfin, 1);
+    }
+}
+
+// r32 = r64 / scale, p = D^-1 r on a restart, rho partials into the parity-0
+// slot, for the replaced systems only
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kb_ru_replace(BatchArgs ba, BatchRu u, const double *__restrict__ r64,
+                                                       const float *__restrict__ dinv, float *__restrict__ r,
+                                                       float *__restrict__ p, double *__restrict__ part_rz) {
+    __shared__ double sh[BLOCK / 64 + 1];
+    const int lb = batch_logical_block(ba.nblocks);
+    const int mesh = ba.blk_mesh[lb];
+    const int nrhs = ba.nrhs;
+    bool rep[4] = {false, false, false, false}, restart[4] = {false, false, false, false};
+    bool any = false;
+    for (int j = 0; j < nrhs; ++j) {
+        const int s = mesh * nrhs + j;
+        rep[j] = u.act[s] != 0 && !u.fin[s];
+        restart[j] = rep[j] && u.mode[s] == 2;
+        any |= rep[j];
+    }
+    if (!any) return;
+    double dinvs[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < nrhs; ++j) dinvs[j] = 1.0 / u.scale[mesh * nrhs + j];
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    double rz[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        const float d = dinv[i];
+        for (int j = 0; j < nrhs; ++j) {
+            if (!rep[j]) continue;
+            const int64_t o = i * nrhs + j;
+            const float ri = d != 0.f ? static_cast<float>(r64[o] * dinvs[j]) : 0.f;
+            r[o] = ri;
+            if (restart[j]) p[o] = d * ri;
+            rz[j] += static_cast<double>(ri) * static_cast<double>(d * ri);
+        }
+    }
+    for (int j = 0; j < nrhs; ++j) {
+        if (!rep[j]) continue;  // uniform over the block
+        const double t = block_sum<BLOCK>(rz[j], sh);
+        if (threadIdx.x == 0) part_rz[lb * nrhs + j] = t;
+    }
+}
+
+__global__ void kb_ru_resume(BatchRu u) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= u.nsys) return;
+    if (u.act[s] != 0 && !u.fin[s]) {
+        u.done[s] = 0;
+        u.status[s] = FK_OK;
+    }
+    u.act[s] = 0;
+}
+
+// initial state: FP32 stop threshold = the scaled FP64 target
+__global__ void kb_ru_start(BatchRu u, const double *__restrict__ target2, double *__restrict__ thresh,
+                            int64_t *__restrict__ iters) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= u.nsys) return;
+    thresh[s] = target2[s];
+    iters[s] = 0;
+    u.status[s] = FK_OK;
+    u.done[s] = u.fin[s];
+    u.act[s] = 0;
+    u.mode[s] = 1;
+    u.nrep[s] = 0;
+    u.restarts[s] = 0;
+    u.rr_last[s] = 1.0;
+    u.rr32[s] = 1.0;
+}
+
 __global__ void kb_zero_i32(int32_t *__restrict__ a, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0;
@@ -452,10 +664,10 @@ static void iteration_tail(fk_batch *b, int parity, int64_t max_iter) {
     fk_mesh *m = b->mesh;
     kb_update_xr<BLOCK><<<nb, BLOCK, 0, st>>>(args_of(b), b->x.get(), b->r.get(), b->p.get(), b->q.get(),
                                               m->w32.dinv.get(), b->part_pq.get(), rz_cur, rz_next,
-                                              b->part_rr.get(), b->done.get());
+                                              b->part_rr.get(), b->done.get(), b->status.get(), b->ndone.get());
     kb_update_p<BLOCK><<<nb, BLOCK, 0, st>>>(args_of(b), b->p.get(), b->r.get(), m->w32.dinv.get(), rz_next, rz_cur,
                                              b->part_rr.get(), b->done.get(), b->status.get(), b->iters.get(),
-                                             b->thresh.get(), max_iter, b->ndone.get());
+                                             b->thresh.get(), max_iter, b->ndone.get(), b->rr32.get());
 }
 
 static void enqueue_batch_iteration(fk_batch *b, int parity, int64_t max_iter) {
@@ -467,11 +679,11 @@ static void enqueue_batch_iteration(fk_batch *b, int parity, int64_t max_iter) {
     }
 }
 
-static void neg_residual_batch(fk_batch *b, const double *phi) {
+static void neg_residual_batch(fk_batch *b, const double *phi, double *part = nullptr, const int32_t *gate = nullptr) {
     fk_mesh *m = b->mesh;
     kb_neg_residual<256><<<b->nblocks, 256, 0, b->ctx->stream>>>(args_of(b), m->rowptr.get(), m->colidx.get(),
                                                                 m->val64.get(), phi, m->mask.get(), b->r64.get(),
-                                                                b->part_rr.get());
+                                                                part ? part : b->part_rr.get(), gate);
     FK_HIP(hipGetLastError());
 }
 
@@ -527,21 +739,139 @@ static void inner_solve(fk_batch *b, double inner_rtol, int64_t max_iter) {
     }
 }
 
-void batch_solve(fk_batch *b, double rtol, double inner_rtol, int max_outer, int64_t max_iter, int64_t *iters_out,
-                 int32_t *outer_out, double *relres_out) {
+static BatchRu ru_of(fk_batch *b, double delta, int64_t max_iter) {
+    return BatchRu{b->ru_act.get(),   b->ru_mode.get(),  b->ru_fin.get(),      b->ru_nrep.get(), b->ru_restarts.get(),
+                   b->ru_status.get(), b->nfin.get(),     b->rr32.get(),        b->rr64.get(),    b->rr_last.get(),
+                   b->thresh64.get(),  b->scale.get(),    b->done.get(),        b->status.get(),  b->iters.get(),
+                   delta * delta,      max_iter,          b->nsys};
+}
+
+// G iterations and the per-system replacement step
+static void enqueue_ru_chunk(fk_batch *b, const BatchRu &u, int G) {
+    hipStream_t st = b->ctx->stream;
     fk_mesh *m = b->mesh;
-    if (!m->have_values) fail(FK_ERR_STATE, "batch not assembled");
-    if (!b->have_dirichlet) fail(FK_ERR_STATE, "no Dirichlet data set");
+    for (int k = 0; k < G; ++k) enqueue_batch_iteration(b, k & 1, int64_t(1) << 40);
+    const unsigned gs = static_cast<unsigned>(grid_for(b->nsys, 64));
+    kb_ru_check<<<gs, 64, 0, st>>>(u);
+    kb_ru_fold<256><<<b->nblocks, 256, 0, st>>>(args_of(b), u, b->phi.get(), b->x.get());
+    neg_residual_batch(b, b->phi.get(), b->part_res.get(), b->ru_act.get());
+    kb_ru_decide<<<gs, 64, 0, st>>>(args_of(b), u, b->part_res.get());
+    kb_ru_replace<256><<<b->nblocks, 256, 0, st>>>(args_of(b), u, b->r64.get(), m->w32.dinv.get(), b->r.get(),
+                                                   b->p.get(), b->part_rz.get());
+    kb_ru_resume<<<gs, 64, 0, st>>>(u);
+    FK_HIP(hipGetLastError());
+}
+
+// FP32 PCG per system with FP64 reliable updates decided on the device
+static void batch_solve_ru(fk_batch *b, double rtol, double delta, int64_t max_iter, int64_t *iters_out,
+                           int32_t *outer_out, double *relres_out) {
+    fk_mesh *m = b->mesh;
     hipStream_t st = b->ctx->stream;
     const int64_t nv = m->nv;
     const int nr = b->nrhs, S = b->nsys;
-    make_val32(m);
-    if (!m->dinv32_valid) {
-        // dinv of the FP32 values, masked by the batch's fixed nodes
-        m->w32.dinv.ensure(nv);
-        compute_dinv_f32(m);
-        m->dinv32_valid = true;
+    b->phi.ensure(nv * nr);
+    b->r64.ensure(nv * nr);
+    kb_init_phi<<<grid_for(nv * nr, 256), 256, 0, st>>>(b->phi.get(), b->g.get(), m->mask.get(), nv, nr);
+    b->have_phi = true;
+    std::vector<double> bb, rr;
+    neg_residual_batch(b, b->g.get());  // b = -A[free,fixed] g per system
+    system_norms(b, bb);
+    neg_residual_batch(b, b->phi.get());  // r64 = residual of the start
+    system_norms(b, rr);
+    std::vector<int32_t> act(S), fin(S);
+    std::vector<double> scale(S), th(S), t2(S);
+    int nfin0 = 0;
+    for (int s = 0; s < S; ++s) {
+        th[s] = rtol * rtol * bb[s];
+        act[s] = (bb[s] > 0.0 && !(rr[s] < th[s])) ? 1 : 0;
+        fin[s] = !act[s];
+        nfin0 += fin[s];
+        scale[s] = act[s] ? std::sqrt(rr[s]) : 1.0;
+        t2[s] = act[s] ? th[s] / rr[s] : 0.0;
     }
+    DevBuf<double> target2(S);
+    std::vector<int32_t> zeros(S, 0);
+    FK_HIP(hipMemcpyAsync(b->active.get(), act.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->ru_fin.get(), fin.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->ru_status.get(), zeros.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->nfin.get(), &nfin0, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->scale.get(), scale.data(), S * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->thresh64.get(), th.data(), S * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->rr64.get(), rr.data(), S * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(target2.get(), t2.data(), S * sizeof(double), hipMemcpyHostToDevice, st));
+    const BatchRu u = ru_of(b, delta, max_iter);
+    kb_init<256><<<b->nblocks, 256, 0, st>>>(args_of(b), b->r64.get(), b->scale.get(), b->active.get(),
+                                             m->w32.dinv.get(), b->x.get(), b->r.get(), b->p.get(), b->part_rz.get(),
+                                             b->part_bb.get());
+    kb_ru_start<<<grid_for(S, 64), 64, 0, st>>>(u, target2.get(), b->thresh.get(), b->iters.get());
+    FK_HIP(hipGetLastError());
+    const int G = 16;
+    const int cfg = b->block * 1000 + b->lanes;
+    const double delta_key = delta;
+    if (!b->ru_graph || b->ru_graph_cfg != cfg || b->ru_delta != delta_key || b->ru_max_iter != max_iter) {
+        if (b->ru_graph) (void)hipGraphExecDestroy(b->ru_graph);
+        b->ru_graph = nullptr;
+        hipGraph_t graph;
+        FK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue_ru_chunk(b, u, G);
+        FK_HIP(hipStreamEndCapture(st, &graph));
+        hipError_t e = hipGraphInstantiate(&b->ru_graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        FK_HIP(e);
+        b->ru_graph_cfg = cfg;
+        b->ru_delta = delta_key;
+        b->ru_max_iter = max_iter;
+    }
+    // one chunk in flight ahead of the count being examined
+    int32_t *hc = b->hcount.ptr;
+    hipEvent_t ev[2];
+    FK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    FK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    try {
+        FK_HIP(hipGraphLaunch(b->ru_graph, st));
+        FK_HIP(hipMemcpyAsync(&hc[0], b->nfin.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipEventRecord(ev[0], st));
+        int cur = 0;
+        while (true) {
+            FK_HIP(hipGraphLaunch(b->ru_graph, st));
+            FK_HIP(hipMemcpyAsync(&hc[1 - cur], b->nfin.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            FK_HIP(hipEventRecord(ev[1 - cur], st));
+            FK_HIP(hipEventSynchronize(ev[cur]));
+            if (hc[cur] >= S) break;
+            cur = 1 - cur;
+        }
+        FK_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        throw;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    std::vector<int64_t> its(S);
+    std::vector<int32_t> nrep(S), fst(S);
+    std::vector<double> rr64(S);
+    FK_HIP(hipMemcpyAsync(its.data(), b->iters.get(), S * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(nrep.data(), b->ru_nrep.get(), S * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(fst.data(), b->ru_status.get(), S * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(rr64.data(), b->rr64.get(), S * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    int status = FK_OK;
+    for (int s = 0; s < S; ++s) {
+        if (iters_out) iters_out[s] = act[s] ? its[s] : 0;
+        if (outer_out) outer_out[s] = act[s] ? nrep[s] : 0;
+        if (relres_out) relres_out[s] = bb[s] > 0.0 ? std::sqrt(rr64[s] / bb[s]) : 0.0;
+        if (fst[s] != FK_OK && status == FK_OK) status = fst[s];
+    }
+    if (status != FK_OK) fail(status, std::string("batch solve: ") + fk_status_string(status));
+}
+
+static void batch_solve_restart(fk_batch *b, double rtol, double inner_rtol, int max_outer, int64_t max_iter,
+                                int64_t *iters_out, int32_t *outer_out, double *relres_out) {
+    fk_mesh *m = b->mesh;
+    hipStream_t st = b->ctx->stream;
+    const int64_t nv = m->nv;
+    const int nr = b->nrhs, S = b->nsys;
     b->phi.ensure(nv * nr);
     b->r64.ensure(nv * nr);
     kb_init_phi<<<grid_for(nv * nr, 256), 256, 0, st>>>(b->phi.get(), b->g.get(), m->mask.get(), nv, nr);
@@ -590,6 +920,24 @@ void batch_solve(fk_batch *b, double rtol, double inner_rtol, int max_outer, int
         else if (bb[s] > 0.0 && !(rr[s] < rtol * rtol * bb[s]) && status == FK_OK) status = FK_ERR_NOT_CONVERGED;
     }
     if (status != FK_OK) fail(status, std::string("batch solve: ") + fk_status_string(status));
+}
+
+void batch_solve(fk_batch *b, double rtol, double inner_rtol, int max_outer, int64_t max_iter, int64_t *iters_out,
+                 int32_t *outer_out, double *relres_out) {
+    fk_mesh *m = b->mesh;
+    if (!m->have_values) fail(FK_ERR_STATE, "batch not assembled");
+    if (!b->have_dirichlet) fail(FK_ERR_STATE, "no Dirichlet data set");
+    make_val32(m);
+    if (!m->dinv32_valid) {
+        // dinv of the FP32 values, masked by the batch's fixed nodes
+        m->w32.dinv.ensure(m->nv);
+        compute_dinv_f32(m);
+        m->dinv32_valid = true;
+    }
+    if (b->precision == FK_PREC_F32_REFINED)
+        batch_solve_ru(b, rtol, inner_rtol, max_iter, iters_out, outer_out, relres_out);
+    else
+        batch_solve_restart(b, rtol, inner_rtol, max_outer, max_iter, iters_out, outer_out, relres_out);
 }
 
 // time the SpMM kernel for every (block, lanes) candidate on the batch itself
@@ -693,6 +1041,12 @@ int fk_batch_create(fk_context *ctx, int32_t nmesh, const int64_t *nv, const int
         b->part_pq.alloc(np); b->part_rz.alloc(2 * np); b->part_rr.alloc(np); b->part_bb.alloc(np);
         b->done.alloc(b->nsys); b->status.alloc(b->nsys); b->active.alloc(b->nsys); b->ndone.alloc(1);
         b->iters.alloc(b->nsys); b->thresh.alloc(b->nsys); b->scale.alloc(b->nsys);
+        // per-system state of the reliable updates; rr32 is written by the
+        // iteration kernels of both solve modes
+        const size_t S = static_cast<size_t>(b->nsys);
+        b->ru_act.alloc(S); b->ru_mode.alloc(S); b->ru_fin.alloc(S); b->ru_nrep.alloc(S);
+        b->ru_restarts.alloc(S); b->ru_status.alloc(S); b->nfin.alloc(1);
+        b->rr32.alloc(S); b->rr64.alloc(S); b->rr_last.alloc(S); b->thresh64.alloc(S); b->part_res.alloc(np);
         b->hpart.alloc(np + 16);
         b->hcount.alloc(4);
     });
@@ -804,6 +1158,12 @@ int fk_batch_autotune(fk_batch *b, int32_t *lanes, int32_t *block, double *best_
         fk::set_device(b->ctx);
         fk::batch_autotune(b, lanes, block, best_us);
     });
+}
+
+int fk_batch_set_precision(fk_batch *b, int32_t precision) {
+    if (!b || (precision != FK_PREC_F32_REFINED && precision != FK_PREC_F32_RESTART)) return FK_ERR_INVALID_ARG;
+    b->precision = precision;
+    return FK_OK;
 }
 
 int fk_batch_set_launch(fk_batch *b, int32_t lanes, int32_t block) {

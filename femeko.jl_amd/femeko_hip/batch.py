@@ -70,8 +70,18 @@ class Batch:
         G = (_lib._dp * self.nmesh)(*[dptr(a) for a in gg])
         self.ctx.check(self._L.fk_batch_set_dirichlet(self.handle, M, G), "fk_batch_set_dirichlet")
 
-    def solve(self, rtol: float = 1e-10, inner_rtol: float = 1e-5, max_outer: int = 50,
+    def set_precision(self, precision: str) -> None:
+        """'f32' (FP32 PCG with FP64 reliable updates, default) or 'f32-restart'."""
+        code = {"f32": _lib.FK_PREC_F32_REFINED, "f32-restart": _lib.FK_PREC_F32_RESTART}[precision]
+        self.ctx.check(self._L.fk_batch_set_precision(self.handle, code), "fk_batch_set_precision")
+        self.precision = precision
+
+    def solve(self, rtol: float = 1e-10, inner_rtol: float | None = None, max_outer: int = 50,
               max_iter: int = 1_000_000) -> BatchReport:
+        """inner_rtol: replacement trigger (default 0.9) with 'f32', inner
+        tolerance (default 1e-5) with 'f32-restart'."""
+        if inner_rtol is None:
+            inner_rtol = 1e-5 if getattr(self, "precision", "f32") == "f32-restart" else 0.9
         S = self.nmesh * self.nrhs
         it = np.zeros(S, np.int64)
         outer = np.zeros(S, np.int32)

@@ -235,9 +235,17 @@ int fk_batch_set_coefficient(fk_batch *batch, const double *const *f);
 int fk_batch_assemble(fk_batch *batch);
 /* mask[m]: nv[m]; g[m]: nrhs * nv[m], right-hand side major (g[m][j*nv[m] + i]) */
 int fk_batch_set_dirichlet(fk_batch *batch, const uint8_t *const *mask, const double *const *g);
-/* per system s = m*nrhs + j: iterations[s], outer[s], relres[s] (FP64 true residual) */
+/* Solve every system to ||r|| < rtol ||b|| (FP64 true residual). With
+ * FK_PREC_F32_REFINED (default): FP32 PCG per system with FP64 reliable
+ * updates decided on the device, inner_rtol = the residual drop that triggers
+ * a replacement (e.g. 0.9), max_outer unused. With FK_PREC_F32_RESTART: FP32
+ * PCG to inner_rtol (e.g. 1e-5) restarted inside an FP64 residual-correction
+ * loop of at most max_outer steps. Per system s = m*nrhs + j: iterations[s],
+ * outer[s] (replacements or restarts), relres[s]. */
 int fk_batch_solve(fk_batch *batch, double rtol, double inner_rtol, int32_t max_outer, int64_t max_iter,
                    int64_t *iterations, int32_t *outer, double *relres);
+/* FK_PREC_F32_REFINED or FK_PREC_F32_RESTART for the following solves. */
+int fk_batch_set_precision(fk_batch *batch, int32_t precision);
 /* phi: nrhs * nv[mesh], right-hand side major */
 int fk_batch_potential(fk_batch *batch, int32_t mesh, double *phi);
 /* Autotuner: times the batched SpMM for lanes per row {4, 8, 16} x block
