@@ -110,7 +110,9 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_gather(
     const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
     const double *__restrict__ coef, int64_t nv, double *__restrict__ val) {
     extern __shared__ double acc[];
-    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kGatherRows;
+    // XCD-aware: each XCD takes a contiguous eighth of the rows, so the
+    // elements shared by rows one grid line or plane apart stay in its L2
+    const int64_t i0 = static_cast<int64_t>(logical_block(static_cast<int>(gridDim.x))) * kGatherRows;
     const int64_t i = i0 + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t s = i >> 6;  // uniform over the wave
