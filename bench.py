@@ -17,7 +17,7 @@ events around every SpMV launch of an instrumented solve), and the CPU oracle
 timed on a bounded sample of the same workload.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160] [--no-cpu]
-                [--shape 1024,1] [--delta 0.9] [--format sell64|sell64-rowidx|csr] [--workload boxes|batch|froehlich|adjoint]
+                [--shape 1024,1] [--delta 0.9] [--format auto|stencil|sell64|sell64-rowidx|csr] [--workload boxes|batch|froehlich|adjoint]
 """
 import argparse
 import json
@@ -87,21 +87,23 @@ def cpu_baseline(pr, iters_f64, nthreads, budget_s=20.0):
     }
 
 
-def _pmc_traffic(args):
+def _pmc_traffic(args, solver_format):
     """HBM bytes per SELL SpMV launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by what the same counters read
     for the streaming kernel k_rr_fold of known traffic), when one exists for
     this workload and launch shape."""
-    path = os.path.join(ROOT, "profiles", "pmc_spmv_sell_f32_boxes160.txt")
-    if args.n != 160 or args.format != "sell64" or not os.path.exists(path):
+    path = os.path.join(ROOT, "profiles", "pmc_spmv_f32_boxes160.txt")
+    if args.n != 160 or not os.path.exists(path):
         return None
     vals = {}
     with open(path) as fh:
         for line in fh:
-            if ":" in line and line.split(":")[0].strip() in ("traffic_bytes_per_launch", "shape"):
+            if ":" in line and line.split(":")[0].strip() in ("traffic_bytes_per_launch", "format", "shape"):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
-    if vals.get("shape") != args.shape or "traffic_bytes_per_launch" not in vals:
+    if vals.get("format") != solver_format or "traffic_bytes_per_launch" not in vals:
+        return None
+    if solver_format == "sell64" and vals.get("shape") != args.shape:
         return None
     return float(vals["traffic_bytes_per_launch"])
 
@@ -248,7 +250,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=160)
     ap.add_argument("--lanes", type=int, default=16)
-    ap.add_argument("--format", default="sell64", choices=["csr", "sell64", "sell64-rowidx"])
+    ap.add_argument("--format", default="auto", choices=["auto", "stencil", "csr", "sell64", "sell64-rowidx"])
     ap.add_argument("--delta", type=float, default=0.9,
                     help="FP32 solve: residual drop that triggers an FP64 residual replacement")
     ap.add_argument("--shape", default="1024,1",
@@ -282,7 +284,9 @@ def main():
     shape = tuple(int(v) for v in args.shape.split(","))
     dm.set_sell_shape(*shape)
     log(f"device mesh: nnz={dm.nnz} colours={dm.ncolours} ({time.perf_counter()-t0:.1f} s)")
-    sell = dm.sell_info() if args.format != "csr" else None
+    fmt_name, stencil_offsets = dm.solver_format()
+    sell = dm.sell_info() if fmt_name == "sell64" else None
+    log(f"solver format: {fmt_name} {stencil_offsets if stencil_offsets else ''}")
     if sell:
         log(f"SELL copy: {sell[0]} slices, {sell[1]} with shared offsets, {sell[2]} stored entries")
     dm.set_coefficient(pr.mu)
@@ -373,7 +377,8 @@ def main():
                          f"replaced by the FP64 true residual after every {args.delta:g}x drop, to FP64 "
                          f"relres 1e-10"),
             "nodes": m.nv, "tets": m.nt, "nnz": dm.nnz, "free_dofs": free, "colours": dm.ncolours,
-            "lanes_per_row": args.lanes, "spmv_format": args.format, "sell_shape": list(shape),
+            "lanes_per_row": args.lanes, "spmv_format": fmt_name, "stencil_offsets": list(stencil_offsets),
+            "sell_shape": list(shape),
             "sell_slices": sell[0] if sell else None, "sell_shared_slices": sell[1] if sell else None,
             "sell_entries": sell[2] if sell else None,
             "update_trigger": args.delta,
@@ -388,8 +393,9 @@ def main():
         "error_vs_fp64_rtol1e-13": {"phi_rel_l2": err_phi, "Hn_rel_l2_VE": err_H, "energy_rel": err_E},
         "roofline": {
             "bound": "hbm",
-            "kernel": ("k_cg_spmv<float,%d>" % args.lanes) if args.format == "csr" else
-                      "k_cg_spmv_sell<float,int16,%d,%d>" % shape,
+            "kernel": {"csr": "k_cg_spmv<float,%d>" % args.lanes,
+                       "sell64": "k_cg_spmv_sell<float,int16,%d,%d>" % shape,
+                       "stencil": "k_cg_spmv_stencil<float,512>"}[fmt_name],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -412,7 +418,7 @@ def main():
     log(f"f32-restart step {t_rs*1e3:.1f} ms, iterations {rep_rs.iterations}, outer {rep_rs.outer}")
     out["f32_restart_inner1e-5"] = {"ms_per_step": t_rs * 1e3, "iterations": rep_rs.iterations,
                                     "outer": rep_rs.outer, "relres": rep_rs.relres}
-    traffic = _pmc_traffic(args)
+    traffic = _pmc_traffic(args, fmt_name)
     if traffic is not None:
         # HBM bytes per launch from the committed counter profile, and the
         # bandwidth they imply at this run's launch time

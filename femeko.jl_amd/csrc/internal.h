@@ -204,7 +204,7 @@ struct fk_mesh {
     fk::HostBuf<fk::RuState> hru;     // pinned copies (2, polled alternately)
 
     // SELL-64 copy of the matrix for the solver (CSR stays the storage format)
-    int format = 1;                   // FK_SPMV_CSR, FK_SPMV_SELL64 (default), FK_SPMV_SELL64_ROWIDX
+    int format = FK_SPMV_AUTO;        // requested solver format (FK_SPMV_*)
     bool have_sell = false, sell64_valid = false, sell32_valid = false;
     int64_t nslices = 0, sell_len = 0, sell_chunk = 0;
     int64_t nshared = 0;              // slices in the shared-offset form
@@ -212,6 +212,17 @@ struct fk_mesh {
     fk::DevBuf<int32_t> sell_width, sell_col, sell_map, sell_soff;
     fk::DevBuf<int16_t> sell_dcol;    // int16 column offsets from the row (when sell_short)
     fk::DevBuf<double> sell_diag64;   // diagonal of the FP64 standard-form SpMV
+    // stencil copy (symmetric diagonal storage) for patterns whose column
+    // offsets are one set of at most 16 +-offsets: column k holds
+    // a(i, i + off[k]) for off[k] > 0; a(i, i - off[k]) is read from row i - off[k]
+    int kind = -1;                    // resolved solver format (FK_SPMV_*), -1 not resolved
+    bool stencil_built = false, stencil_ok = false;
+    int32_t st_K = 0;
+    int32_t st_off[16] = {};
+    int64_t st_ld = 0;
+    fk::DevBuf<int32_t> st_map;       // K x ld: CSR position or -1
+    fk::DevBuf<double> st_val64;
+    fk::DevBuf<float> st_val32;
     bool sell_short = false, allow_short = true;
     int32_t bandwidth = 0;
     fk::DevBuf<double> sell_val64;

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <thread>
+#include <type_traits>
 
 #include "device.h"
 #include "internal.h"
@@ -462,6 +463,143 @@ __global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const Sell<double,
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
+// ---- stencil copy: symmetric diagonal storage -------------------------------
+// Column k (k < K, at most 16) of val holds a(i, i + off[k]) for the positive
+// offsets off[k] (0 where row i has no such column, or past the last row);
+// a(i, i - off[k]) = a(i - off[k], i) is the same column read at row
+// i - off[k]. One thread per row; the shifted reads trail the streamed ones
+// by off[k] rows of the same band and are served by L2.
+template <typename T>
+struct Stencil {
+    const T *val;   // K x ld
+    const T *diag;  // FP64 standard form only
+    int64_t ld;
+    int64_t n;
+    int32_t K;
+    int32_t off[16];
+};
+
+// K offsets known at compile time. FP32: all 4K loads are issued
+// unconditionally (indices clamped into range; a clamped forward entry holds
+// value 0, a clamped backward term is dropped), then the terms are summed.
+// FP64 keeps guarded loads over the runtime count (measured faster: the
+// unrolled register arrays cost it occupancy).
+template <bool DIFF, int K, typename T>
+__device__ __forceinline__ T stencil_row(const Stencil<T> &A, const T *__restrict__ x, int64_t i, T xi) {
+    if (sizeof(T) == 8) {
+        T a0 = T(0), a1 = T(0);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k >= A.K) break;
+            const int64_t o = A.off[k];
+            const T *__restrict__ v = A.val + k * A.ld;
+            if (i + o < A.n) a0 += term<DIFF>(v[i], x[i + o], xi);
+            if (i >= o) a1 += term<DIFF>(v[i - o], x[i - o], xi);
+        }
+        return a0 + a1;
+    }
+    T vf[K], vb[K], xf[K], xb[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t o = A.off[k];
+        const T *__restrict__ v = A.val + k * A.ld;
+        const int64_t f = min(i + o, A.n - 1);
+        const int64_t b = max(i - o, int64_t(0));
+        vf[k] = v[i];
+        vb[k] = v[b];
+        xf[k] = x[f];
+        xb[k] = x[b];
+    }
+    T a0 = T(0), a1 = T(0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        a0 += term<DIFF>(vf[k], xf[k], xi);
+        if (i >= A.off[k]) a1 += term<DIFF>(vb[k], xb[k], xi);
+    }
+    return a0 + a1;
+}
+
+template <int K, typename T>
+__device__ __forceinline__ T stencil_apply(const Stencil<T> &A, const T *__restrict__ x, int64_t i, T xi) {
+    constexpr bool DIFF = sizeof(T) == 4;
+    const T off = stencil_row<DIFF, K>(A, x, i, xi);
+    return DIFF ? off : off + A.diag[i] * xi;
+}
+
+// Rows of the stencil kernels: each XCD (blocks b = x mod 8) owns one
+// contiguous eighth of the rows and its blocks sweep it together, block b
+// taking every (nblocks/8)-th run of BLOCK rows. The XCD's active window is
+// then nblocks/8 * BLOCK rows wide and moves through the band, so rows one
+// grid plane back (the largest offsets) were read a window or two earlier
+// and are still in its L2.
+template <int BLOCK>
+struct StencilRows {
+    int64_t start, end, step;
+    __device__ StencilRows(int64_t n, int nblocks) {
+        if ((nblocks & 7) != 0) {  // too few blocks: one contiguous run each
+            const int64_t c = (n + nblocks - 1) / nblocks;
+            start = static_cast<int64_t>(blockIdx.x) * c;
+            end = min(n, start + c);
+            step = BLOCK;
+            return;
+        }
+        const int x = blockIdx.x & 7, b = blockIdx.x >> 3, nbx = nblocks >> 3;
+        const int64_t band = ((n + 7) / 8 + BLOCK - 1) / BLOCK * BLOCK;
+        start = x * band + static_cast<int64_t>(b) * BLOCK;
+        end = min(n, (x + 1) * band);
+        step = static_cast<int64_t>(nbx) * BLOCK;
+    }
+};
+
+// K1 in stencil form: q = A p, partial p.q
+template <typename T, int BLOCK, int K>
+__global__ __launch_bounds__(BLOCK) void k_cg_spmv_stencil(const Stencil<T> A, const T *__restrict__ p,
+                                                           T *__restrict__ q, double *__restrict__ part,
+                                                           int64_t chunk, int nblocks, CgState *__restrict__ st,
+                                                           int parity) {
+    if (stopped_entry(st, parity)) return;
+    __shared__ double sh[BLOCK / 64 + 1];
+    (void)chunk;
+    const StencilRows<BLOCK> R(A.n, nblocks);
+    double acc = 0.0;
+    for (int64_t row0 = R.start; row0 < R.end; row0 += R.step) {
+        const int64_t i = row0 + threadIdx.x;
+        if (i >= R.end) break;
+        const T xi = p[i];
+        const T y = stencil_apply<K>(A, p, i, xi);
+        q[i] = y;
+        acc += static_cast<double>(xi) * static_cast<double>(y);
+    }
+    const double tot = block_sum<BLOCK>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// FP64 r = -mask(A phi), stencil form (gate as in k_neg_residual_sell)
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_neg_residual_stencil(const Stencil<double> A,
+                                                                 const double *__restrict__ phi,
+                                                                 const uint8_t *__restrict__ mask,
+                                                                 double *__restrict__ r, double *__restrict__ part,
+                                                                 int64_t chunk, int nblocks,
+                                                                 const RuState *__restrict__ gate) {
+    if (gate && gate->act != 1) return;
+    __shared__ double sh[kBlock / 64 + 1];
+    (void)chunk;
+    const StencilRows<kBlock> R(A.n, nblocks);
+    double acc = 0.0;
+    for (int64_t row0 = R.start; row0 < R.end; row0 += R.step) {
+        const int64_t i = row0 + threadIdx.x;
+        if (i >= R.end) break;
+        const double y = stencil_apply<K>(A, phi, i, phi[i]);
+        const double v = mask[i] ? 0.0 : -y;
+        r[i] = v;
+        acc += v * v;
+    }
+    const double tot = block_sum<kBlock>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+
 // x = 0, p = z = D^-1 r, partials of r.z and r.r (initial residual r given)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_init(T *__restrict__ x, T *__restrict__ r, T *__restrict__ p,
@@ -698,7 +836,7 @@ static void build_sell(fk_mesh *m) {
     FK_HIP(hipMemcpyAsync(ci.data(), m->colidx.get(), nnz * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
     const int64_t ns = (nv + 63) / 64;
-    const bool allow_shared = m->format == FK_SPMV_SELL64;
+    const bool allow_shared = m->format != FK_SPMV_SELL64_ROWIDX;
     // pass 1: widths, shared offset sets, bandwidth
     std::vector<int32_t> width(static_cast<size_t>(ns));
     std::vector<std::vector<int32_t>> shared(static_cast<size_t>(ns));
@@ -829,12 +967,121 @@ static void build_sell(fk_mesh *m) {
     m->have_sell = true;
 }
 
+// Stencil copy (layout at k_cg_spmv_stencil), host build once per mesh:
+// stencil_ok when the positive column offsets of all rows form one set of at
+// most 16 and the columns hold at most 25% padding.
+static void build_stencil(fk_mesh *m) {
+    if (m->stencil_built) return;
+    m->stencil_built = true;
+    m->stencil_ok = false;
+    m->w64.reset_graph();
+    m->w32.reset_graph();
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = m->nv, nnz = m->nnz;
+    std::vector<int32_t> rp(static_cast<size_t>(nv + 1)), ci(static_cast<size_t>(nnz));
+    FK_HIP(hipMemcpyAsync(rp.data(), m->rowptr.get(), (nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(ci.data(), m->colidx.get(), nnz * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    // union of the positive offsets, per thread then merged (gives up past 16)
+    const int nth = static_cast<int>(std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<std::vector<int32_t>> local(static_cast<size_t>(nth));
+    {
+        std::vector<std::thread> th;
+        for (int q = 0; q < nth; ++q)
+            th.emplace_back([&, q] {
+                std::vector<int32_t> &P = local[q];
+                for (int64_t i = nv * q / nth; i < nv * (q + 1) / nth; ++i)
+                    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                        const int32_t d = ci[k] - static_cast<int32_t>(i);
+                        if (d <= 0) continue;
+                        auto it = std::lower_bound(P.begin(), P.end(), d);
+                        if (it == P.end() || *it != d) {
+                            if (P.size() >= 16) {
+                                P.assign(17, 0);  // marks "too many"
+                                return;
+                            }
+                            P.insert(it, d);
+                        }
+                    }
+            });
+        for (auto &x : th) x.join();
+    }
+    std::vector<int32_t> P;
+    for (auto &l : local) {
+        if (l.size() > 16) return;
+        for (int32_t d : l) {
+            auto it = std::lower_bound(P.begin(), P.end(), d);
+            if (it == P.end() || *it != d) P.insert(it, d);
+        }
+        if (P.size() > 16) return;
+    }
+    const int32_t K = static_cast<int32_t>(P.size());
+    const int64_t ld = (nv + 63) / 64 * 64;
+    const int64_t pairs = (nnz - nv) / 2;  // off-diagonal pairs
+    if (K == 0 || 4 * K * nv > 5 * pairs + 256 * K) return;
+    std::vector<int32_t> map(static_cast<size_t>(K * ld), -1);
+    parallel_for(nv, [&](int64_t i) {
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            const int32_t d = ci[k] - static_cast<int32_t>(i);
+            if (d <= 0) continue;
+            const int64_t c = std::lower_bound(P.begin(), P.end(), d) - P.begin();
+            map[c * ld + i] = k;
+        }
+    });
+    m->st_map.alloc(K * ld);
+    FK_HIP(hipMemcpyAsync(m->st_map.get(), map.data(), K * ld * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipStreamSynchronize(st));
+    m->st_K = K;
+    for (int k = 0; k < 16; ++k) m->st_off[k] = k < K ? P[k] : 0;
+    m->st_ld = ld;
+    m->stencil_ok = true;
+}
+
+// the solver format actually used (FK_SPMV_CSR, FK_SPMV_SELL64, FK_SPMV_STENCIL)
+static int solver_kind(fk_mesh *m) {
+    if (m->kind >= 0) return m->kind;
+    switch (m->format) {
+        case FK_SPMV_CSR: m->kind = FK_SPMV_CSR; break;
+        case FK_SPMV_STENCIL:
+            build_stencil(m);
+            if (!m->stencil_ok)
+                fail(FK_ERR_UNSUPPORTED, "the matrix pattern is not a stencil of at most 16 offsets");
+            m->kind = FK_SPMV_STENCIL;
+            break;
+        case FK_SPMV_AUTO:
+            build_stencil(m);
+            m->kind = m->stencil_ok ? FK_SPMV_STENCIL : FK_SPMV_SELL64;
+            break;
+        default: m->kind = FK_SPMV_SELL64; break;
+    }
+    return m->kind;
+}
+
+template <typename T>
+static Stencil<T> stencil_view(fk_mesh *m, const T *val, const T *diag) {
+    Stencil<T> A{val, diag, m->st_ld, m->nv, m->st_K, {}};
+    for (int k = 0; k < 16; ++k) A.off[k] = m->st_off[k];
+    return A;
+}
+
 // bring the solver's copy of the values up to date with val64 / val32
 template <typename T>
 static void sync_solver_values(fk_mesh *m);
 template <>
 void sync_solver_values<double>(fk_mesh *m) {
     if (m->format == FK_SPMV_CSR || m->sell64_valid) return;
+    if (solver_kind(m) == FK_SPMV_STENCIL) {
+        const int64_t len = m->st_K * m->st_ld;
+        m->st_val64.ensure(len);
+        m->sell_diag64.ensure(m->nv);
+        k_sell_gather<double><<<grid_for(len, 256), 256, 0, m->ctx->stream>>>(m->val64.get(), m->st_map.get(), len,
+                                                                            m->st_val64.get());
+        k_sell_gather<double><<<grid_for(m->nv, 256), 256, 0, m->ctx->stream>>>(m->val64.get(), m->diag.get(),
+                                                                                m->nv, m->sell_diag64.get());
+        FK_HIP(hipGetLastError());
+        m->sell64_valid = true;
+        return;
+    }
     build_sell(m);
     m->sell_val64.ensure(m->sell_len);
     m->sell_diag64.ensure(m->nv);
@@ -849,8 +1096,17 @@ void sync_solver_values<double>(fk_mesh *m) {
 template <>
 void sync_solver_values<float>(fk_mesh *m) {
     if (m->format == FK_SPMV_CSR || m->sell32_valid) return;
-    build_sell(m);
     make_val32(m);
+    if (solver_kind(m) == FK_SPMV_STENCIL) {
+        const int64_t len = m->st_K * m->st_ld;
+        m->st_val32.ensure(len);
+        k_sell_gather<float><<<grid_for(len, 256), 256, 0, m->ctx->stream>>>(m->val32.get(), m->st_map.get(), len,
+                                                                           m->st_val32.get());
+        FK_HIP(hipGetLastError());
+        m->sell32_valid = true;
+        return;
+    }
+    build_sell(m);
     m->sell_val32.ensure(m->sell_len);
     if (m->sell_len > 0)
         k_sell_gather<float><<<grid_for(m->sell_len, 256), 256, 0, m->ctx->stream>>>(
@@ -906,7 +1162,50 @@ static void launch_sell_shape(fk_mesh *m, PcgWork<T> &w, const IDX *col, int par
 }
 
 template <typename T>
+static const T *stencil_values(fk_mesh *m);
+template <>
+const double *stencil_values<double>(fk_mesh *m) { return m->st_val64.get(); }
+template <>
+const float *stencil_values<float>(fk_mesh *m) { return m->st_val32.get(); }
+
+template <typename T, int BLOCK, int K>
+static void launch_stencil_k(fk_mesh *m, PcgWork<T> &w, int parity) {
+    k_cg_spmv_stencil<T, BLOCK, K><<<m->nblocks, BLOCK, 0, m->ctx->stream>>>(
+        stencil_view<T>(m, stencil_values<T>(m), sell_diag<T>(m)), w.p.get(), w.q.get(), w.part_pq.get(), m->chunk,
+        m->nblocks, w.state.get(), parity);
+}
+
+// dispatch on the number of positive offsets (1..16, a compile-time constant
+// of the kernel)
+template <int K = 1>
+struct StencilK {
+    template <typename F>
+    static void run(int k, F &&f) {
+        if (k == K)
+            f(std::integral_constant<int, K>{});
+        else
+            StencilK<K + 1>::run(k, static_cast<F &&>(f));
+    }
+};
+template <>
+struct StencilK<17> {
+    template <typename F>
+    static void run(int, F &&) {
+        fail(FK_ERR_UNSUPPORTED, "stencil with more than 16 offsets");
+    }
+};
+
+template <typename T, int BLOCK>
+static void launch_stencil(fk_mesh *m, PcgWork<T> &w, int parity) {
+    StencilK<>::run(m->st_K, [&](auto kc) { launch_stencil_k<T, BLOCK, decltype(kc)::value>(m, w, parity); });
+}
+
+template <typename T>
 static void launch_spmv_lanes(fk_mesh *m, PcgWork<T> &w, int parity) {
+    if (m->kind == FK_SPMV_STENCIL) {
+        launch_stencil<T, 512>(m, w, parity);  // measured best of 256/512/1024 at n=160
+        return;
+    }
     if (m->format != FK_SPMV_CSR) {
         if (m->sell_short)
             launch_sell_shape<T, int16_t>(m, w, m->sell_dcol.get(), parity);
@@ -1337,7 +1636,13 @@ static void launch_neg_residual(fk_mesh *m, const double *phi, double *r, double
 // (no host sync); gate as in the kernels
 static void enqueue_neg_residual(fk_mesh *m, const double *phi, double *r, const RuState *gate) {
     hipStream_t st = m->ctx->stream;
-    if (m->format != FK_SPMV_CSR) {
+    if (m->kind == FK_SPMV_STENCIL) {
+        StencilK<>::run(m->st_K, [&](auto kc) {
+            k_neg_residual_stencil<decltype(kc)::value><<<m->nblocks, kBlock, 0, st>>>(
+                stencil_view<double>(m, m->st_val64.get(), m->sell_diag64.get()), phi, m->mask.get(), r,
+                m->part_misc.get(), m->chunk, m->nblocks, gate);
+        });
+    } else if (m->format != FK_SPMV_CSR) {
         if (m->sell_short)
             k_neg_residual_sell<int16_t><<<m->nblocks, kBlock, 0, st>>>(
                 sell_view<double, int16_t>(m, m->sell_dcol.get()), phi, m->mask.get(), r, m->part_misc.get(),
@@ -1654,22 +1959,34 @@ int fk_copy_probe(fk_context *ctx, int64_t bytes, int32_t reps, double *ms_per_c
 }
 
 int fk_set_spmv_format(fk_mesh *m, int32_t format) {
-    if (!m || (format != FK_SPMV_CSR && format != FK_SPMV_SELL64 && format != FK_SPMV_SELL64_ROWIDX))
-        return FK_ERR_INVALID_ARG;
+    if (!m || format < FK_SPMV_CSR || format > FK_SPMV_AUTO) return FK_ERR_INVALID_ARG;
     if (format != m->format) {
-        // the two SELL forms differ in layout: rebuild on next use
+        // the SELL forms differ in layout (rebuilt on next use); the
+        // resolved format is decided again
         m->have_sell = false;
         m->sell64_valid = false;
         m->sell32_valid = false;
+        m->kind = -1;
     }
     m->format = format;
     return FK_OK;
 }
 
+int fk_solver_format(fk_mesh *m, int32_t *format, int32_t *stencil_offsets, int32_t *offsets) {
+    if (!m || !format) return FK_ERR_INVALID_ARG;
+    return fk::guard(m->ctx, "fk_solver_format", [&] {
+        fk::set_device(m->ctx);
+        *format = fk::solver_kind(m);
+        if (stencil_offsets) *stencil_offsets = *format == FK_SPMV_STENCIL ? m->st_K : 0;
+        if (offsets && *format == FK_SPMV_STENCIL)
+            for (int k = 0; k < m->st_K; ++k) offsets[k] = m->st_off[k];
+    });
+}
+
 int fk_sell_info(fk_mesh *m, int64_t *nslices, int64_t *shared_slices, int64_t *entries) {
     if (!m) return FK_ERR_INVALID_ARG;
     return fk::guard(m->ctx, "fk_sell_info", [&] {
-        if (m->format == FK_SPMV_CSR) fk::fail(FK_ERR_STATE, "the solver format is CSR");
+        if (fk::solver_kind(m) != FK_SPMV_SELL64) fk::fail(FK_ERR_STATE, "the solver format is not SELL-64");
         fk::build_sell(m);
         if (nslices) *nslices = m->nslices;
         if (shared_slices) *shared_slices = m->nshared;

@@ -101,6 +101,7 @@ SIGNATURES = {
     "fk_set_spmv_format": (_c.c_int, [_vp, _c.c_int32]),
     "fk_set_sell_shape": (_c.c_int, [_vp, _c.c_int32, _c.c_int32]),
     "fk_sell_info": (_c.c_int, [_vp, _c.c_void_p, _c.c_void_p, _c.c_void_p]),
+    "fk_solver_format": (_c.c_int, [_vp, _c.c_void_p, _c.c_void_p, _c.c_void_p]),
 }
 
 _lib = None

@@ -300,10 +300,22 @@ class DeviceMesh:
         return float(res.value)
 
     def set_format(self, fmt: str) -> None:
-        """Solver SpMV format: 'csr', 'sell64' (shared slice offsets where the
-        rows allow) or 'sell64-rowidx' (per-row indices only)."""
-        code = {"csr": 0, "sell64": 1, "sell": 1, "sell64-rowidx": 2}[fmt]
+        """Solver SpMV format: 'auto' (default: 'stencil' when the pattern is a
+        stencil of at most 16 offsets, else 'sell64'), 'stencil' (symmetric
+        diagonal storage), 'sell64' (shared slice offsets where the rows allow),
+        'sell64-rowidx' (per-row indices only) or 'csr'."""
+        code = {"csr": 0, "sell64": 1, "sell": 1, "sell64-rowidx": 2, "stencil": 3, "auto": 4}[fmt]
         self.ctx.check(self._L.fk_set_spmv_format(self.handle, code), "fk_set_spmv_format")
+
+    def solver_format(self):
+        """(format name, stencil offsets or ()) the solver resolved for this mesh."""
+        f = ctypes.c_int32(0)
+        k = ctypes.c_int32(0)
+        offs = (ctypes.c_int32 * 16)()
+        self.ctx.check(self._L.fk_solver_format(self.handle, ctypes.byref(f), ctypes.byref(k), offs),
+                       "fk_solver_format")
+        name = {0: "csr", 1: "sell64", 3: "stencil"}[int(f.value)]
+        return name, tuple(int(offs[i]) for i in range(int(k.value)))
 
     def sell_info(self):
         """(slices, shared-offset slices, stored entries) of the SELL copy."""

@@ -267,20 +267,34 @@ int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms);
 int fk_copy_probe(fk_context *ctx, int64_t bytes, int32_t reps, double *ms_per_copy);
 /* SpMV tuning: lanes per row (4, 8, 16, 32, 64) of the CSR kernels. */
 int fk_set_lanes_per_row(fk_mesh *mesh, int32_t lanes);
-/* Matrix format of the solver's SpMV: FK_SPMV_CSR (the assembled CSR) or a
- * sliced-ELL copy of the off-diagonal entries (64 rows per slice = one
- * wavefront, refreshed from the CSR values after each assembly, diagonal
- * kept apart). FK_SPMV_SELL64 (default) stores the column offsets of a slice
- * once when its rows share one offset set (structured mesh regions) and per
- * row elsewhere; FK_SPMV_SELL64_ROWIDX always stores them per row. The SELL
- * kernels compute the FP32 product in the difference form
- * sum_j a_ij (x_j - x_i) (rows of a P1 stiffness matrix sum to zero), the
- * FP64 product in the standard form. Results agree to rounding; the exported
- * matrix is always CSR. */
+/* Matrix format of the solver's SpMV (copies are refreshed from the CSR
+ * values after each assembly; the exported matrix is always CSR):
+ *   FK_SPMV_CSR: the assembled CSR.
+ *   FK_SPMV_SELL64: a sliced-ELL copy of the off-diagonal entries (64 rows
+ *     per slice = one wavefront, diagonal kept apart) that stores the column
+ *     offsets of a slice once when its rows share one offset set (structured
+ *     mesh regions) and per row elsewhere; FK_SPMV_SELL64_ROWIDX always per
+ *     row.
+ *   FK_SPMV_STENCIL: for patterns whose column offsets j - i form one set of
+ *     at most 16 +-offsets (node numberings of structured meshes): one value
+ *     column per positive offset, a(i, i + o); a(i, i - o) = a(i - o, i) is
+ *     read from the same column (the matrix is exactly symmetric), so each
+ *     off-diagonal pair is stored once. FK_ERR_UNSUPPORTED on other patterns.
+ *   FK_SPMV_AUTO (default): FK_SPMV_STENCIL when the pattern allows it (at
+ *     most 25% padding), else FK_SPMV_SELL64.
+ * The SELL and stencil kernels compute the FP32 product in the difference
+ * form sum_j a_ij (x_j - x_i) (rows of a P1 stiffness matrix sum to zero),
+ * the FP64 product in the standard form. Results agree to rounding. */
 #define FK_SPMV_CSR 0
 #define FK_SPMV_SELL64 1
 #define FK_SPMV_SELL64_ROWIDX 2
+#define FK_SPMV_STENCIL 3      /* symmetric diagonal storage, see below */
+#define FK_SPMV_AUTO 4         /* default: FK_SPMV_STENCIL when the pattern allows, else FK_SPMV_SELL64 */
 int fk_set_spmv_format(fk_mesh *mesh, int32_t format);
+/* The format the solver uses for this mesh (FK_SPMV_CSR, FK_SPMV_SELL64 or
+ * FK_SPMV_STENCIL, resolved on demand); for the stencil its number of
+ * positive offsets and the offsets (offsets: room for 16; NULL allowed). */
+int fk_solver_format(fk_mesh *mesh, int32_t *format, int32_t *stencil_offsets, int32_t *offsets);
 /* SELL copy statistics (built on demand): slices, slices in the shared-offset
  * form, stored entries (values including padding). Any pointer may be NULL. */
 int fk_sell_info(fk_mesh *mesh, int64_t *nslices, int64_t *shared_slices, int64_t *entries);

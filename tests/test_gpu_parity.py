@@ -205,11 +205,11 @@ def test_solve_f32_refined_linear20(lin20, delta):
     assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) < 1e-9
 
 
-@pytest.mark.parametrize("fmt", ["csr", "sell64", "sell64-rowidx"])
+@pytest.mark.parametrize("fmt", ["csr", "sell64", "sell64-rowidx", "stencil", "auto"])
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32-restart"])
 def test_spmv_formats_solve_agree(lin20, prec, fmt):
-    # the solver's CSR kernels and its SELL-64 copies (shared slice offsets or
-    # per-row indices) give the same solution
+    # the solver's CSR kernels, its SELL-64 copies (shared slice offsets or
+    # per-row indices) and its stencil copy give the same solution
     pr, A, dm = lin20
     golden = np.load(GOLDEN)
     try:
@@ -222,7 +222,7 @@ def test_spmv_formats_solve_agree(lin20, prec, fmt):
         if prec == "f64":
             assert abs(rep.iterations - it_ref) <= max(3, 0.05 * it_ref)
     finally:
-        dm.set_format("sell64")
+        dm.set_format("auto")
 
 
 @pytest.mark.parametrize("shape", [(256, 1), (256, 2), (512, 1), (512, 2), (1024, 1)])
@@ -265,6 +265,7 @@ def test_sell_layout_statistics(lin20):
     # off-diagonal rows
     pr, A, dm = lin20
     try:
+        dm.set_format("sell64")
         ns, nsh, ent = dm.sell_info()
         assert ns == (dm.nv + 63) // 64 and nsh == ns
         dm.set_format("sell64-rowidx")
@@ -286,7 +287,43 @@ def test_sell_layout_statistics(lin20):
             assert rep.relres < 1e-10
             assert np.linalg.norm(dm.potential() - golden["phi"]) / np.linalg.norm(golden["phi"]) < 1e-9
     finally:
-        dm.set_format("sell64")
+        dm.set_format("auto")
+
+
+def test_stencil_format_on_kuhn_meshes(lin20):
+    # Kuhn meshes in their natural numbering: 7 positive offsets
+    # 1, n, n+1, n^2, n^2+1, n^2+n, n^2+n+1 of (n+1)-point grid lines
+    pr, A, dm = lin20
+    name, offs = dm.solver_format()
+    m = 21
+    assert name == "stencil"
+    assert set(offs) <= {1, m, m + 1, m * m, m * m + 1, m * m + m, m * m + m + 1}
+    assert len(offs) == 7
+    # every positive column offset of the CSR pattern is one of them
+    rowptr, colidx = dm.pattern(base=0)
+    rows = np.repeat(np.arange(dm.nv), np.diff(rowptr))
+    d = colidx - rows
+    assert set(np.unique(d[d > 0]).tolist()) == set(offs)
+
+
+def test_stencil_rejected_on_renumbered_mesh(oracle):
+    # a random node numbering is no stencil: 'stencil' fails, 'auto' uses SELL
+    m = kuhn_box_mesh(4, 0.1, seed=9)
+    rng = np.random.default_rng(9)
+    pn = rng.permutation(m.nv)
+    inv = np.empty_like(pn)
+    inv[pn] = np.arange(m.nv)
+    p = np.asfortranarray(m.p[:, pn])
+    t = np.asfortranarray((inv[m.t - 1] + 1).astype(np.int32))
+    mm = MESH(p=p, t=t, VE=element_volumes(p, t))
+    dm = DeviceMesh(mm, context(0))
+    try:
+        assert dm.solver_format()[0] == "sell64"
+        dm.set_format("stencil")
+        with pytest.raises(FemekoHIPError):
+            dm.solver_format()
+    finally:
+        dm.close()
 
 
 def test_true_residual_of_gpu_solution(lin20):
