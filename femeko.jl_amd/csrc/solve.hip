@@ -33,9 +33,10 @@ namespace fk {
 // ---------------------------------------------------------------------------
 // kernels
 
+// D^-1 of the FP64 diagonal, stored in T (0 on fixed rows)
 template <typename T>
-__global__ void k_dinv(const T *__restrict__ val, const int32_t *__restrict__ diag, const uint8_t *__restrict__ mask,
-                       int64_t nv, T *__restrict__ dinv, int32_t *__restrict__ bad) {
+__global__ void k_dinv(const double *__restrict__ val, const int32_t *__restrict__ diag,
+                       const uint8_t *__restrict__ mask, int64_t nv, T *__restrict__ dinv, int32_t *__restrict__ bad) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     if (mask[i]) {
@@ -791,8 +792,8 @@ static void compute_dinv(fk_mesh *m) {
     PcgWork<T> &w = work<T>(m);
     DevBuf<int32_t> bad(1);
     FK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), st));
-    k_dinv<T><<<grid_for(m->nv, 256), 256, 0, st>>>(values<T>(m), m->diag.get(), m->mask.get(), m->nv, w.dinv.get(),
-                                                    bad.get());
+    k_dinv<T><<<grid_for(m->nv, 256), 256, 0, st>>>(m->val64.get(), m->diag.get(), m->mask.get(), m->nv,
+                                                    w.dinv.get(), bad.get());
     FK_HIP(hipGetLastError());
     int32_t h = 0;
     FK_HIP(hipMemcpyAsync(&h, bad.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -967,6 +968,25 @@ static void build_sell(fk_mesh *m) {
     m->have_sell = true;
 }
 
+// Stencil values from the FP64 CSR values, one thread per row: the row's
+// entries lie in one or two cache lines, read once for all K columns (and the
+// diagonal for the FP64 standard form).
+template <typename T>
+__global__ void k_stencil_fill(const double *__restrict__ val64, const int32_t *__restrict__ map,
+                               const int32_t *__restrict__ diagpos, int K, int64_t ld, int64_t n, T *__restrict__ out,
+                               double *__restrict__ diag64) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ld) return;
+    for (int k = 0; k < K; ++k) {
+        const int32_t pos = map[k * ld + i];
+        out[k * ld + i] = pos >= 0 ? static_cast<T>(val64[pos]) : T(0);
+    }
+    if (diag64 && i < n) {
+        const int32_t d = diagpos[i];
+        diag64[i] = d >= 0 ? val64[d] : 0.0;
+    }
+}
+
 // Stencil copy (layout at k_cg_spmv_stencil), host build once per mesh:
 // stencil_ok when the positive column offsets of all rows form one set of at
 // most 16 and the columns hold at most 25% padding.
@@ -1074,10 +1094,9 @@ void sync_solver_values<double>(fk_mesh *m) {
         const int64_t len = m->st_K * m->st_ld;
         m->st_val64.ensure(len);
         m->sell_diag64.ensure(m->nv);
-        k_sell_gather<double><<<grid_for(len, 256), 256, 0, m->ctx->stream>>>(m->val64.get(), m->st_map.get(), len,
-                                                                            m->st_val64.get());
-        k_sell_gather<double><<<grid_for(m->nv, 256), 256, 0, m->ctx->stream>>>(m->val64.get(), m->diag.get(),
-                                                                                m->nv, m->sell_diag64.get());
+        k_stencil_fill<double><<<grid_for(m->st_ld, 256), 256, 0, m->ctx->stream>>>(
+            m->val64.get(), m->st_map.get(), m->diag.get(), m->st_K, m->st_ld, m->nv, m->st_val64.get(),
+            m->sell_diag64.get());
         FK_HIP(hipGetLastError());
         m->sell64_valid = true;
         return;
@@ -1096,16 +1115,16 @@ void sync_solver_values<double>(fk_mesh *m) {
 template <>
 void sync_solver_values<float>(fk_mesh *m) {
     if (m->format == FK_SPMV_CSR || m->sell32_valid) return;
-    make_val32(m);
     if (solver_kind(m) == FK_SPMV_STENCIL) {
         const int64_t len = m->st_K * m->st_ld;
         m->st_val32.ensure(len);
-        k_sell_gather<float><<<grid_for(len, 256), 256, 0, m->ctx->stream>>>(m->val32.get(), m->st_map.get(), len,
-                                                                           m->st_val32.get());
+        k_stencil_fill<float><<<grid_for(m->st_ld, 256), 256, 0, m->ctx->stream>>>(
+            m->val64.get(), m->st_map.get(), m->diag.get(), m->st_K, m->st_ld, m->nv, m->st_val32.get(), nullptr);
         FK_HIP(hipGetLastError());
         m->sell32_valid = true;
         return;
     }
+    make_val32(m);
     build_sell(m);
     m->sell_val32.ensure(m->sell_len);
     if (m->sell_len > 0)
@@ -1731,7 +1750,7 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
     const bool mixed = precision == FK_PREC_F32_REFINED || precision == FK_PREC_F32_RESTART;
     if (mixed) {
         ensure_work<float>(m);
-        make_val32(m);
+        if (m->format == FK_SPMV_CSR) make_val32(m);  // the CSR kernels read the FP32 CSR copy
     }
     if (!m->dinv64_valid) {
         compute_dinv<double>(m);
