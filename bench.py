@@ -228,10 +228,10 @@ def bench_adjoint(args):
         fd = (ADJ.fieldObjective(m, rp, tg, Hs, pr.mask, pr.g, rtol=1e-13)
               - ADJ.fieldObjective(m, rm, tg, Hs, pr.mask, pr.g, rtol=1e-13)) / (2 * h)
         err_scaled.append(abs(fd - grad[e]) / gmax)
-        if abs(grad[e]) >= 1e-3 * gmax:
+        if abs(grad[e]) >= 1e-2 * gmax:  # below, the difference's rounding noise dominates
             err_rel_large.append(abs(fd - grad[e]) / abs(grad[e]))
     log(f"adjoint: {t*1e3:.1f} ms per state+adjoint+gradient; FD max |fd-g|/max|g| {max(err_scaled):.2e}, "
-        f"max rel err where |g| >= 1e-3 max|g| {max(err_rel_large):.2e} ({len(err_rel_large)} elements)")
+        f"max rel err where |g| >= 1e-2 max|g| {max(err_rel_large):.2e} ({len(err_rel_large)} elements)")
     res = _common(args, "state+adjoint+gradient DOF/s (FP64)", m.nv / t, "DOF/s", t * 1e3, "f64",
                   {"workload": "adjoint80: Kuhn n=80, mu_r = 1 + 999 rho^3, target [-0.1,0.1]^3, H* = (2,0,0)",
                    "nodes": m.nv, "tets": m.nt})
@@ -255,6 +255,9 @@ def main():
                     help="FP32 solve: residual drop that triggers an FP64 residual replacement")
     ap.add_argument("--shape", default="1024,1",
                     help="SELL SpMV launch shape: threads per block, slices in flight per wave")
+    ap.add_argument("--renumber", default="none", choices=["none", "rcm", "random"],
+                    help="renumber the nodes (reverse Cuthill-McKee or random) to measure a general, "
+                         "non-stencil numbering of the same mesh")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="boxes", choices=["boxes", "batch", "froehlich", "adjoint"],
                     help="boxes: the headline n=160 mixed-precision line (default); the others measure the "
@@ -274,9 +277,25 @@ def main():
 
     t0 = time.perf_counter()
     pr = W.boxes(n=args.n) if args.n != 160 else W.boxes()
+    ctx = context(0)
+    if args.renumber != "none":
+        # a general numbering of the same mesh: the pattern is no stencil and
+        # the solver uses its SELL copy
+        if args.renumber == "random":
+            perm = np.random.default_rng(args.n).permutation(pr.mesh.nv)
+        else:
+            import scipy.sparse as sp
+            from scipy.sparse.csgraph import reverse_cuthill_mckee
+            tmp = DeviceMesh(pr.mesh, ctx)
+            rp, ci = tmp.pattern(base=0)
+            tmp.close()
+            G = sp.csr_matrix((np.ones(ci.size, np.int8), ci, rp), shape=(pr.mesh.nv, pr.mesh.nv))
+            perm = reverse_cuthill_mckee(G, symmetric_mode=True)
+            del G, rp, ci
+        pr = W.renumbered(pr, perm, args.renumber)
+        log(f"renumbered nodes ({args.renumber})")
     m = pr.mesh
     log(f"mesh n={args.n}: nt={m.nt} nv={m.nv} ({time.perf_counter()-t0:.1f} s)")
-    ctx = context(0)
     t0 = time.perf_counter()
     dm = DeviceMesh(m, ctx)
     dm.set_lanes(args.lanes)
@@ -372,7 +391,8 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"boxes{args.n}: Kuhn box mesh n={args.n}, jitter 0.15h, 8 log-uniform mu_r boxes in "
+            "workload": (f"boxes{args.n}{'' if args.renumber == 'none' else '-' + args.renumber}: "
+                         f"Kuhn box mesh n={args.n}, jitter 0.15h, 8 log-uniform mu_r boxes in "
                          f"[1,1e3], Dirichlet phi=-x; step = value assembly + FP32 Jacobi-PCG whose residual is "
                          f"replaced by the FP64 true residual after every {args.delta:g}x drop, to FP64 "
                          f"relres 1e-10"),

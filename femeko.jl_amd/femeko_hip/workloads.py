@@ -61,6 +61,24 @@ def _finish(name, mesh, mu, H0, magnet, meta):
     return Problem(name, mesh, np.ascontiguousarray(mu), mask, g, np.asarray(H0, float), magnet, meta)
 
 
+def renumbered(pr: Problem, perm: np.ndarray, tag: str) -> Problem:
+    """The same problem with node k of the new numbering = node perm[k] of
+    the old one (elements keep their order): a general, non-structured
+    numbering of the same mesh."""
+    perm = np.asarray(perm, dtype=np.int64)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    m = pr.mesh
+    p = np.asfortranarray(m.p[:, perm])
+    t = np.asfortranarray((inv[np.asarray(m.t, dtype=np.int64) - 1] + 1).astype(np.int32))
+    mesh = MESH(p=p, t=t, VE=m.VE)
+    set_inside(mesh, pr.magnet)
+    meta = dict(pr.meta)
+    meta["renumbered"] = tag
+    return Problem(f"{pr.name}-{tag}", mesh, pr.mu, np.ascontiguousarray(pr.mask[perm]),
+                   np.ascontiguousarray(pr.g[perm]), pr.H0, pr.magnet, meta)
+
+
 def linear(n: int = 20, jitter: float = 0.2, seed: int = 20, mu_r: float = 1000.0,
            box=0.25, H0=(1.0, 0.0, 0.0)) -> Problem:
     mesh = kuhn_box_mesh(n, jitter, seed)

@@ -386,3 +386,18 @@ def test_kuhn_counts():
         c = kuhn_counts(n)
         assert m.nt == c["nt"] and m.nv == c["nv"]
         assert int(boundary_node_mask(m).sum()) == c["dirichlet"]
+
+
+def test_renumbered_problem_is_the_same_problem(oracle):
+    # a node renumbering permutes the solution and nothing else
+    from femeko_hip import workloads as W
+    pr = W.linear(6)
+    perm = np.random.default_rng(3).permutation(pr.mesh.nv)
+    pr2 = W.renumbered(pr, perm, "random")
+    A = oracle.stiffnessMatrix(pr.mesh, pr.mu)
+    A2 = oracle.stiffnessMatrix(pr2.mesh, pr2.mu)
+    phi, _, _, s = oracle.pcg_dirichlet(A, pr.mask, pr.g, 1e-13)
+    phi2, _, _, s2 = oracle.pcg_dirichlet(A2, pr2.mask, pr2.g, 1e-13)
+    assert s == 0 and s2 == 0
+    assert np.abs(phi2 - phi[perm]).max() < 1e-10 * np.abs(phi).max()
+    assert np.array_equal(pr2.mesh.VE, pr.mesh.VE)
