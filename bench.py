@@ -126,13 +126,15 @@ def bench_batch(args):
     probs = W.batch()
     log(f"batch: {len(probs)} meshes, {sum(p.mesh.nv for p in probs)} nodes ({time.perf_counter()-t0:.1f} s)")
     b = Batch([p.mesh for p in probs], nrhs=3)
+    b.set_format(args.format if args.format in ("auto", "stencil") else "csr")
     b.set_coefficient([p.mu for p in probs])
     b.set_dirichlet([p.mask for p in probs],
                     [np.stack([np.where(p.mask == 1, dirichlet_values(p.mesh, e), 0.0) for e in np.eye(3)])
                      for p in probs])
     b.assemble()
     lanes, block, us = b.autotune()
-    log(f"autotuner: lanes {lanes} block {block} ({us:.1f} us per SpMM)")
+    fmt = b.solver_format()
+    log(f"autotuner: SpMM {fmt} lanes {lanes} block {block} ({us:.1f} us per SpMM)")
 
     def step():
         b.assemble()
@@ -151,7 +153,7 @@ def bench_batch(args):
                   "DOF/s", t * 1e3, "f32+f64",
                   {"workload": "batch64: 64 Kuhn meshes n~U{10..60}, one mu box each, H0 = x, y, z",
                    "meshes": len(probs), "systems": 3 * len(probs), "nodes": nodes, "nnz": b.total_nnz,
-                   "autotuned_lanes": lanes, "autotuned_block": block})
+                   "solver_format": fmt, "autotuned_lanes": lanes, "autotuned_block": block})
     out["per_solve_ms"] = t * 1e3 / (3 * len(probs))
     out["iterations_max"] = int(rep.iterations.max())
     out["iterations_mean"] = float(rep.iterations.mean())

@@ -64,6 +64,16 @@ struct fk_batch {
     int ru_graph_cfg = -1;
     double ru_delta = -1.0;     // baked into the captured chunk
     int64_t ru_max_iter = -1;
+    // per-mesh stencil copy (every mesh's pattern a stencil of <= 16 offsets)
+    int stencil = -1;           // -1 not decided, 0 no (CSR SpMM), 1 yes
+    int format = FK_SPMV_AUTO;  // FK_SPMV_CSR, FK_SPMV_STENCIL or FK_SPMV_AUTO
+    fk::DevBuf<int64_t> node_off_dev;
+    bool st_valid = false;      // FP32 stencil values match the assembly
+    int64_t st_len = 0;
+    int st_Kc = 0;              // common offset count of all meshes, 0 if they differ
+    fk::DevBuf<int32_t> st_off, st_K, st_map;
+    fk::DevBuf<int64_t> st_vbase;
+    fk::DevBuf<float> st_val32;
     ~fk_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (ru_graph) (void)hipGraphExecDestroy(ru_graph);
@@ -151,6 +161,82 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm(BatchArgs ba, const int32_t *__
         const double t = block_sum<BLOCK>(acc[j], sh);
         if (threadIdx.x == 0) part[lb * nrhs + j] = t;
     }
+}
+
+// K1 from the per-mesh stencil copy: mesh m has K[m] positive offsets
+// off[16 m + k] and value columns of its n_m rows from vbase[m]; column k
+// holds a(i, i + o) (zero where i + o is past the mesh) and a(i, i - o) is
+// read at row i - o (the matrix is exactly symmetric). Difference form; NR
+// right-hand sides per row. KC > 0: every mesh has KC offsets (unrolled);
+// KC = 0: the mesh's own count. Loads are unconditional: out-of-mesh
+// neighbours are clamped to a row of the mesh and weighted zero.
+template <int BLOCK, int NR, int KC>
+__global__ __launch_bounds__(BLOCK) void kb_spmm_stencil(BatchArgs ba, const int64_t *__restrict__ node_off,
+                                                         const int32_t *__restrict__ Ks,
+                                                         const int32_t *__restrict__ offs,
+                                                         const int64_t *__restrict__ vbase,
+                                                         const float *__restrict__ val, const float *__restrict__ p,
+                                                         float *__restrict__ q, double *__restrict__ part,
+                                                         const int32_t *__restrict__ done) {
+    __shared__ double sh[BLOCK / 64 + 1];
+    const int lb = batch_logical_block(ba.nblocks);
+    const int mesh = ba.blk_mesh[lb];
+    if (mesh_all_done(done, mesh, NR)) return;
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    const int64_t a = node_off[mesh], e = node_off[mesh + 1], n = e - a;
+    const int K = KC > 0 ? KC : Ks[mesh];
+    const float *__restrict__ v0 = val + vbase[mesh];
+    int32_t off[KC > 0 ? KC : 1];
+#pragma unroll
+    for (int k = 0; k < (KC > 0 ? KC : 1); ++k) off[k] = offs[16 * mesh + k];
+    double acc[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) acc[j] = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        const int32_t li = static_cast<int32_t>(i - a);
+        float xi[NR], s[NR];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            xi[j] = p[i * NR + j];
+            s[j] = 0.f;
+        }
+        auto term = [&](int k, int32_t o) {
+            const float *__restrict__ v = v0 + static_cast<int64_t>(k) * n;
+            const float wu = v[li];
+            const int32_t lu = min(li + o, static_cast<int32_t>(n - 1));
+            const bool lo_ok = li >= o;
+            const float wl = lo_ok ? v[lo_ok ? li - o : 0] : 0.f;
+            const int32_t ll = lo_ok ? li - o : 0;
+            const float *pu = p + (a + lu) * NR;
+            const float *pl = p + (a + ll) * NR;
+#pragma unroll
+            for (int j = 0; j < NR; ++j) s[j] += wu * (pu[j] - xi[j]) + wl * (pl[j] - xi[j]);
+        };
+        if constexpr (KC > 0) {
+#pragma unroll
+            for (int k = 0; k < KC; ++k) term(k, off[k]);
+        } else {
+            for (int k = 0; k < K; ++k) term(k, offs[16 * mesh + k]);
+        }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            q[i * NR + j] = s[j];
+            acc[j] += static_cast<double>(xi[j]) * static_cast<double>(s[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const double t = block_sum<BLOCK>(acc[j], sh);
+        if (threadIdx.x == 0) part[lb * NR + j] = t;
+    }
+}
+
+__global__ void kb_stencil_fill(const double *__restrict__ val64, const int32_t *__restrict__ map, int64_t len,
+                                float *__restrict__ out) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= len) return;
+    const int32_t k = map[t];
+    out[t] = k >= 0 ? static_cast<float>(val64[k]) : 0.f;
 }
 
 // K2: per system alpha, x += alpha p, r -= alpha q, partials of r.z, r.r
@@ -647,7 +733,43 @@ static void launch_spmm_lanes(fk_batch *b, int lanes) {
     }
 }
 
+template <int BLOCK, int NR, int KC>
+static void launch_spmm_stencil_k(fk_batch *b) {
+    kb_spmm_stencil<BLOCK, NR, KC><<<b->nblocks, BLOCK, 0, b->ctx->stream>>>(
+        args_of(b), b->node_off_dev.get(), b->st_K.get(), b->st_off.get(), b->st_vbase.get(), b->st_val32.get(),
+        b->p.get(), b->q.get(), b->part_pq.get(), b->done.get());
+}
+
+// 7 offsets: the Kuhn pattern of structured boxes; other counts run the
+// mesh's own count
+template <int BLOCK, int NR>
+static void launch_spmm_stencil_nr(fk_batch *b) {
+    if (b->st_Kc == 7)
+        launch_spmm_stencil_k<BLOCK, NR, 7>(b);
+    else
+        launch_spmm_stencil_k<BLOCK, NR, 0>(b);
+}
+
+template <int BLOCK>
+static void launch_spmm_stencil(fk_batch *b) {
+    switch (b->nrhs) {
+        case 1: launch_spmm_stencil_nr<BLOCK, 1>(b); break;
+        case 2: launch_spmm_stencil_nr<BLOCK, 2>(b); break;
+        case 3: launch_spmm_stencil_nr<BLOCK, 3>(b); break;
+        default: launch_spmm_stencil_nr<BLOCK, 4>(b); break;
+    }
+    FK_HIP(hipGetLastError());
+}
+
 static void spmm(fk_batch *b, int block, int lanes) {
+    if (b->stencil == 1) {
+        switch (block) {
+            case 128: launch_spmm_stencil<128>(b); break;
+            case 512: launch_spmm_stencil<512>(b); break;
+            default: launch_spmm_stencil<256>(b); break;
+        }
+        return;
+    }
     switch (block) {
         case 128: launch_spmm_lanes<128>(b, lanes); break;
         case 512: launch_spmm_lanes<512>(b, lanes); break;
@@ -922,12 +1044,107 @@ static void batch_solve_restart(fk_batch *b, double rtol, double inner_rtol, int
     if (status != FK_OK) fail(status, std::string("batch solve: ") + fk_status_string(status));
 }
 
+// Per-mesh stencil copy, decided once per batch: every mesh's positive column
+// offsets form one set of at most 16 with at most 25% padding.
+static void build_batch_stencil(fk_batch *b) {
+    if (b->stencil >= 0) return;
+    b->stencil = 0;
+    if (b->format == FK_SPMV_CSR) return;
+    const bool must = b->format == FK_SPMV_STENCIL;
+    auto reject = [&](const char *why) {
+        if (must) fail(FK_ERR_INVALID_ARG, std::string("batch stencil format: ") + why);
+    };
+    fk_mesh *m = b->mesh;
+    hipStream_t st = b->ctx->stream;
+    const int64_t nv = m->nv, nnz = m->nnz;
+    std::vector<int32_t> rp(static_cast<size_t>(nv + 1)), ci(static_cast<size_t>(nnz));
+    FK_HIP(hipMemcpyAsync(rp.data(), m->rowptr.get(), (nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(ci.data(), m->colidx.get(), nnz * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    const int M = b->nmesh;
+    std::vector<int32_t> K(M), offs(16 * static_cast<size_t>(M), 0);
+    std::vector<int64_t> vbase(M + 1, 0);
+    for (int q = 0; q < M; ++q) {
+        const int64_t a = b->node_off[q], e = b->node_off[q + 1];
+        std::vector<int32_t> P;
+        for (int64_t i = a; i < e; ++i)
+            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const int32_t d = ci[k] - static_cast<int32_t>(i);
+                if (d <= 0) continue;
+                auto it = std::lower_bound(P.begin(), P.end(), d);
+                if (it == P.end() || *it != d) {
+                    if (P.size() >= 16) return reject("more than 16 column offsets");
+                    P.insert(it, d);
+                }
+            }
+        const int64_t n = e - a;
+        const int64_t pairs = (rp[e] - rp[a] - n) / 2;
+        if (P.empty() || 4 * static_cast<int64_t>(P.size()) * n > 5 * pairs + 256 * static_cast<int64_t>(P.size()))
+            return reject("more than 25% padding");
+        K[q] = static_cast<int32_t>(P.size());
+        for (size_t k = 0; k < P.size(); ++k) offs[16 * q + k] = P[k];
+        vbase[q + 1] = vbase[q] + K[q] * n;
+    }
+    const int64_t len = vbase[M];
+    std::vector<int32_t> map(static_cast<size_t>(len), -1);
+    for (int q = 0; q < M; ++q) {
+        const int64_t a = b->node_off[q], e = b->node_off[q + 1], n = e - a;
+        const int32_t *P = offs.data() + 16 * q;
+        for (int64_t i = a; i < e; ++i)
+            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const int32_t d = ci[k] - static_cast<int32_t>(i);
+                if (d <= 0) continue;
+                const int64_t c = std::lower_bound(P, P + K[q], d) - P;
+                map[vbase[q] + c * n + (i - a)] = k;
+            }
+    }
+    b->st_K.alloc(M);
+    b->st_off.alloc(16 * static_cast<size_t>(M));
+    b->st_vbase.alloc(M);
+    b->st_map.alloc(len);
+    b->st_val32.alloc(len);
+    b->node_off_dev.alloc(M + 1);
+    FK_HIP(hipMemcpyAsync(b->st_K.get(), K.data(), M * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->st_off.get(), offs.data(), offs.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->st_vbase.get(), vbase.data(), M * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->st_map.get(), map.data(), len * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(b->node_off_dev.get(), b->node_off.data(), (M + 1) * sizeof(int64_t),
+                          hipMemcpyHostToDevice, st));
+    FK_HIP(hipStreamSynchronize(st));
+    b->st_len = len;
+    b->st_Kc = K[0];
+    for (int q = 1; q < M; ++q)
+        if (K[q] != K[0]) b->st_Kc = 0;
+    b->st_valid = false;
+    b->stencil = 1;
+    // captured chunks launched the CSR SpMM
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    b->graph_cfg = -1;
+    if (b->ru_graph) (void)hipGraphExecDestroy(b->ru_graph);
+    b->ru_graph = nullptr;
+    b->ru_graph_cfg = -1;
+}
+
+// the SpMM copy for the current values
+static void prepare_spmm(fk_batch *b) {
+    fk_mesh *m = b->mesh;
+    build_batch_stencil(b);
+    if (b->stencil == 0) make_val32(m);
+    if (b->stencil == 1 && !b->st_valid) {
+        kb_stencil_fill<<<grid_for(b->st_len, 256), 256, 0, b->ctx->stream>>>(m->val64.get(), b->st_map.get(),
+                                                                             b->st_len, b->st_val32.get());
+        FK_HIP(hipGetLastError());
+        b->st_valid = true;
+    }
+}
+
 void batch_solve(fk_batch *b, double rtol, double inner_rtol, int max_outer, int64_t max_iter, int64_t *iters_out,
                  int32_t *outer_out, double *relres_out) {
     fk_mesh *m = b->mesh;
     if (!m->have_values) fail(FK_ERR_STATE, "batch not assembled");
     if (!b->have_dirichlet) fail(FK_ERR_STATE, "no Dirichlet data set");
-    make_val32(m);
+    prepare_spmm(b);
     if (!m->dinv32_valid) {
         // dinv of the FP32 values, masked by the batch's fixed nodes
         m->w32.dinv.ensure(m->nv);
@@ -944,7 +1161,7 @@ void batch_solve(fk_batch *b, double rtol, double inner_rtol, int max_outer, int
 void batch_autotune(fk_batch *b, int32_t *lanes_out, int32_t *block_out, double *best_us_out) {
     fk_mesh *m = b->mesh;
     if (!m->have_values) fail(FK_ERR_STATE, "batch not assembled");
-    make_val32(m);
+    prepare_spmm(b);
     hipStream_t st = b->ctx->stream;
     const int64_t nvr = m->nv * b->nrhs;
     // all systems active, p = 1 so the kernel does full work
@@ -1093,6 +1310,7 @@ int fk_batch_assemble(fk_batch *b) {
     return guard(b->ctx, "fk_batch_assemble", [&] {
         fk::set_device(b->ctx);
         fk::assemble(b->mesh, FK_ASM_GATHER);
+        b->st_valid = false;
     });
 }
 
@@ -1163,6 +1381,23 @@ int fk_batch_autotune(fk_batch *b, int32_t *lanes, int32_t *block, double *best_
 int fk_batch_set_precision(fk_batch *b, int32_t precision) {
     if (!b || (precision != FK_PREC_F32_REFINED && precision != FK_PREC_F32_RESTART)) return FK_ERR_INVALID_ARG;
     b->precision = precision;
+    return FK_OK;
+}
+
+int fk_batch_set_format(fk_batch *b, int32_t format) {
+    if (!b || (format != FK_SPMV_CSR && format != FK_SPMV_STENCIL && format != FK_SPMV_AUTO))
+        return FK_ERR_INVALID_ARG;
+    if (b->stencil >= 0 && format != b->format) {
+        b->ctx->err = "fk_batch_set_format: the SpMM copy is already chosen";
+        return FK_ERR_STATE;
+    }
+    b->format = format;
+    return FK_OK;
+}
+
+int fk_batch_solver_format(fk_batch *b, int32_t *format) {
+    if (!b || !format) return FK_ERR_INVALID_ARG;
+    *format = b->stencil < 0 ? -1 : (b->stencil == 1 ? FK_SPMV_STENCIL : FK_SPMV_CSR);
     return FK_OK;
 }
 

@@ -93,6 +93,8 @@ SIGNATURES = {
     "fk_batch_potential": (_c.c_int, [_vp, _c.c_int32, _dp]),
     "fk_batch_autotune": (_c.c_int, [_vp, _ip, _ip, _dp]),
     "fk_batch_set_precision": (_c.c_int, [_vp, _c.c_int32]),
+    "fk_batch_set_format": (_c.c_int, [_vp, _c.c_int32]),
+    "fk_batch_solver_format": (_c.c_int, [_vp, _c.POINTER(_c.c_int32)]),
     "fk_batch_set_launch": (_c.c_int, [_vp, _c.c_int32, _c.c_int32]),
     "fk_set_timing": (_c.c_int, [_vp, _c.c_int32]),
     "fk_spmv_timing": (_c.c_int, [_vp, _i64p, _dp]),

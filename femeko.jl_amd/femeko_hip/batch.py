@@ -76,6 +76,19 @@ class Batch:
         self.ctx.check(self._L.fk_batch_set_precision(self.handle, code), "fk_batch_set_precision")
         self.precision = precision
 
+    def set_format(self, fmt: str) -> None:
+        """SpMM copy: 'auto' (default: 'stencil' when every mesh's pattern is a
+        stencil of at most 16 offsets, else 'csr'), 'stencil' or 'csr'. Set it
+        before the first solve or autotune."""
+        code = {"csr": 0, "stencil": 3, "auto": 4}[fmt]
+        self.ctx.check(self._L.fk_batch_set_format(self.handle, code), "fk_batch_set_format")
+
+    def solver_format(self) -> str | None:
+        """'stencil' or 'csr' once a solve or autotune chose the copy, else None."""
+        f = ctypes.c_int32(0)
+        self.ctx.check(self._L.fk_batch_solver_format(self.handle, ctypes.byref(f)), "fk_batch_solver_format")
+        return {-1: None, 0: "csr", 3: "stencil"}[int(f.value)]
+
     def solve(self, rtol: float = 1e-10, inner_rtol: float | None = None, max_outer: int = 50,
               max_iter: int = 1_000_000) -> BatchReport:
         """inner_rtol: replacement trigger (default 0.9) with 'f32', inner

@@ -246,6 +246,16 @@ int fk_batch_solve(fk_batch *batch, double rtol, double inner_rtol, int32_t max_
                    int64_t *iterations, int32_t *outer, double *relres);
 /* FK_PREC_F32_REFINED or FK_PREC_F32_RESTART for the following solves. */
 int fk_batch_set_precision(fk_batch *batch, int32_t precision);
+/* SpMM copy of the batched solves: FK_SPMV_CSR, FK_SPMV_STENCIL (every mesh's
+ * pattern must be a stencil of at most 16 positive offsets with at most 25%
+ * padding, else the next solve fails with FK_ERR_INVALID_ARG) or FK_SPMV_AUTO
+ * (default: stencil when every mesh qualifies, else CSR). Must be set before
+ * the first solve or autotune of the batch. No counterpart in the reference
+ * (its solves run one mesh at a time). */
+int fk_batch_set_format(fk_batch *batch, int32_t format);
+/* The SpMM copy in use: FK_SPMV_CSR or FK_SPMV_STENCIL, or -1 before the
+ * first solve or autotune. */
+int fk_batch_solver_format(fk_batch *batch, int32_t *format);
 /* phi: nrhs * nv[mesh], right-hand side major */
 int fk_batch_potential(fk_batch *batch, int32_t mesh, double *phi);
 /* Autotuner: times the batched SpMM for lanes per row {4, 8, 16} x block
