@@ -45,6 +45,13 @@ def spmv_bytes(nnz, nv, s):
     return nnz * (s + 4) + 4 * (nv + 1) + 2 * s * nv
 
 
+def stencil_spmv_bytes(K, nv, s):
+    """compulsory bytes of one difference-form SpMV on the symmetric stencil
+    copy: K value columns of nv rows (each off-diagonal pair stored once, no
+    index), x read once, y written once"""
+    return K * s * nv + 2 * s * nv
+
+
 def cpu_baseline(pr, iters_f64, nthreads, budget_s=20.0):
     """Oracle (C restatement of the CPU path) on the host cores: full assembly
     of the same mesh, then PCG iterations timed for a bounded count and scaled
@@ -375,7 +382,9 @@ def main():
     launches, total_ms = ctx.spmv_timing()
     ctx.set_timing(False)
     spmv_us = total_ms / max(launches, 1) * 1e3
-    B32 = spmv_bytes(dm.nnz, dm.nv, 4)
+    B_csr = spmv_bytes(dm.nnz, dm.nv, 4)
+    # the compulsory bytes of the copy the solver streams
+    B32 = stencil_spmv_bytes(len(stencil_offsets), dm.nv, 4) if fmt_name == "stencil" else B_csr
     achieved = B32 / (spmv_us * 1e-6) / 1e9
     log(f"SpMV f32: {launches} launches, {spmv_us:.1f} us avg, {achieved:.0f} GB/s")
 
@@ -424,6 +433,10 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "bytes_per_launch": B32,
+            "bytes_basis": ("stencil copy: K*4*nv values + 4*nv x + 4*nv y" if fmt_name == "stencil"
+                            else "CSR: nnz*(4+4) + 4*(nv+1) + 8*nv"),
+            "csr_formula_bytes": B_csr,
+            "csr_formula_GBs": B_csr / (spmv_us * 1e-6) / 1e9,
             "copy_probe_GBs": copy_gbs,
             "avg_launch_us": spmv_us,
             "launches": launches,

@@ -40,6 +40,11 @@ struct fk_batch {
     fk::DevBuf<int64_t> blk_r0, blk_r1;
     fk::DevBuf<int32_t> mesh_b0, mesh_b1;
     std::vector<int32_t> h_mesh_b0, h_mesh_b1;
+    // block table of the stencil SpMM (finer: 1024 rows per block)
+    int sp_nblocks = 0;
+    int sp_block = 256;         // threads per block of the stencil SpMM
+    fk::DevBuf<int32_t> sp_blk_mesh, sp_mesh_b0, sp_mesh_b1;
+    fk::DevBuf<int64_t> sp_blk_r0, sp_blk_r1;
     // Dirichlet values and potentials, node major [node][rhs]
     fk::DevBuf<double> g, phi, r64;
     bool have_dirichlet = false, have_phi = false;
@@ -55,7 +60,7 @@ struct fk_batch {
     fk::HostBuf<int32_t> hcount;
     hipGraphExec_t graph = nullptr;
     int graph_cfg = -1;
-    int lanes = 8, block = 256;
+    int lanes = 8, block = 512;  // block: the update kernels (and the CSR SpMM)
     // reliable updates, per system
     int precision = FK_PREC_F32_REFINED;
     fk::DevBuf<int32_t> ru_act, ru_mode, ru_fin, ru_nrep, ru_restarts, ru_status, nfin;
@@ -239,68 +244,97 @@ __global__ void kb_stencil_fill(const double *__restrict__ val64, const int32_t 
     out[t] = k >= 0 ? static_cast<float>(val64[k]) : 0.f;
 }
 
-// K2: per system alpha, x += alpha p, r -= alpha q, partials of r.z, r.r
-template <int BLOCK>
+// K2: per system alpha, x += alpha p, r -= alpha q, partials of r.z, r.r.
+// NR right-hand sides per node (contiguous). The block's first row is loaded
+// before the per-mesh reductions so the loads overlap them.
+template <int BLOCK, int NR>
 __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__restrict__ x, float *__restrict__ r,
                                                       const float *__restrict__ p, const float *__restrict__ q,
                                                       const float *__restrict__ dinv,
                                                       const double *__restrict__ part_pq,
+                                                      const int32_t *__restrict__ pq_b0,
+                                                      const int32_t *__restrict__ pq_b1,
                                                       const double *__restrict__ rz_cur, double *__restrict__ rz_next,
                                                       double *__restrict__ part_rr, int32_t *__restrict__ done,
                                                       int32_t *__restrict__ status, int32_t *__restrict__ ndone) {
     __shared__ double sh[BLOCK / 64 + 1];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
-    const int nrhs = ba.nrhs;
-    if (mesh_all_done(done, mesh, nrhs)) return;
+    if (mesh_all_done(done, mesh, NR)) return;
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    const int64_t i0 = r0 + threadIdx.x;
+    float d0 = 0.f, x0[NR], r0v[NR], p0[NR], q0[NR];
+    if (i0 < r1) {
+        d0 = dinv[i0];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            x0[j] = x[i0 * NR + j];
+            r0v[j] = r[i0 * NR + j];
+            p0[j] = p[i0 * NR + j];
+            q0[j] = q[i0 * NR + j];
+        }
+    }
     const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
-    float alpha[4] = {0.f, 0.f, 0.f, 0.f};
-    bool act[4] = {false, false, false, false};
-    for (int j = 0; j < nrhs; ++j) {
-        const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, nrhs, sh);
-        const double pq = seg_sum<BLOCK>(part_pq, b0, b1, j, nrhs, sh);
-        const bool live = !done[mesh * nrhs + j];
+    float alpha[NR];
+    bool act[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, NR, sh);
+        const double pq = seg_sum<BLOCK>(part_pq, pq_b0[mesh], pq_b1[mesh], j, NR, sh);
+        const bool live = !done[mesh * NR + j];
         act[j] = live && pq > 0.0 && isfinite(pq);
         alpha[j] = act[j] ? static_cast<float>(rho / pq) : 0.f;
         if (live && !act[j] && lb == b0 && threadIdx.x == 0) {
             // breakdown (every block of the mesh sees the same pq): stop the
             // system, its caller restarts or reports it
-            status[mesh * nrhs + j] = FK_ERR_SINGULAR;
-            done[mesh * nrhs + j] = 1;
+            status[mesh * NR + j] = FK_ERR_SINGULAR;
+            done[mesh * NR + j] = 1;
             atomicAdd(ndone, 1);
         }
     }
-    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
-    double rz[4] = {0.0, 0.0, 0.0, 0.0}, rr[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
-        const float d = dinv[i];
+    double rz[NR], rr[NR];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (j >= nrhs) continue;
-            const int64_t o = i * nrhs + j;
-            float ri = r[o];
+    for (int j = 0; j < NR; ++j) rz[j] = rr[j] = 0.0;
+    auto row = [&](int64_t i, float d, float *xv, float *rv, const float *pv, const float *qv) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            float ri = rv[j];
             if (act[j]) {
-                x[o] += alpha[j] * p[o];
-                ri = d != 0.f ? ri - alpha[j] * q[o] : 0.f;
-                r[o] = ri;
+                x[i * NR + j] = xv[j] + alpha[j] * pv[j];
+                ri = d != 0.f ? ri - alpha[j] * qv[j] : 0.f;
+                r[i * NR + j] = ri;
             }
             const float z = d * ri;
             rz[j] += static_cast<double>(ri) * static_cast<double>(z);
             rr[j] += static_cast<double>(ri) * static_cast<double>(ri);
         }
+    };
+    if (i0 < r1) row(i0, d0, x0, r0v, p0, q0);
+    for (int64_t i = i0 + BLOCK; i < r1; i += BLOCK) {
+        float xv[NR], rv[NR], pv[NR], qv[NR];
+        const float d = dinv[i];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            xv[j] = x[i * NR + j];
+            rv[j] = r[i * NR + j];
+            pv[j] = p[i * NR + j];
+            qv[j] = q[i * NR + j];
+        }
+        row(i, d, xv, rv, pv, qv);
     }
-    for (int j = 0; j < nrhs; ++j) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
         const double a = block_sum<BLOCK>(rz[j], sh);
-        const double b = block_sum<BLOCK>(rr[j], sh);
+        const double c = block_sum<BLOCK>(rr[j], sh);
         if (threadIdx.x == 0) {
-            rz_next[lb * nrhs + j] = a;
-            part_rr[lb * nrhs + j] = b;
+            rz_next[lb * NR + j] = a;
+            part_rr[lb * NR + j] = c;
         }
     }
 }
 
 // K3: per system convergence test (first block of the mesh), p = D^-1 r + beta p
-template <int BLOCK>
+template <int BLOCK, int NR>
 __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__restrict__ p, const float *__restrict__ r,
                                                      const float *__restrict__ dinv,
                                                      const double *__restrict__ rz_next,
@@ -312,19 +346,30 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
     __shared__ double sh[BLOCK / 64 + 1];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
-    const int nrhs = ba.nrhs;
-    if (mesh_all_done(done, mesh, nrhs)) return;
+    if (mesh_all_done(done, mesh, NR)) return;
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    const int64_t i0 = r0 + threadIdx.x;
+    float d0 = 0.f, rv0[NR], pv0[NR];
+    if (i0 < r1) {
+        d0 = dinv[i0];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            rv0[j] = r[i0 * NR + j];
+            pv0[j] = p[i0 * NR + j];
+        }
+    }
     const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
-    float beta[4] = {0.f, 0.f, 0.f, 0.f};
-    bool act[4] = {false, false, false, false};
-    for (int j = 0; j < nrhs; ++j) {
-        const int s = mesh * nrhs + j;
+    float beta[NR];
+    bool act[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const int s = mesh * NR + j;
         act[j] = !done[s];
-        const double rho_new = seg_sum<BLOCK>(rz_next, b0, b1, j, nrhs, sh);
-        const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, nrhs, sh);
+        const double rho_new = seg_sum<BLOCK>(rz_next, b0, b1, j, NR, sh);
+        const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, NR, sh);
         beta[j] = act[j] ? static_cast<float>(rho_new / rho) : 0.f;
         if (lb == b0) {
-            const double rr = seg_sum<BLOCK>(part_rr, b0, b1, j, nrhs, sh);
+            const double rr = seg_sum<BLOCK>(part_rr, b0, b1, j, NR, sh);
             if (threadIdx.x == 0 && act[j]) {
                 const int64_t it = iters[s] + 1;
                 iters[s] = it;
@@ -346,15 +391,16 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
             }
         }
     }
-    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+    if (i0 < r1) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+            if (act[j]) p[i0 * NR + j] = d0 * rv0[j] + beta[j] * pv0[j];
+    }
+    for (int64_t i = i0 + BLOCK; i < r1; i += BLOCK) {
         const float d = dinv[i];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (j >= nrhs || !act[j]) continue;
-            const int64_t o = i * nrhs + j;
-            p[o] = d * r[o] + beta[j] * p[o];
-        }
+        for (int j = 0; j < NR; ++j)
+            if (act[j]) p[i * NR + j] = d * r[i * NR + j] + beta[j] * p[i * NR + j];
     }
 }
 
@@ -681,38 +727,85 @@ static BatchArgs args_of(fk_batch *b) {
                      b->nblocks, b->nrhs};
 }
 
-// logical blocks cut at mesh boundaries, ~4 per CU overall
-static void build_block_table(fk_batch *b) {
-    const int64_t total = b->node_off.back();
-    const int64_t target_blocks = std::max<int64_t>(b->nmesh, 4 * static_cast<int64_t>(b->ctx->num_cus));
-    const int64_t chunk = std::max<int64_t>(256, (total + target_blocks - 1) / target_blocks);
-    std::vector<int32_t> bm;
+// Logical blocks cut at mesh boundaries, at most `chunk` rows each; with
+// pad8 the count is padded to a multiple of 8 with empty blocks of the last
+// mesh, so that logical_block gives each XCD a contiguous eighth.
+struct TableHost {
+    std::vector<int32_t> bm, b0, b1;
     std::vector<int64_t> r0, r1;
-    b->h_mesh_b0.assign(b->nmesh, 0);
-    b->h_mesh_b1.assign(b->nmesh, 0);
+};
+
+static TableHost make_table(const fk_batch *b, int64_t chunk, bool pad8) {
+    TableHost t;
+    t.b0.assign(b->nmesh, 0);
+    t.b1.assign(b->nmesh, 0);
     for (int m = 0; m < b->nmesh; ++m) {
         const int64_t a = b->node_off[m], e = b->node_off[m + 1];
         const int64_t nb = std::max<int64_t>(1, (e - a + chunk - 1) / chunk);
         const int64_t c = (e - a + nb - 1) / nb;
-        b->h_mesh_b0[m] = static_cast<int32_t>(bm.size());
+        t.b0[m] = static_cast<int32_t>(t.bm.size());
         for (int64_t k = 0; k < nb; ++k) {
-            bm.push_back(m);
-            r0.push_back(std::min(e, a + k * c));
-            r1.push_back(std::min(e, a + (k + 1) * c));
+            t.bm.push_back(m);
+            t.r0.push_back(std::min(e, a + k * c));
+            t.r1.push_back(std::min(e, a + (k + 1) * c));
         }
-        b->h_mesh_b1[m] = static_cast<int32_t>(bm.size());
+        t.b1[m] = static_cast<int32_t>(t.bm.size());
     }
-    b->nblocks = static_cast<int>(bm.size());
-    b->blk_mesh.alloc(bm.size());
-    b->blk_r0.alloc(bm.size());
-    b->blk_r1.alloc(bm.size());
-    b->mesh_b0.alloc(b->nmesh);
-    b->mesh_b1.alloc(b->nmesh);
-    FK_HIP(hipMemcpy(b->blk_mesh.get(), bm.data(), bm.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    FK_HIP(hipMemcpy(b->blk_r0.get(), r0.data(), r0.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-    FK_HIP(hipMemcpy(b->blk_r1.get(), r1.data(), r1.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-    FK_HIP(hipMemcpy(b->mesh_b0.get(), b->h_mesh_b0.data(), b->nmesh * sizeof(int32_t), hipMemcpyHostToDevice));
-    FK_HIP(hipMemcpy(b->mesh_b1.get(), b->h_mesh_b1.data(), b->nmesh * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (pad8) {
+        const int last = b->nmesh - 1;
+        const int64_t e = b->node_off[last + 1];
+        while (t.bm.size() % 8 != 0) {
+            t.bm.push_back(last);
+            t.r0.push_back(e);
+            t.r1.push_back(e);
+        }
+        t.b1[last] = static_cast<int32_t>(t.bm.size());
+    }
+    return t;
+}
+
+static void upload_table(const TableHost &t, int nmesh, fk::DevBuf<int32_t> &bm, fk::DevBuf<int64_t> &r0,
+                         fk::DevBuf<int64_t> &r1, fk::DevBuf<int32_t> &b0, fk::DevBuf<int32_t> &b1) {
+    bm.alloc(t.bm.size());
+    r0.alloc(t.bm.size());
+    r1.alloc(t.bm.size());
+    b0.alloc(nmesh);
+    b1.alloc(nmesh);
+    FK_HIP(hipMemcpy(bm.get(), t.bm.data(), t.bm.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    FK_HIP(hipMemcpy(r0.get(), t.r0.data(), t.r0.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    FK_HIP(hipMemcpy(r1.get(), t.r1.data(), t.r1.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    FK_HIP(hipMemcpy(b0.get(), t.b0.data(), nmesh * sizeof(int32_t), hipMemcpyHostToDevice));
+    FK_HIP(hipMemcpy(b1.get(), t.b1.data(), nmesh * sizeof(int32_t), hipMemcpyHostToDevice));
+}
+
+// the table of every kernel but the stencil SpMM: ~4 blocks per CU overall
+static void build_block_table(fk_batch *b) {
+    const int64_t total = b->node_off.back();
+    const int64_t target_blocks = std::max<int64_t>(b->nmesh, 4 * static_cast<int64_t>(b->ctx->num_cus));
+    const int64_t chunk = std::max<int64_t>(256, (total + target_blocks - 1) / target_blocks);
+    TableHost t = make_table(b, chunk, false);
+    b->nblocks = static_cast<int>(t.bm.size());
+    b->h_mesh_b0 = t.b0;
+    b->h_mesh_b1 = t.b1;
+    upload_table(t, b->nmesh, b->blk_mesh, b->blk_r0, b->blk_r1, b->mesh_b0, b->mesh_b1);
+}
+
+// Stencil SpMM table: 1024 rows per block. Measured on the single-mesh
+// stencil kernel (4.17M rows): blocks of about two 512-row runs, swept in
+// dispatch order inside each XCD's band, take 26-27 us against 36 us for
+// blocks of 8 runs, whose rows one grid plane back have left L2 more often.
+constexpr int64_t kSpmmRows = 1024;
+static void build_spmm_table(fk_batch *b) {
+    TableHost t = make_table(b, kSpmmRows, true);
+    b->sp_nblocks = static_cast<int>(t.bm.size());
+    upload_table(t, b->nmesh, b->sp_blk_mesh, b->sp_blk_r0, b->sp_blk_r1, b->sp_mesh_b0, b->sp_mesh_b1);
+    const size_t need = static_cast<size_t>(std::max(b->sp_nblocks, b->nblocks)) * b->nrhs;
+    b->part_pq.ensure(need);
+}
+
+static BatchArgs spmm_args_of(fk_batch *b) {
+    return BatchArgs{b->sp_blk_mesh.get(), b->sp_blk_r0.get(), b->sp_blk_r1.get(), b->sp_mesh_b0.get(),
+                     b->sp_mesh_b1.get(), b->sp_nblocks, b->nrhs};
 }
 
 template <int BLOCK, int LPR>
@@ -735,8 +828,8 @@ static void launch_spmm_lanes(fk_batch *b, int lanes) {
 
 template <int BLOCK, int NR, int KC>
 static void launch_spmm_stencil_k(fk_batch *b) {
-    kb_spmm_stencil<BLOCK, NR, KC><<<b->nblocks, BLOCK, 0, b->ctx->stream>>>(
-        args_of(b), b->node_off_dev.get(), b->st_K.get(), b->st_off.get(), b->st_vbase.get(), b->st_val32.get(),
+    kb_spmm_stencil<BLOCK, NR, KC><<<b->sp_nblocks, BLOCK, 0, b->ctx->stream>>>(
+        spmm_args_of(b), b->node_off_dev.get(), b->st_K.get(), b->st_off.get(), b->st_vbase.get(), b->st_val32.get(),
         b->p.get(), b->q.get(), b->part_pq.get(), b->done.get());
 }
 
@@ -761,6 +854,7 @@ static void launch_spmm_stencil(fk_batch *b) {
     FK_HIP(hipGetLastError());
 }
 
+// block: the CSR SpMM's threads per block, or with the stencil copy its own
 static void spmm(fk_batch *b, int block, int lanes) {
     if (b->stencil == 1) {
         switch (block) {
@@ -777,27 +871,39 @@ static void spmm(fk_batch *b, int block, int lanes) {
     }
 }
 
-template <int BLOCK>
+template <int BLOCK, int NR>
 static void iteration_tail(fk_batch *b, int parity, int64_t max_iter) {
     hipStream_t st = b->ctx->stream;
     const int nb = b->nblocks, nr = b->nrhs;
     double *rz_cur = b->part_rz.get() + (parity ? nb * nr : 0);
     double *rz_next = b->part_rz.get() + (parity ? 0 : nb * nr);
     fk_mesh *m = b->mesh;
-    kb_update_xr<BLOCK><<<nb, BLOCK, 0, st>>>(args_of(b), b->x.get(), b->r.get(), b->p.get(), b->q.get(),
-                                              m->w32.dinv.get(), b->part_pq.get(), rz_cur, rz_next,
+    kb_update_xr<BLOCK, NR><<<nb, BLOCK, 0, st>>>(args_of(b), b->x.get(), b->r.get(), b->p.get(), b->q.get(),
+                                              m->w32.dinv.get(), b->part_pq.get(),
+                                              b->stencil == 1 ? b->sp_mesh_b0.get() : b->mesh_b0.get(),
+                                              b->stencil == 1 ? b->sp_mesh_b1.get() : b->mesh_b1.get(), rz_cur, rz_next,
                                               b->part_rr.get(), b->done.get(), b->status.get(), b->ndone.get());
-    kb_update_p<BLOCK><<<nb, BLOCK, 0, st>>>(args_of(b), b->p.get(), b->r.get(), m->w32.dinv.get(), rz_next, rz_cur,
+    kb_update_p<BLOCK, NR><<<nb, BLOCK, 0, st>>>(args_of(b), b->p.get(), b->r.get(), m->w32.dinv.get(), rz_next, rz_cur,
                                              b->part_rr.get(), b->done.get(), b->status.get(), b->iters.get(),
                                              b->thresh.get(), max_iter, b->ndone.get(), b->rr32.get());
 }
 
+template <int BLOCK>
+static void iteration_tail_nr(fk_batch *b, int parity, int64_t max_iter) {
+    switch (b->nrhs) {
+        case 1: iteration_tail<BLOCK, 1>(b, parity, max_iter); break;
+        case 2: iteration_tail<BLOCK, 2>(b, parity, max_iter); break;
+        case 3: iteration_tail<BLOCK, 3>(b, parity, max_iter); break;
+        default: iteration_tail<BLOCK, 4>(b, parity, max_iter); break;
+    }
+}
+
 static void enqueue_batch_iteration(fk_batch *b, int parity, int64_t max_iter) {
-    spmm(b, b->block, b->lanes);
+    spmm(b, b->stencil == 1 ? b->sp_block : b->block, b->lanes);
     switch (b->block) {
-        case 128: iteration_tail<128>(b, parity, max_iter); break;
-        case 512: iteration_tail<512>(b, parity, max_iter); break;
-        default: iteration_tail<256>(b, parity, max_iter); break;
+        case 128: iteration_tail_nr<128>(b, parity, max_iter); break;
+        case 512: iteration_tail_nr<512>(b, parity, max_iter); break;
+        default: iteration_tail_nr<256>(b, parity, max_iter); break;
     }
 }
 
@@ -836,7 +942,7 @@ static void inner_solve(fk_batch *b, double inner_rtol, int64_t max_iter) {
                                                    b->ndone.get(), b->nsys);
     FK_HIP(hipGetLastError());
     const int G = 16;
-    const int cfg = b->block * 1000 + b->lanes;
+    const int cfg = b->block * 1000 + b->lanes + 1000000 * b->sp_block;
     if (!b->graph || b->graph_cfg != cfg) {
         if (b->graph) (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -928,7 +1034,7 @@ static void batch_solve_ru(fk_batch *b, double rtol, double delta, int64_t max_i
     kb_ru_start<<<grid_for(S, 64), 64, 0, st>>>(u, target2.get(), b->thresh.get(), b->iters.get());
     FK_HIP(hipGetLastError());
     const int G = 16;
-    const int cfg = b->block * 1000 + b->lanes;
+    const int cfg = b->block * 1000 + b->lanes + 1000000 * b->sp_block;
     const double delta_key = delta;
     if (!b->ru_graph || b->ru_graph_cfg != cfg || b->ru_delta != delta_key || b->ru_max_iter != max_iter) {
         if (b->ru_graph) (void)hipGraphExecDestroy(b->ru_graph);
@@ -1111,6 +1217,7 @@ static void build_batch_stencil(fk_batch *b) {
     FK_HIP(hipMemcpyAsync(b->node_off_dev.get(), b->node_off.data(), (M + 1) * sizeof(int64_t),
                           hipMemcpyHostToDevice, st));
     FK_HIP(hipStreamSynchronize(st));
+    build_spmm_table(b);
     b->st_len = len;
     b->st_Kc = K[0];
     for (int q = 1; q < M; ++q)
@@ -1174,11 +1281,13 @@ void batch_autotune(fk_batch *b, int32_t *lanes_out, int32_t *block_out, double 
     FK_HIP(hipEventCreate(&e0));
     FK_HIP(hipEventCreate(&e1));
     double best = 1e30;
-    int bl = 8, bb = 256;
+    int bl = b->lanes, bb = 256;
     const int blocks[3] = {128, 256, 512};
     const int lanes[3] = {4, 8, 16};
+    // the stencil SpMM has no lanes per row: block sizes only
+    const int nl = b->stencil == 1 ? 1 : 3;
     for (int bi = 0; bi < 3; ++bi)
-        for (int li = 0; li < 3; ++li) {
+        for (int li = 0; li < nl; ++li) {
             spmm(b, blocks[bi], lanes[li]);  // warm
             FK_HIP(hipEventRecord(e0, st));
             const int reps = 10;
@@ -1190,14 +1299,18 @@ void batch_autotune(fk_batch *b, int32_t *lanes_out, int32_t *block_out, double 
             const double us = ms * 1e3 / reps;
             if (us < best) {
                 best = us;
-                bl = lanes[li];
+                bl = b->stencil == 1 ? b->lanes : lanes[li];
                 bb = blocks[bi];
             }
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    b->lanes = bl;
-    b->block = bb;
+    if (b->stencil == 1) {
+        b->sp_block = bb;  // the update kernels keep their block size
+    } else {
+        b->lanes = bl;
+        b->block = bb;
+    }
     if (lanes_out) *lanes_out = bl;
     if (block_out) *block_out = bb;
     if (best_us_out) *best_us_out = best;

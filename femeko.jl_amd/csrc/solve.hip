@@ -114,7 +114,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *__restrict__ r,
                                                          const T *__restrict__ p, const T *__restrict__ q,
                                                          const T *__restrict__ dinv,
-                                                         const double *__restrict__ part_pq,
+                                                         const double *__restrict__ part_pq, int npq,
                                                          const double *__restrict__ part_rz_cur,
                                                          double *__restrict__ part_rz_next,
                                                          double *__restrict__ part_rr, int64_t n, int64_t chunk,
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *_
         }
     }
     const double rho = sum_partials<kBlock>(part_rz_cur, nblocks, sh);
-    const double pq = sum_partials<kBlock>(part_pq, nblocks, sh);
+    const double pq = sum_partials<kBlock>(part_pq, npq, sh);
     if (!(pq > 0.0) || !isfinite(pq)) {
         // breakdown: A not positive definite on the Krylov space (every
         // block sees the same pq and skips the update)
@@ -777,12 +777,29 @@ static int solver_config(const fk_mesh *m) {
     return m->lanes + 100 * m->format + 1000 * m->sell_unroll + 10000 * m->sell_block;
 }
 
+// Blocks of the FP32 stencil SpMV: two runs of 512 rows per block in each
+// XCD's band. Measured at n=160 (4.17M rows): 26.7 us against 35.9 us for the
+// 1024-block grid of the other kernels, whose blocks sweep 8 runs each and
+// drift apart, so the rows one grid plane back have left L2 more often.
+constexpr int kStencilBlock = 512;
+static int stencil_blocks(const fk_mesh *m) {
+    const int64_t band = ((m->nv + 7) / 8 + kStencilBlock - 1) / kStencilBlock * kStencilBlock;
+    const int64_t runs = band / kStencilBlock;
+    return static_cast<int>(8 * std::max<int64_t>(1, (runs + 1) / 2));
+}
+
+// blocks of the SpMV kernel, which writes one p.q partial per block
+template <typename T>
+static int spmv_blocks(const fk_mesh *m) {
+    return sizeof(T) == 4 && m->kind == FK_SPMV_STENCIL ? stencil_blocks(m) : m->nblocks;
+}
+
 template <typename T>
 static void ensure_work(fk_mesh *m) {
     PcgWork<T> &w = work<T>(m);
     const size_t nv = static_cast<size_t>(m->nv), nb = static_cast<size_t>(m->nblocks);
     w.x.ensure(nv); w.r.ensure(nv); w.p.ensure(nv); w.q.ensure(nv); w.dinv.ensure(nv);
-    w.part_pq.ensure(nb); w.part_rz.ensure(2 * nb); w.part_rr.ensure(nb); w.part_bb.ensure(nb);
+    w.part_pq.ensure(std::max<size_t>(nb, static_cast<size_t>(stencil_blocks(m)))); w.part_rz.ensure(2 * nb); w.part_rr.ensure(nb); w.part_bb.ensure(nb);
     w.state.ensure(1);
 }
 
@@ -1189,9 +1206,10 @@ const float *stencil_values<float>(fk_mesh *m) { return m->st_val32.get(); }
 
 template <typename T, int BLOCK, int K>
 static void launch_stencil_k(fk_mesh *m, PcgWork<T> &w, int parity) {
-    k_cg_spmv_stencil<T, BLOCK, K><<<m->nblocks, BLOCK, 0, m->ctx->stream>>>(
+    const int nb = spmv_blocks<T>(m);
+    k_cg_spmv_stencil<T, BLOCK, K><<<nb, BLOCK, 0, m->ctx->stream>>>(
         stencil_view<T>(m, stencil_values<T>(m), sell_diag<T>(m)), w.p.get(), w.q.get(), w.part_pq.get(), m->chunk,
-        m->nblocks, w.state.get(), parity);
+        nb, w.state.get(), parity);
 }
 
 // dispatch on the number of positive offsets (1..16, a compile-time constant
@@ -1222,7 +1240,7 @@ static void launch_stencil(fk_mesh *m, PcgWork<T> &w, int parity) {
 template <typename T>
 static void launch_spmv_lanes(fk_mesh *m, PcgWork<T> &w, int parity) {
     if (m->kind == FK_SPMV_STENCIL) {
-        launch_stencil<T, 512>(m, w, parity);  // measured best of 256/512/1024 at n=160
+        launch_stencil<T, kStencilBlock>(m, w, parity);  // measured best of 256/512/1024 at n=160
         return;
     }
     if (m->format != FK_SPMV_CSR) {
@@ -1264,8 +1282,8 @@ static void enqueue_iteration(fk_mesh *m, PcgWork<T> &w, int parity, bool timed)
         te.used += 2;
     }
     k_cg_update_xr<T><<<nb, kBlock, 0, st>>>(w.x.get(), w.r.get(), w.p.get(), w.q.get(), w.dinv.get(),
-                                             w.part_pq.get(), rz_cur, rz_next, w.part_rr.get(), m->nv, m->chunk, nb,
-                                             w.state.get(), parity);
+                                             w.part_pq.get(), spmv_blocks<T>(m), rz_cur, rz_next, w.part_rr.get(),
+                                             m->nv, m->chunk, nb, w.state.get(), parity);
     k_cg_update_p<T><<<nb, kBlock, 0, st>>>(w.p.get(), w.r.get(), w.dinv.get(), rz_next, rz_cur, w.part_rr.get(),
                                             m->nv, m->chunk, nb, w.state.get(), parity);
 }
