@@ -1885,6 +1885,47 @@ void field(fk_mesh *m, double *H, double *Hn, double *energy) {
     }
 }
 
+// Back-to-back launches of the solver's SpMV (precision T, the format the
+// solver resolved) on its own work vectors, timed by two events on the
+// library stream: the kernel's duration without per-launch event overhead.
+// p is zeroed first (the duration does not depend on the values); a state
+// of its own keeps the kernels from returning at entry.
+template <typename T>
+static double spmv_burst(fk_mesh *m, int reps) {
+    hipStream_t st = m->ctx->stream;
+    ensure_work<T>(m);
+    if (solver_kind(m) == FK_SPMV_CSR && sizeof(T) == 4) make_val32(m);
+    sync_solver_values<T>(m);
+    PcgWork<T> &w = work<T>(m);
+    FK_HIP(hipMemsetAsync(w.p.get(), 0, m->nv * sizeof(T), st));
+    DevBuf<CgState> fresh(1);
+    FK_HIP(hipMemsetAsync(fresh.get(), 0, sizeof(CgState), st));
+    std::swap(w.state, fresh);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto restore = [&] {
+        std::swap(w.state, fresh);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+        FK_HIP(hipEventCreate(&e0));
+        FK_HIP(hipEventCreate(&e1));
+        launch_spmv_lanes<T>(m, w, 0);  // warm-up
+        FK_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < reps; ++k) launch_spmv_lanes<T>(m, w, 0);
+        FK_HIP(hipEventRecord(e1, st));
+        FK_HIP(hipGetLastError());
+        FK_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        restore();
+        return static_cast<double>(ms) * 1e3 / reps;
+    } catch (...) {
+        restore();
+        throw;
+    }
+}
+
 }  // namespace fk
 
 // ---------------------------------------------------------------------------
@@ -1992,6 +2033,16 @@ int fk_copy_probe(fk_context *ctx, int64_t bytes, int32_t reps, double *ms_per_c
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         *ms_per_copy = static_cast<double>(ms) / reps;
+    });
+}
+
+int fk_spmv_burst(fk_mesh *m, int32_t precision, int32_t reps, double *us_per_launch) {
+    if (!m || !us_per_launch || reps < 1 || (precision != FK_PREC_F64 && precision != FK_PREC_F32_REFINED))
+        return FK_ERR_INVALID_ARG;
+    return fk::guard(m->ctx, "fk_spmv_burst", [&] {
+        fk::set_device(m->ctx);
+        if (!m->have_values) fk::fail(FK_ERR_STATE, "matrix not assembled");
+        *us_per_launch = precision == FK_PREC_F64 ? fk::spmv_burst<double>(m, reps) : fk::spmv_burst<float>(m, reps);
     });
 }
 

@@ -99,6 +99,7 @@ SIGNATURES = {
     "fk_set_timing": (_c.c_int, [_vp, _c.c_int32]),
     "fk_spmv_timing": (_c.c_int, [_vp, _i64p, _dp]),
     "fk_copy_probe": (_c.c_int, [_vp, _c.c_int64, _c.c_int32, _dp]),
+    "fk_spmv_burst": (_c.c_int, [_vp, _c.c_int32, _c.c_int32, _c.POINTER(_c.c_double)]),
     "fk_set_lanes_per_row": (_c.c_int, [_vp, _c.c_int32]),
     "fk_set_spmv_format": (_c.c_int, [_vp, _c.c_int32]),
     "fk_set_sell_shape": (_c.c_int, [_vp, _c.c_int32, _c.c_int32]),

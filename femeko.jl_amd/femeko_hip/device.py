@@ -307,6 +307,14 @@ class DeviceMesh:
         code = {"csr": 0, "sell64": 1, "sell": 1, "sell64-rowidx": 2, "stencil": 3, "auto": 4}[fmt]
         self.ctx.check(self._L.fk_set_spmv_format(self.handle, code), "fk_set_spmv_format")
 
+    def spmv_burst(self, precision: str = "f32", reps: int = 50) -> float:
+        """Microseconds per launch of the solver's SpMV over reps back-to-back
+        launches (overwrites the solver's direction vector)."""
+        code = {"f64": _lib.FK_PREC_F64, "f32": _lib.FK_PREC_F32_REFINED}[precision]
+        us = ctypes.c_double(0.0)
+        self.ctx.check(self._L.fk_spmv_burst(self.handle, code, int(reps), ctypes.byref(us)), "fk_spmv_burst")
+        return float(us.value)
+
     def solver_format(self):
         """(format name, stencil offsets or ()) the solver resolved for this mesh."""
         f = ctypes.c_int32(0)

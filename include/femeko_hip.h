@@ -278,6 +278,12 @@ int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms);
  * `bytes` each per copy: the practical HBM ceiling of the device, and the
  * kernel of known traffic that calibrates hardware byte counters. */
 int fk_copy_probe(fk_context *ctx, int64_t bytes, int32_t reps, double *ms_per_copy);
+/* Duration of the solver's SpMV kernel (FK_PREC_F64: the FP64 kernel,
+ * FK_PREC_F32_REFINED: the FP32 one, in the format the solver uses for this
+ * mesh) from reps back-to-back launches between two events on the library
+ * stream, microseconds per launch. Overwrites the solver's direction vector;
+ * call between solves. A measurement aid with no counterpart in the reference. */
+int fk_spmv_burst(fk_mesh *mesh, int32_t precision, int32_t reps, double *us_per_launch);
 /* SpMV tuning: lanes per row (4, 8, 16, 32, 64) of the CSR kernels. */
 int fk_set_lanes_per_row(fk_mesh *mesh, int32_t lanes);
 /* Matrix format of the solver's SpMV (copies are refreshed from the CSR

@@ -56,6 +56,7 @@ def test_null_arguments_rejected_without_gpu():
     assert L.fk_set_sell_shape(None, 256, 1) == _lib.FK_ERR_INVALID_ARG
     assert L.fk_set_spmv_format(None, 1) == _lib.FK_ERR_INVALID_ARG
     assert L.fk_batch_set_format(None, 4) == _lib.FK_ERR_INVALID_ARG
+    assert L.fk_spmv_burst(None, 1, 10, ctypes.byref(ms)) == _lib.FK_ERR_INVALID_ARG
     assert L.fk_batch_solver_format(None, None) == _lib.FK_ERR_INVALID_ARG
 
 

@@ -445,3 +445,24 @@ def test_copy_probe_bandwidth_is_physical():
     assert 100.0 < gbs < 8000.0  # below the HBM peak, far above PCIe
     with pytest.raises(FemekoHIPError):
         ctx.copy_probe(8, 1)
+
+
+@pytest.mark.parametrize("fmt", ["stencil", "sell64", "csr"])
+def test_spmv_burst_leaves_solver_usable(lin20, fmt):
+    # the burst timing swaps in a state of its own and zeroes the direction
+    # vector: the next solve (graph replays included) must be unaffected
+    pr, A, dm = lin20
+    try:
+        dm.set_format(fmt)
+        golden = np.load(GOLDEN)
+        for prec in ("f32", "f64"):
+            dm.solve(prec, rtol=1e-10)
+            us = dm.spmv_burst(prec, 5)
+            assert 0.0 < us < 1e5
+            rep = dm.solve(prec, rtol=1e-10)
+            assert rep.relres < 1e-10
+            assert np.linalg.norm(dm.potential() - golden["phi"]) / np.linalg.norm(golden["phi"]) < 1e-9
+        with pytest.raises(FemekoHIPError):
+            dm.spmv_burst("f32", 0)
+    finally:
+        dm.set_format("auto")
