@@ -107,9 +107,14 @@ __device__ __forceinline__ bool mesh_all_done(const int32_t *__restrict__ done, 
 template <int BLOCK>
 __device__ __forceinline__ double seg_sum(const double *__restrict__ part, int b0, int b1, int k, int nrhs,
                                           double *sh) {
-    double s = 0.0;
-    for (int b = b0 + static_cast<int>(threadIdx.x); b < b1; b += BLOCK) s += part[b * nrhs + k];
-    return block_sum<BLOCK>(s, sh);
+    double s0 = 0.0, s1 = 0.0;
+    int b = b0 + static_cast<int>(threadIdx.x);
+    for (; b + BLOCK < b1; b += 2 * BLOCK) {
+        s0 += part[b * nrhs + k];
+        s1 += part[(b + BLOCK) * nrhs + k];
+    }
+    for (; b < b1; b += BLOCK) s0 += part[b * nrhs + k];
+    return block_sum<BLOCK>(s0 + s1, sh);
 }
 
 // K1: q = A p for all right-hand sides of the block's mesh, partials of p.q.
