@@ -6,8 +6,12 @@ metric is quoted on): Kuhn box mesh n = 160 (24,576,000 tets, 4,173,281 nodes,
 log-uniform mu_r in [1, 1e3], Dirichlet phi = -x on the outer faces. A step is
 value-only assembly of A(mu) on the GPU plus the FP32 Jacobi-PCG whose
 recursive residual is replaced by the FP64 true residual b - A64 phi each time
-it has fallen by --delta (reliable updates), to FP64 relres 1e-10; mesh,
-pattern, mu and the Dirichlet data are resident in HBM before timing starts.
+it has fallen by --delta (reliable updates), to FP64 relres 5e-11 (--rtol);
+mesh, pattern, mu and the Dirichlet data are resident in HBM before timing
+starts. The plain FP64 path runs to the configured 1e-10 (--rtol64): at
+those tolerances both potentials are within about 1.5e-10 of an FP64 solve
+at 1e-13, while the FP32-refined solve at 1e-10 left up to 1.06e-9 (its
+remaining error sits in smooth modes that weigh little in the residual).
 
 Reported beside the headline: the plain FP64 GPU solve of the same system
 (speed-up), the classic restarted refinement (FP32 PCG to 1e-5 inside an FP64
@@ -260,6 +264,12 @@ def main():
     ap.add_argument("--n", type=int, default=160)
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--format", default="auto", choices=["auto", "stencil", "csr", "sell64", "sell64-rowidx"])
+    ap.add_argument("--rtol", type=float, default=5e-11,
+                    help="relative FP64 residual the FP32-refined solve stops at (5e-11: its phi error then "
+                         "stays at or below the FP64 path's at 1e-10 on every seed measured; at 1e-10 it "
+                         "ranged 1.4e-10 to 1.06e-9)")
+    ap.add_argument("--rtol64", type=float, default=1e-10,
+                    help="relative residual the plain FP64 solve stops at (the configured 1e-10)")
     ap.add_argument("--delta", type=float, default=0.9,
                     help="FP32 solve: residual drop that triggers an FP64 residual replacement")
     ap.add_argument("--shape", default="1024,1",
@@ -323,7 +333,9 @@ def main():
 
     def step(precision):
         dm.assemble()
-        return dm.solve(precision, rtol=1e-10, inner_rtol=args.delta if precision == "f32" else None)
+        if precision == "f32":
+            return dm.solve("f32", rtol=args.rtol, inner_rtol=args.delta)
+        return dm.solve("f64", rtol=args.rtol64)
 
     # ---- headline: FP32-refined
     for _ in range(args.warmup):
@@ -356,6 +368,7 @@ def main():
     ctx.synchronize()
     t_step64 = (time.perf_counter() - t0) / nrep64
     log(f"f64 step {t_step64*1e3:.1f} ms, iterations {rep64.iterations}, relres {rep64.relres:.2e}")
+    phi64 = dm.potential()
 
     # ---- FP64 reference at rtol 1e-13 for the error report
     dm.solve("f64", rtol=1e-13)
@@ -365,7 +378,9 @@ def main():
     err_phi = float(np.linalg.norm(phi32 - phi_ref) / np.linalg.norm(phi_ref))
     err_H = float(np.sqrt(np.sum(VE * (Hn32 - Hn_ref) ** 2) / np.sum(VE * Hn_ref ** 2)))
     err_E = float(abs(E32 - E_ref) / abs(E_ref))
-    log(f"errors vs FP64 rtol 1e-13: phi {err_phi:.2e} |H| {err_H:.2e} energy {err_E:.2e}")
+    err_phi64 = float(np.linalg.norm(phi64 - phi_ref) / np.linalg.norm(phi_ref))
+    log(f"errors vs FP64 rtol 1e-13: phi {err_phi:.2e} |H| {err_H:.2e} energy {err_E:.2e}; "
+        f"the FP64 path's phi {err_phi64:.2e}")
 
     # ---- practical streaming ceiling: plain 1 GiB device copy
     copy_ms = ctx.copy_probe(1 << 30, 5)
@@ -376,7 +391,7 @@ def main():
     ctx.set_timing(True)
     dm.assemble()
     t0 = time.perf_counter()
-    rep_t = dm.solve("f32", rtol=1e-10, inner_rtol=args.delta)
+    rep_t = dm.solve("f32", rtol=args.rtol, inner_rtol=args.delta)
     ctx.synchronize()
     t_instr = time.perf_counter() - t0
     launches, total_ms = ctx.spmv_timing()
@@ -410,7 +425,7 @@ def main():
                          f"Kuhn box mesh n={args.n}, jitter 0.15h, 8 log-uniform mu_r boxes in "
                          f"[1,1e3], Dirichlet phi=-x; step = value assembly + FP32 Jacobi-PCG whose residual is "
                          f"replaced by the FP64 true residual after every {args.delta:g}x drop, to FP64 "
-                         f"relres 1e-10"),
+                         f"relres {args.rtol:g} (FP64 comparison path to {args.rtol64:g})"),
             "nodes": m.nv, "tets": m.nt, "nnz": dm.nnz, "free_dofs": free, "colours": dm.ncolours,
             "lanes_per_row": args.lanes, "spmv_format": fmt_name, "stencil_offsets": list(stencil_offsets),
             "sell_shape": list(shape),
@@ -423,6 +438,7 @@ def main():
         "relres": rep.relres,
         "ms_assembly": t_asm * 1e3,
         "fp64": {"ms_per_step": t_step64 * 1e3, "iterations": rep64.iterations, "relres": rep64.relres,
+                 "phi_rel_l2_vs_rtol1e-13": err_phi64,
                  "dof_per_s": m.nv / t_step64},
         "speedup_vs_fp64": t_step64 / t_step,
         "error_vs_fp64_rtol1e-13": {"phi_rel_l2": err_phi, "Hn_rel_l2_VE": err_H, "energy_rel": err_E},
