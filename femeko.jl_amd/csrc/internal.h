@@ -180,6 +180,7 @@ struct fk_mesh {
     fk::DevBuf<double> coef;          // per element (colour order)
     bool have_coef = false;
     fk::DevBuf<double> dmu;           // d mu / d|H| per element (colour order), nonlinear loop
+    fk::DevBuf<double> hn_prev;       // |H| of the previous step (colour order), tabulated-law loop
     fk::DevBuf<uint8_t> region;       // nonlinear material region (colour order)
     fk::DevBuf<uint8_t> target;       // objective region of the adjoint (colour order)
     fk::DevBuf<double> val64;

@@ -298,6 +298,7 @@ const char *fk_status_string(int s) {
         case FK_ERR_OOM: return "out of memory";
         case FK_ERR_STATE: return "wrong call order";
         case FK_ERR_UNSUPPORTED: return "unsupported mesh";
+        case FK_ERR_DOMAIN: return "argument outside the table range";
         default: return "unknown status";
     }
 }

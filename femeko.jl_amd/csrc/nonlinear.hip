@@ -1,7 +1,8 @@
 // Nonlinear magnetostatics on the GPU: the Picard/Newton mu(|H|) loop of
-// Magnetostatics/NewtonRaphson.jl:148-212 with the Froehlich law
-// mu = mu0 (1 + Ms/(a + |H|)) in the magnet region, and lineSearch
-// (src/FEM3D.jl:123-152).
+// Magnetostatics/NewtonRaphson.jl:148-212 with either the Froehlich law
+// mu = mu0 (1 + Ms/(a + |H|)) or the reference's tabulated law (two cubic
+// splines of |H|, NewtonRaphson.jl:57-63, 204-207) in the magnet region, and
+// lineSearch (src/FEM3D.jl:123-152).
 //
 // Per outer iteration, everything stays on the device:
 //   k_assemble_newton (one launch per colour): gradients, H = -grad(phi) of
@@ -11,9 +12,11 @@
 //       separate SpMV for the residual;
 //   FP64 Jacobi-PCG for the update du (rtol relative to ||r||);
 //   phi += du;
-//   k_field_mu: H of the new potential, mu and mu' from the law (the
-//       "mu update fused into the element gradient kernel"), block maxima of
-//       |mu_new - mu| / mu_new over the magnet and a non-finite counter.
+//   k_field_mu / k_field_spline: H of the new potential, mu and mu' from the
+//       law (the "mu update fused into the element gradient kernel"), block
+//       maxima of the law's change measure over the magnet (Froehlich:
+//       |mu_new - mu| / mu_new; tabulated: mu0 | |H| - |H|_prev |, the
+//       reference's) and a counter of non-finite or out-of-table results.
 // The host reads one max per iteration to decide convergence and whether the
 // next step is Newton (previous change below `switch_tol`) or Picard;
 // iteration 1 starts from mu = mu0 with no tangent, as the reference's
@@ -132,6 +135,90 @@ __global__ __launch_bounds__(kBlock) void k_field_mu(const int4 *__restrict__ te
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
+// Cubic spline sum_j c[j] B_j(x) with n knots t (FITPACK form, c[0:n-4]), the
+// value Dierckx.jl's Spline1D returns. The four B-splines that are nonzero on
+// the knot interval t[l] <= x < t[l+1] (3 <= l <= n-5; the right end t[n-4]
+// belongs to the last interval) come from the de Boor-Cox recurrence
+// B_{i,j} = w_{i,j} B_{i,j-1} + (1 - w_{i+1,j}) B_{i+1,j-1} with
+// w_{i,j}(x) = (x - t_i) / (t_{i+j} - t_i), starting from B_{l,0} = 1.
+// Outside [t[3], t[n-4]] it sets *out and returns 0.
+__device__ __forceinline__ double bspline3(const double *__restrict__ t, const double *__restrict__ c, int n,
+                                           double x, int *out) {
+    if (!(x >= t[3] && x <= t[n - 4])) {
+        *out = 1;
+        return 0.0;
+    }
+    int lo = 3, hi = n - 5;  // largest l in [3, n-5] with t[l] <= x
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t[mid] <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int l = lo;
+    double b[5] = {0.0, 0.0, 0.0, 1.0, 0.0};  // b[i] = B_{l-3+i, j}
+#pragma unroll
+    for (int j = 1; j <= 3; ++j) {
+#pragma unroll
+        for (int i = 3 - j; i <= 3; ++i) {
+            const int idx = l - 3 + i;
+            const double d0 = t[idx + j] - t[idx];
+            const double d1 = t[idx + j + 1] - t[idx + 1];
+            const double left = d0 > 0.0 ? (x - t[idx]) / d0 * b[i] : 0.0;
+            const double right = d1 > 0.0 ? (t[idx + j + 1] - x) / d1 * b[i + 1] : 0.0;
+            b[i] = left + right;
+        }
+    }
+    return c[l - 3] * b[0] + c[l - 2] * b[1] + c[l - 1] * b[2] + c[l] * b[3];
+}
+
+struct SplineLaw {
+    const double *t_mu, *c_mu, *t_dmu, *c_dmu;
+    int n_mu, n_dmu;
+};
+
+// H of phi, tabulated mu and mu' in the region, block max of
+// mu0 | |H| - |H|_prev |, count of out-of-table or non-finite results
+__global__ __launch_bounds__(kBlock) void k_field_spline(const int4 *__restrict__ tet,
+                                                         const double *__restrict__ X,
+                                                         const double *__restrict__ Y,
+                                                         const double *__restrict__ Z,
+                                                         const double *__restrict__ phi,
+                                                         const uint8_t *__restrict__ region, double mu0,
+                                                         const SplineLaw law, int64_t nt,
+                                                         double *__restrict__ mu, double *__restrict__ dmu,
+                                                         double *__restrict__ hn_prev, double *__restrict__ part,
+                                                         int32_t *__restrict__ bad, int32_t *__restrict__ outside) {
+    __shared__ double sh[kBlock / 64 + 1];
+    double dev = 0.0;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; e < nt;
+         e += static_cast<int64_t>(gridDim.x) * kBlock) {
+        if (!region[e]) continue;
+        const int4 nd = tet[e];
+        const Grad G = element_gradients(X, Y, Z, nd);
+        const double u0 = phi[nd.x], u1 = phi[nd.y], u2 = phi[nd.z], u3 = phi[nd.w];
+        double hh = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double h = -(u0 * G.g[0][c] + u1 * G.g[1][c] + u2 * G.g[2][c] + u3 * G.g[3][c]);
+            hh += h * h;
+        }
+        const double hn = sqrt(hh);
+        int out = 0;
+        const double mnew = bspline3(law.t_mu, law.c_mu, law.n_mu, hn, &out);
+        const double dnew = bspline3(law.t_dmu, law.c_dmu, law.n_dmu, hn, &out);
+        if (out) atomicAdd(outside, 1);
+        if (!isfinite(mnew) || !isfinite(dnew)) atomicAdd(bad, 1);
+        dev = fmax(dev, mu0 * fabs(hn - hn_prev[e]));
+        hn_prev[e] = hn;
+        mu[e] = mnew;
+        dmu[e] = dnew;
+    }
+    const double m = block_max<kBlock>(dev, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
 // y = u + alpha du (trial of the line search)
 __global__ void k_axpy_to(const double *__restrict__ u, const double *__restrict__ du, double alpha,
                           double *__restrict__ y, int64_t n) {
@@ -180,8 +267,33 @@ static double host_sum(const double *p, int n) {
     return s;
 }
 
-void nonlinear_froehlich(fk_mesh *m, const uint8_t *region_host, double Ms, double a, double rtol, double tol_mu,
-                         int max_outer, double switch_tol, int32_t *outer_out, int64_t *cg_out, double *dev_out) {
+// the law of one loop: Froehlich (spline == nullptr) or tabulated
+struct LawArgs {
+    double Ms = 0.0, a = 1.0;
+    const SplineLaw *spline = nullptr;
+};
+
+static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const LawArgs &law, double rtol,
+                                double tol, int max_outer, double switch_tol, int32_t *outer_out, int64_t *cg_out,
+                                double *dev_out);
+
+// The loop leaves A + At of its last step in the values: they are marked
+// stale on every exit, errors included, so a later solve reassembles A(mu).
+static void nonlinear_loop(fk_mesh *m, const uint8_t *region_host, const LawArgs &law, double rtol, double tol,
+                           int max_outer, double switch_tol, int32_t *outer_out, int64_t *cg_out,
+                           double *dev_out) {
+    try {
+        nonlinear_loop_body(m, region_host, law, rtol, tol, max_outer, switch_tol, outer_out, cg_out, dev_out);
+    } catch (...) {
+        m->have_values = false;
+        throw;
+    }
+    m->have_values = false;
+}
+
+static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const LawArgs &law, double rtol,
+                                double tol, int max_outer, double switch_tol, int32_t *outer_out, int64_t *cg_out,
+                                double *dev_out) {
     if (!m->have_dirichlet) fail(FK_ERR_STATE, "no Dirichlet data set");
     if (m->nfixed == 0) fail(FK_ERR_SINGULAR, "no fixed node: the pure Neumann system is singular");
     hipStream_t st = m->ctx->stream;
@@ -204,13 +316,17 @@ void nonlinear_froehlich(fk_mesh *m, const uint8_t *region_host, double Ms, doub
     ensure_work_f64(m);
     k_fill_f64b<<<grid_for(nt, 256), 256, 0, st>>>(m->coef.get(), nt, mu0);
     k_fill_f64b<<<grid_for(nt, 256), 256, 0, st>>>(m->dmu.get(), nt, 0.0);
+    if (law.spline) {
+        m->hn_prev.ensure(nt);
+        k_fill_f64b<<<grid_for(nt, 256), 256, 0, st>>>(m->hn_prev.get(), nt, 0.0);
+    }
     k_init_phi_dev(m, 0);  // phi = g on fixed nodes, 0 elsewhere
     m->have_coef = true;
     m->coef_nat_valid = false;  // the loop updates coef (colour order) on the device
     m->have_phi = true;
     const int nb = std::max(1, m->nblocks);
     m->part_misc.ensure(nb);
-    DevBuf<int32_t> bad(1);
+    DevBuf<int32_t> bad(2);  // non-finite, outside the table
     double dev = INFINITY;
     int32_t it = 0;
     int64_t cg = 0;
@@ -235,25 +351,66 @@ void nonlinear_froehlich(fk_mesh *m, const uint8_t *region_host, double Ms, doub
         if (s.status != FK_OK && s.status != FK_ERR_NOT_CONVERGED)
             fail(s.status, std::string("inner solve: ") + fk_status_string(s.status));
         phi_axpy_f64(m, w.x.get(), 1.0);
-        FK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), st));
-        k_field_mu<<<nb, kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->phi.get(),
-                                          m->region.get(), mu0, Ms, a, nt, m->coef.get(), m->dmu.get(),
-                                          m->part_misc.get(), bad.get());
+        FK_HIP(hipMemsetAsync(bad.get(), 0, 2 * sizeof(int32_t), st));
+        if (law.spline)
+            k_field_spline<<<nb, kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->phi.get(),
+                                                  m->region.get(), mu0, *law.spline, nt, m->coef.get(),
+                                                  m->dmu.get(), m->hn_prev.get(), m->part_misc.get(), bad.get(),
+                                                  bad.get() + 1);
+        else
+            k_field_mu<<<nb, kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->phi.get(),
+                                              m->region.get(), mu0, law.Ms, law.a, nt, m->coef.get(),
+                                              m->dmu.get(), m->part_misc.get(), bad.get());
         FK_HIP(hipGetLastError());
-        int32_t hbad = 0;
+        int32_t hbad[2] = {0, 0};
         FK_HIP(hipMemcpyAsync(m->hpart.ptr, m->part_misc.get(), nb * sizeof(double), hipMemcpyDeviceToHost, st));
-        FK_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipMemcpyAsync(hbad, bad.get(), 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
         FK_HIP(hipStreamSynchronize(st));
-        if (hbad) fail(FK_ERR_NONFINITE, "Nans/Infs in mu");  // NewtonRaphson.jl:210-212
+        if (hbad[1]) fail(FK_ERR_DOMAIN, std::to_string(hbad[1]) + " element(s) with |H| outside the table");
+        if (hbad[0]) fail(FK_ERR_NONFINITE, "Nans/Infs in mu");  // NewtonRaphson.jl:210-212
         dev = host_max(m->hpart.ptr, nb);
-        if (dev < tol_mu) break;
+        if (dev < tol) break;
     }
-    // the values hold A + At of the last step: reassemble A(mu) for later use
-    m->have_values = false;
     if (outer_out) *outer_out = std::min(it, max_outer);
     if (cg_out) *cg_out = cg;
     if (dev_out) *dev_out = dev;
-    if (!(dev < tol_mu)) fail(FK_ERR_NOT_CONVERGED, "mu update did not reach tol_mu");
+    if (!(dev < tol)) fail(FK_ERR_NOT_CONVERGED, "mu update did not reach its tolerance");
+}
+
+void nonlinear_froehlich(fk_mesh *m, const uint8_t *region_host, double Ms, double a, double rtol, double tol_mu,
+                         int max_outer, double switch_tol, int32_t *outer_out, int64_t *cg_out, double *dev_out) {
+    LawArgs law;
+    law.Ms = Ms;
+    law.a = a;
+    nonlinear_loop(m, region_host, law, rtol, tol_mu, max_outer, switch_tol, outer_out, cg_out, dev_out);
+}
+
+// knots non-decreasing and finite, a non-empty domain, finite coefficients
+static void check_spline(const double *t, int n, const double *c, const char *name) {
+    if (n < 8) fail(FK_ERR_INVALID_ARG, std::string(name) + ": a cubic spline needs at least 8 knots");
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(t[i]) || (i > 0 && t[i] < t[i - 1]))
+            fail(FK_ERR_INVALID_ARG, std::string(name) + ": knots must be finite and non-decreasing");
+    if (!(t[3] < t[n - 4])) fail(FK_ERR_INVALID_ARG, std::string(name) + ": empty knot domain");
+    for (int i = 0; i < n - 4; ++i)
+        if (!std::isfinite(c[i])) fail(FK_ERR_NONFINITE, std::string(name) + ": non-finite coefficient");
+}
+
+void nonlinear_spline(fk_mesh *m, const uint8_t *region_host, const double *t_mu, int n_mu, const double *c_mu,
+                      const double *t_dmu, int n_dmu, const double *c_dmu, double rtol, double tol_H, int max_outer,
+                      double switch_tol, int32_t *outer_out, int64_t *cg_out, double *dev_out) {
+    check_spline(t_mu, n_mu, c_mu, "mu spline");
+    check_spline(t_dmu, n_dmu, c_dmu, "dmu spline");
+    hipStream_t st = m->ctx->stream;
+    DevBuf<double> tm(n_mu), cm(n_mu - 4), td(n_dmu), cd(n_dmu - 4);
+    FK_HIP(hipMemcpyAsync(tm.get(), t_mu, n_mu * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(cm.get(), c_mu, (n_mu - 4) * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(td.get(), t_dmu, n_dmu * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(cd.get(), c_dmu, (n_dmu - 4) * sizeof(double), hipMemcpyHostToDevice, st));
+    const SplineLaw sl{tm.get(), cm.get(), td.get(), cd.get(), n_mu, n_dmu};
+    LawArgs law;
+    law.spline = &sl;
+    nonlinear_loop(m, region_host, law, rtol, tol_H, max_outer, switch_tol, outer_out, cg_out, dev_out);
 }
 
 double line_search(fk_mesh *m, double *u_host, const double *du_host, const double *rhs_host, double min_step) {
@@ -316,6 +473,20 @@ int fk_nonlinear_froehlich(fk_mesh *m, const uint8_t *region, double Ms, double 
         fk::set_device(m->ctx);
         fk::nonlinear_froehlich(m, region, Ms, a, rtol, tol_mu, max_outer, switch_tol, outer, cg_iterations,
                                 final_change);
+    });
+}
+
+int fk_nonlinear_spline(fk_mesh *m, const uint8_t *region, const double *t_mu, int32_t n_mu, const double *c_mu,
+                        const double *t_dmu, int32_t n_dmu, const double *c_dmu, double rtol, double tol_H,
+                        int32_t max_outer, double switch_tol, int32_t *outer, int64_t *cg_iterations,
+                        double *final_change) {
+    if (!m || !region || !t_mu || !c_mu || !t_dmu || !c_dmu || n_mu < 8 || n_dmu < 8 || !(rtol > 0.0) ||
+        !(tol_H > 0.0) || max_outer < 1)
+        return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_nonlinear_spline", [&] {
+        fk::set_device(m->ctx);
+        fk::nonlinear_spline(m, region, t_mu, n_mu, c_mu, t_dmu, n_dmu, c_dmu, rtol, tol_H, max_outer, switch_tol,
+                             outer, cg_iterations, final_change);
     });
 }
 

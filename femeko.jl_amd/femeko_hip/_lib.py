@@ -23,6 +23,7 @@ FK_ERR_SINGULAR = 7
 FK_ERR_OOM = 8
 FK_ERR_STATE = 9
 FK_ERR_UNSUPPORTED = 10
+FK_ERR_DOMAIN = 11
 
 FK_PREC_F64 = 1
 FK_PREC_F32_REFINED = 2
@@ -79,6 +80,8 @@ SIGNATURES = {
     "fk_solve_neumann": (_c.c_int, [_vp, _dp, _c.c_double, _c.c_int64, _dp, _dp, _i64p, _dp]),
     "fk_nonlinear_froehlich": (_c.c_int, [_vp, _up, _c.c_double, _c.c_double, _c.c_double, _c.c_double,
                                           _c.c_int32, _c.c_double, _ip, _i64p, _dp]),
+    "fk_nonlinear_spline": (_c.c_int, [_vp, _up, _dp, _c.c_int32, _dp, _dp, _c.c_int32, _dp, _c.c_double,
+                                       _c.c_double, _c.c_int32, _c.c_double, _ip, _i64p, _dp]),
     "fk_line_search": (_c.c_int, [_vp, _dp, _dp, _dp, _c.c_double, _dp]),
     "fk_get_coefficient": (_c.c_int, [_vp, _dp]),
     "fk_adjoint_field_target": (_c.c_int, [_vp, _up, _dp, _c.c_double, _dp, _dp, _i64p]),

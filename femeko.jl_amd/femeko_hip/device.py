@@ -101,6 +101,17 @@ def _f64(a, n=None, name="array"):
     return a
 
 
+def _spline_arrays(spl, name):
+    """(knots, coefficients) of a cubic spline in FITPACK form: n knots and
+    n - 4 coefficients (extra trailing coefficients, as scipy's splrep
+    returns, are dropped)."""
+    t = np.ascontiguousarray(getattr(spl, "t", spl[0] if isinstance(spl, tuple) else None), dtype=np.float64)
+    c = np.ascontiguousarray(getattr(spl, "c", spl[1] if isinstance(spl, tuple) else None), dtype=np.float64)
+    if t.ndim != 1 or c.ndim != 1 or t.size < 8 or c.size < t.size - 4:
+        raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, f"{name}: need n >= 8 knots and n - 4 coefficients")
+    return t, np.ascontiguousarray(c[:t.size - 4])
+
+
 class DeviceMesh:
     """A MESH resident on the GPU: coordinates, connectivity in colour order,
     CSR pattern and element -> slot map, coefficients, values, Dirichlet data,
@@ -273,6 +284,28 @@ class DeviceMesh:
                                            ctypes.byref(cg), ctypes.byref(ch))
         if raise_on_error:
             self.ctx.check(s, "fk_nonlinear_froehlich")
+        return {"outer": int(outer.value), "cg_iterations": int(cg.value), "change": float(ch.value), "status": s}
+
+    def nonlinear_spline(self, region, spl_mu, spl_dmu, rtol: float = 1e-10, tol_H: float = 1e-4,
+                         max_outer: int = 100, switch_tol: float = float("inf"), raise_on_error: bool = True):
+        """The mu(|H|) loop with the tabulated law of NewtonRaphson.jl:204-207:
+        spl_mu, spl_dmu are cubic splines in FITPACK form (objects with
+        knots t and coefficients c, e.g. femeko_hip.material.Spline1D).
+        Stops when mu0 max | |H| - |H|_prev | over the region < tol_H.
+        Returns dict(outer, cg_iterations, change, status)."""
+        reg = np.ascontiguousarray(region, dtype=np.uint8)
+        if reg.size != self.nt:
+            raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "region size")
+        tm, cm = _spline_arrays(spl_mu, "spl_mu")
+        td, cd = _spline_arrays(spl_dmu, "spl_dmu")
+        outer = ctypes.c_int32(0)
+        cg = ctypes.c_int64(0)
+        ch = ctypes.c_double(0.0)
+        s = self._L.fk_nonlinear_spline(self.handle, uptr(reg), dptr(tm), tm.size, dptr(cm), dptr(td), td.size,
+                                        dptr(cd), float(rtol), float(tol_H), int(max_outer), float(switch_tol),
+                                        ctypes.byref(outer), ctypes.byref(cg), ctypes.byref(ch))
+        if raise_on_error:
+            self.ctx.check(s, "fk_nonlinear_spline")
         return {"outer": int(outer.value), "cg_iterations": int(cg.value), "change": float(ch.value), "status": s}
 
     def adjoint_field_target(self, target, Hstar, rtol: float = 1e-10):

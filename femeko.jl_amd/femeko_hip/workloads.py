@@ -105,6 +105,17 @@ def froehlich(n: int = 100, jitter: float = 0.2, seed: int = 100, Ms: float = 1.
                    {"n": n, "jitter": jitter, "seed": seed, "Ms": Ms, "a": a})
 
 
+def tabulated(n: int = 100, jitter: float = 0.2, seed: int = 101, Bext: float = 0.5, box=0.25) -> Problem:
+    """Cube magnet [-box, box]^3 in the air box under the applied field
+    Bext/mu0 along x (Magnetostatics/NewtonRaphson.jl:33-35: Bext = 0.5 T),
+    for the loop with a tabulated mu(|H|) law."""
+    mesh = kuhn_box_mesh(n, jitter, seed)
+    magnet = elements_in_box(mesh, (-box,) * 3, (box,) * 3)
+    mu = np.full(mesh.nt, MU0)
+    return _finish(f"tabulated{n}", mesh, mu, (Bext / MU0, 0.0, 0.0), magnet,
+                   {"n": n, "jitter": jitter, "seed": seed, "Bext": Bext})
+
+
 def random_boxes(seed: int, count: int, k0: int = 16):
     """count boxes: centre U(-0.7, 0.7)^3, edge lengths U(0.1, 0.6)^3."""
     u = _draw(seed, k0, 6 * count).reshape(count, 6)

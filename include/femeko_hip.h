@@ -48,6 +48,7 @@ extern "C" {
 #define FK_ERR_OOM 8           /* device or host allocation failed */
 #define FK_ERR_STATE 9         /* call order: e.g. solve before assemble */
 #define FK_ERR_UNSUPPORTED 10  /* mesh beyond the library's limits */
+#define FK_ERR_DOMAIN 11       /* argument outside a table's range (Spline1D bc = "error") */
 
 /* precision of the linear solve */
 #define FK_PREC_F64 1          /* FP64 matrix, FP64 Jacobi-PCG */
@@ -195,6 +196,19 @@ int fk_solve_neumann(fk_mesh *mesh, const double *b, double rtol, int64_t max_it
 int fk_nonlinear_froehlich(fk_mesh *mesh, const uint8_t *region, double Ms, double a, double rtol,
                            double tol_mu, int32_t max_outer, double switch_tol, int32_t *outer,
                            int64_t *cg_iterations, double *final_change);
+/* The same loop with the tabulated law of Magnetostatics/NewtonRaphson.jl:
+ * 57-63, 204-207: mu = spl(|H|), dmu = spl_dmu(|H|) on the region, two cubic
+ * splines in FITPACK form as Dierckx.jl's Spline1D(H, mu; k=3, s=0) holds
+ * them (get_knots, get_coeffs): n knots t and n - 4 coefficients c each,
+ * evaluated on the GPU. |H| outside [t[3], t[n-4]] returns FK_ERR_DOMAIN
+ * (the reference's bc = "error"). Stops when mu0 max | |H| - |H|_prev | over
+ * the region < tol_H (NewtonRaphson.jl:196-200, |H|_prev = 0 before the
+ * first step); switch_tol = INFINITY takes the Newton tangent from the second
+ * iteration on, as the reference (att > 1). final_change: that maximum. */
+int fk_nonlinear_spline(fk_mesh *mesh, const uint8_t *region, const double *t_mu, int32_t n_mu,
+                        const double *c_mu, const double *t_dmu, int32_t n_dmu, const double *c_dmu,
+                        double rtol, double tol_H, int32_t max_outer, double switch_tol, int32_t *outer,
+                        int64_t *cg_iterations, double *final_change);
 /* lineSearch(u, du, A, RHS, minStep) (src/FEM3D.jl:123-152) with A the last
  * assembled matrix (all rows): alpha <- 0.9 alpha until
  * ||RHS - A(u + alpha du)|| <= ||RHS - A u||; u (nv) is updated in place

@@ -289,24 +289,20 @@ def froehlich(Hn: np.ndarray, Ms: float, a: float):
     return MU0 * (1.0 + Ms / d), -MU0 * Ms / (d * d)
 
 
-def newton_froehlich(mesh, mask, g, magnet: np.ndarray, Ms: float, a: float,
-                     rtol: float = 1e-10, tol_mu: float = 1e-6, max_outer: int = 20,
-                     newton: bool = True, switch: float = 0.25):
-    """Picard/Newton mu-update loop of Magnetostatics/NewtonRaphson.jl:148-212
-    with Dirichlet data instead of the Neumann+Lagrange saddle system and the
-    correct H_z. Iteration 1 starts from mu = mu0 with no tangent (the
-    reference's att == 1 branch). An iteration adds the tangent
-    tangentialStiffnessMatrix (a Newton step) once the previous relative mu
-    change is below `switch`, otherwise it is a Picard step; far from the
-    solution undamped Newton oscillates for this law, which the reference
-    counters with lineSearch. Stops when max |mu_new - mu| / mu_new over magnet
-    elements < tol_mu. Returns dict(phi, H, Hn, mu, iters, cg_iters, dmu_hist,
-    steps)."""
+def _newton_loop(mesh, mask, g, magnet, law, deviation, tol, max_outer, newton, switch, rtol):
+    """The loop of Magnetostatics/NewtonRaphson.jl:148-212 with Dirichlet data
+    instead of the Neumann+Lagrange saddle system and the correct H_z.
+    law(Hn) -> (mu, dmu) on the magnet; deviation(mu_new, mu_old, Hn_new,
+    Hn_old) -> the scalar the loop stops on. Iteration 1 starts from mu = mu0
+    with no tangent (the reference's att == 1 branch); later iterations add
+    tangentialStiffnessMatrix (a Newton step) once the previous deviation is
+    below `switch`, otherwise they are Picard steps."""
     Ak = localStiffnessMatrix(mesh)
     mu = np.full(mesh.nt, MU0)
     dmu = np.zeros(mesh.nt)
     u = np.where(mask.astype(bool), g, 0.0)
     H = np.zeros((3, mesh.nt))
+    Hn = np.zeros(mesh.nt)
     hist = []
     steps = []
     cg_total = 0
@@ -327,20 +323,118 @@ def newton_froehlich(mesh, mask, g, magnet: np.ndarray, Ms: float, a: float,
             raise RuntimeError(f"oracle PCG status {s}")
         cg_total += cgit
         u = u + du
+        Hn_old = Hn
         H, Hn = field(mesh, u)
-        mu_new, dmu_new = froehlich(Hn, Ms, a)
-        mu_new = np.where(magnet, mu_new, MU0)
-        dmu = np.where(magnet, dmu_new, 0.0)
-        dev = float(np.max(np.abs(mu_new - mu)[magnet] / mu_new[magnet])) if magnet.any() else 0.0
+        mu_new, dmu_new = law(Hn[magnet])
+        mu_next = mu.copy()
+        mu_next[magnet] = mu_new
+        dmu = np.zeros(mesh.nt)
+        dmu[magnet] = dmu_new
+        dev = float(deviation(mu_next[magnet], mu[magnet], Hn[magnet], Hn_old[magnet])) if magnet.any() else 0.0
         hist.append(dev)
-        mu = mu_new
+        mu = mu_next
         if not (np.all(np.isfinite(mu)) and np.all(np.isfinite(dmu))):
             raise FloatingPointError("Nans/Infs in mu")  # NewtonRaphson.jl:210-212
-        if dev < tol_mu:
+        if dev < tol:
             break
     H, Hn = field(mesh, u)
-    return {"phi": u, "H": H, "Hn": Hn, "mu": mu, "iters": it, "cg_iters": cg_total, "dmu_hist": hist,
-            "steps": steps}
+    return {"phi": u, "H": H, "Hn": Hn, "mu": mu, "dmu": dmu, "iters": it, "cg_iters": cg_total,
+            "dmu_hist": hist, "steps": steps}
+
+
+def newton_froehlich(mesh, mask, g, magnet: np.ndarray, Ms: float, a: float,
+                     rtol: float = 1e-10, tol_mu: float = 1e-6, max_outer: int = 20,
+                     newton: bool = True, switch: float = 0.25):
+    """The loop with the Froehlich law (a workload law: no reference
+    counterpart); far from the solution undamped Newton oscillates for it,
+    which the reference counters with lineSearch, hence the Picard start.
+    Stops when max |mu_new - mu| / mu_new over magnet elements < tol_mu.
+    Returns dict(phi, H, Hn, mu, dmu, iters, cg_iters, dmu_hist, steps)."""
+    return _newton_loop(mesh, mask, g, magnet, lambda h: froehlich(h, Ms, a),
+                        lambda mn, mo, hn, ho: np.max(np.abs(mn - mo) / mn), tol_mu, max_outer, newton,
+                        switch, rtol)
+
+
+# --------------------------------------------------------------------------
+# Tabulated permeability (NewtonRaphson.jl:40-63, 204-207)
+def gradient1d(x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """src/FEM.jl:66-90: one-sided differences at the two ends, central
+    differences (y[i+1]-y[i-1])/(x[i+1]-x[i-1]) inside."""
+    x = np.asarray(x, float)
+    y = np.asarray(y, float)
+    if x.size != y.size or x.size < 2:
+        raise ValueError("gradient needs two equal-length arrays of at least 2 points")
+    z = np.empty_like(y)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        z[0] = (y[1] - y[0]) / (x[1] - x[0])
+        z[-1] = (y[-1] - y[-2]) / (x[-1] - x[-2])
+        z[1:-1] = (y[2:] - y[:-2]) / (x[2:] - x[:-2])
+    return z
+
+
+def material_permeability(HofM: np.ndarray, B: np.ndarray):
+    """src/magneticProperties.jl:163-188 (materialPermeability): mu = B/H with
+    the non-finite entries set to the largest finite-or-zeroed value, dmu =
+    gradient(H, mu) with the non-finite entries set to the smallest value.
+    Returns (mu, dmu)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        mu = np.asarray(B, float) / np.asarray(HofM, float)
+    bad = ~np.isfinite(mu)
+    mu[bad] = 0.0
+    mu[bad] = np.max(mu)
+    dmu = gradient1d(HofM, mu)
+    bad = ~np.isfinite(dmu)
+    dmu[bad] = 0.0
+    dmu[bad] = np.min(dmu)
+    return mu, dmu
+
+
+def bspline_eval(t: np.ndarray, c: np.ndarray, x: np.ndarray, k: int = 3) -> np.ndarray:
+    """Value of the spline sum_j c[j] B_{j,k}(x) with knots t (length n,
+    coefficients c[0:n-k-1]) -- what FITPACK's splev returns, and so what
+    Dierckx.jl's Spline1D evaluates (NewtonRaphson.jl:57-63, 204-207). The
+    basis comes from the de Boor-Cox recurrence B_{i,j} = w_{i,j} B_{i,j-1}
+    + (1 - w_{i+1,j}) B_{i+1,j-1}, w_{i,j}(x) = (x - t_i)/(t_{i+j} - t_i),
+    over the knot interval t[l] <= x < t[l+1] (l <= n-k-2, so the right end
+    t[n-k-1] belongs to the last interval). x outside [t[k], t[n-k-1]] raises
+    (Spline1D's bc = "error")."""
+    t = np.asarray(t, float)
+    c = np.asarray(c, float)
+    x = np.atleast_1d(np.asarray(x, float))
+    n = t.size
+    lo, hi = t[k], t[n - k - 1]
+    if np.any(~(x >= lo) | ~(x <= hi)):
+        raise ValueError("spline argument outside the table range")
+    out = np.empty_like(x)
+    for q, xv in enumerate(x):
+        l = int(np.searchsorted(t, xv, side="right")) - 1
+        l = min(max(l, k), n - k - 2)
+        # b[i] = B_{l-k+i, j} at xv; degree 0: only B_{l,0} = 1. Updating in
+        # increasing i reads b[i+1] before it is overwritten.
+        b = np.zeros(k + 2)
+        b[k] = 1.0
+        for j in range(1, k + 1):
+            for i in range(k - j, k + 1):
+                idx = l - k + i
+                d0 = t[idx + j] - t[idx]
+                d1 = t[idx + j + 1] - t[idx + 1]
+                left = (xv - t[idx]) / d0 * b[i] if d0 > 0.0 else 0.0
+                right = (t[idx + j + 1] - xv) / d1 * b[i + 1] if d1 > 0.0 else 0.0
+                b[i] = left + right
+        out[q] = float(np.dot(c[l - k:l + 1], b[:k + 1]))
+    return out
+
+
+def newton_spline(mesh, mask, g, magnet: np.ndarray, spl_mu, spl_dmu, rtol: float = 1e-10,
+                  tol_H: float = 1e-4, max_outer: int = 100, newton: bool = True, switch: float = np.inf):
+    """The loop with the tabulated law of NewtonRaphson.jl:204-207: mu and
+    dmu from two splines (each (t, c), FITPACK form, cubic) of |H| on the
+    magnet; stops when mu0 max |H_new - H_old| over magnet elements < tol_H
+    (NewtonRaphson.jl:200, H_old = 0 before the first step). switch = inf
+    takes Newton steps from iteration 2 on, as the reference (att > 1)."""
+    law = lambda h: (bspline_eval(spl_mu[0], spl_mu[1], h), bspline_eval(spl_dmu[0], spl_dmu[1], h))
+    return _newton_loop(mesh, mask, g, magnet, law, lambda mn, mo, hn, ho: MU0 * np.max(np.abs(hn - ho)),
+                        tol_H, max_outer, newton, switch, rtol)
 
 
 # --------------------------------------------------------------------------

@@ -401,3 +401,108 @@ def test_renumbered_problem_is_the_same_problem(oracle):
     assert s == 0 and s2 == 0
     assert np.abs(phi2 - phi[perm]).max() < 1e-10 * np.abs(phi).max()
     assert np.array_equal(pr2.mesh.VE, pr.mesh.VE)
+
+
+# ---------------------------------------------------------------------------
+# Tabulated mu(|H|): NewtonRaphson.jl:57-63, 204-207; magneticProperties.jl:163-188
+GD270 = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "gd270_bh.npz")
+
+
+def _gd_splines(oracle):
+    from scipy.interpolate import splrep
+    tab = np.load(GD270)
+    mu, dmu = oracle.material_permeability(tab["H"], tab["B"])
+    t, c, _ = splrep(tab["H"], mu, k=3, s=0)
+    td, cd, _ = splrep(tab["H"], dmu, k=3, s=0)
+    return (t, c[:t.size - 4]), (td, cd[:td.size - 4]), tab
+
+
+def test_gradient1d_known_answer(oracle):
+    z = oracle.gradient1d(np.array([0.0, 1.0, 3.0, 6.0]), np.array([0.0, 1.0, 9.0, 36.0]))
+    np.testing.assert_array_equal(z, [1.0, 3.0, 7.0, 9.0])
+
+
+def test_material_permeability_known_answer(oracle):
+    # mu = B/H: 0/... at H = 0 is non-finite -> zeroed, then set to the maximum
+    mu, dmu = oracle.material_permeability(np.array([0.0, 1.0, 2.0, 4.0]), np.array([0.5, 1.0, 1.5, 2.0]))
+    np.testing.assert_array_equal(mu, [1.0, 1.0, 0.75, 0.5])
+    np.testing.assert_allclose(dmu, [0.0, -0.125, -1.0 / 6.0, -0.125], rtol=0, atol=1e-15)
+    # H[0] == H[2] makes the central difference at 1 infinite -> zeroed, then
+    # set to the minimum of the rest
+    mu, dmu = oracle.material_permeability(np.array([1.0, 2.0, 1.0, 4.0]), np.array([1.0, 3.0, 2.0, 4.0]))
+    np.testing.assert_array_equal(mu, [1.0, 1.5, 2.0, 1.0])
+    np.testing.assert_allclose(dmu, [0.5, -1.0 / 3.0, -0.25, -1.0 / 3.0], rtol=1e-15)
+
+
+def test_bspline_eval_matches_fitpack(oracle):
+    from scipy.interpolate import splev
+    (t, c), (td, cd), tab = _gd_splines(oracle)
+    H = tab["H"]
+    xs = np.concatenate([H, np.linspace(H[0], H[-1], 2001), t[3:-3]])
+    for (tt, cc) in ((t, c), (td, cd)):
+        ref = splev(xs, (tt, np.concatenate([cc, np.zeros(4)]), 3))
+        got = oracle.bspline_eval(tt, cc, xs)
+        assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+    # the interpolant passes through the table
+    mu, _ = oracle.material_permeability(tab["H"], tab["B"])
+    np.testing.assert_allclose(oracle.bspline_eval(t, c, H), mu, rtol=1e-12)
+    with pytest.raises(ValueError):
+        oracle.bspline_eval(t, c, [H[-1] * 1.0001])
+    with pytest.raises(ValueError):
+        oracle.bspline_eval(t, c, [-1.0])
+
+
+def test_product_material_matches_oracle(oracle):
+    from femeko_hip import material
+    tab = np.load(GD270)
+    mu, dmu, nu = material.material_permeability(tab["H"], tab["B"])
+    mu_o, dmu_o = oracle.material_permeability(tab["H"], tab["B"])
+    np.testing.assert_array_equal(mu, mu_o)
+    np.testing.assert_array_equal(dmu, dmu_o)
+    np.testing.assert_allclose(nu[1:], MU0 * tab["H"][1:] / tab["B"][1:])
+    spl = material.Spline1D(tab["H"], mu)
+    (t, c), _, _ = _gd_splines(oracle)
+    np.testing.assert_array_equal(spl.t, t)
+    np.testing.assert_array_equal(spl.c, c)
+    np.testing.assert_allclose(spl(tab["H"][5:9]), mu[5:9], rtol=1e-12)
+    with pytest.raises(ValueError):
+        spl(tab["H"][-1] * 2)
+
+
+def test_newton_spline_gd_converges_quadratically(oracle):
+    (sm, sd, _tab) = _gd_splines(oracle)
+    pr = W.tabulated(n=6, seed=21)
+    out = oracle.newton_spline(pr.mesh, pr.mask, pr.g, pr.magnet, sm, sd, tol_H=1e-8, max_outer=30)
+    h = out["dmu_hist"]
+    assert out["iters"] < 12 and h[-1] < 1e-8 and out["steps"][0] == "P" and set(out["steps"][1:]) == {"N"}
+    # superlinear tail (Newton): the contraction ratio itself shrinks tenfold
+    # per step over the last three steps
+    r = [h[i + 1] / h[i] for i in range(len(h) - 4, len(h) - 1)]
+    assert r[1] < r[0] / 10 and r[2] < r[1] / 10
+    # Picard steps only: linear convergence, many more iterations
+    pic = oracle.newton_spline(pr.mesh, pr.mask, pr.g, pr.magnet, sm, sd, tol_H=1e-8, max_outer=60, newton=False)
+    assert pic["iters"] > 2 * out["iters"]
+    # fixed point: mu = spl(|H|) on the magnet and A(mu) phi = 0 on free rows
+    np.testing.assert_allclose(out["mu"][pr.magnet], oracle.bspline_eval(*sm, out["Hn"][pr.magnet]), rtol=1e-6)
+    A = oracle.stiffnessMatrix(pr.mesh, out["mu"]).to_scipy()
+    r = (A @ out["phi"])[pr.mask == 0]
+    assert np.linalg.norm(r) / np.linalg.norm((A @ pr.g)[pr.mask == 0]) < 1e-8
+
+
+def test_newton_spline_reproduces_froehlich_loop(oracle):
+    # a densely tabulated Froehlich law through the spline loop lands on the
+    # Froehlich loop's fixed point (to the interpolation error)
+    from scipy.interpolate import splrep
+    Ms, a = 1.2e6, 500.0
+    Ht = np.concatenate([[0.0], np.geomspace(1.0, 1e6, 600)])
+    mu_t, dmu_t = oracle.froehlich(Ht, Ms, a)
+    t, c, _ = splrep(Ht, mu_t, k=3, s=0)
+    td, cd, _ = splrep(Ht, dmu_t, k=3, s=0)
+    pr = W.froehlich(n=6, seed=12)
+    ref = oracle.newton_froehlich(pr.mesh, pr.mask, pr.g, pr.magnet, Ms, a, tol_mu=1e-10, max_outer=40)
+    # Newton steps once mu0 |dH| < 1e-5 T (undamped Newton from the start
+    # oscillates for this law)
+    out = oracle.newton_spline(pr.mesh, pr.mask, pr.g, pr.magnet, (t, c[:t.size - 4]), (td, cd[:td.size - 4]),
+                               tol_H=1e-12, max_outer=60, switch=1e-5)
+    assert np.linalg.norm(out["phi"] - ref["phi"]) / np.linalg.norm(ref["phi"]) < 1e-7
+    np.testing.assert_allclose(out["mu"], ref["mu"], rtol=1e-6)
