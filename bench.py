@@ -209,6 +209,43 @@ def bench_froehlich(args):
     return res
 
 
+def bench_tabulated(args):
+    """n=100 gadolinium cube (B(H) table at 270 K, the reference's material)
+    under 0.5 T, mu(|H|) loop with the tabulated law, FP64 PCG to 1e-10,
+    stop at mu0 max| |H| - |H|_prev | < 1e-4 T (NewtonRaphson.jl:36). Picard
+    steps until that change is below 1e-2 T, Newton steps after: undamped
+    Newton from the second step (the reference damps it with lineSearch)
+    drives a few cube-corner elements, where the field is singular, past the
+    table's end at this resolution."""
+    from femeko_hip import workloads as W
+    from femeko_hip.device import DeviceMesh, context
+    from femeko_hip.material import Spline1D, material_permeability
+    tab = np.load(os.path.join(ROOT, "tests", "golden", "gd270_bh.npz"))
+    mu_t, dmu_t, _nu = material_permeability(tab["H"], tab["B"])
+    spl, spl_dmu = Spline1D(tab["H"], mu_t), Spline1D(tab["H"], dmu_t)
+    pr = W.tabulated()
+    dm = DeviceMesh(pr.mesh, context(0))
+    dm.set_dirichlet(pr.mask, pr.g)
+    reg = pr.magnet.astype(np.uint8)
+    for _ in range(args.warmup):
+        out = dm.nonlinear_spline(reg, spl, spl_dmu, rtol=1e-10, tol_H=1e-4, switch_tol=1e-2)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = dm.nonlinear_spline(reg, spl, spl_dmu, rtol=1e-10, tol_H=1e-4, switch_tol=1e-2)
+    t = (time.perf_counter() - t0) / args.steps
+    log(f"tabulated: {t*1e3:.1f} ms, outer {out['outer']}, CG {out['cg_iterations']}, change {out['change']:.2e}")
+    res = _common(args, "nonlinear magnetostatics DOF/s (tabulated mu(|H|) Newton loop, FP64)", pr.mesh.nv / t,
+                  "DOF/s", t * 1e3, "f64",
+                  {"workload": "tabulated100: Kuhn n=100, Gd B(H) at 270 K (62-point table, cubic splines of mu "
+                               "and dmu) in [-0.25,0.25]^3, Bext = 0.5 T along x; Newton once the change "
+                               "is below 1e-2 T", "nodes": pr.mesh.nv,
+                   "tets": pr.mesh.nt})
+    res.update({"outer_iterations": out["outer"], "cg_iterations": out["cg_iterations"],
+                "final_H_change_T": out["change"]})
+    dm.close()
+    return res
+
+
 def bench_adjoint(args):
     """n=80 adjoint gradient of the field-target objective; central differences
     (step 1e-6) on 32 seeded elements."""
@@ -278,7 +315,7 @@ def main():
                     help="renumber the nodes (reverse Cuthill-McKee or random) to measure a general, "
                          "non-stencil numbering of the same mesh")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="boxes", choices=["boxes", "batch", "froehlich", "adjoint"],
+    ap.add_argument("--workload", default="boxes", choices=["boxes", "batch", "froehlich", "tabulated", "adjoint"],
                     help="boxes: the headline n=160 mixed-precision line (default); the others measure the "
                          "batched, nonlinear and adjoint configurations")
     args = ap.parse_args()
@@ -287,7 +324,8 @@ def main():
               f"note only)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "boxes":
-        out = {"batch": bench_batch, "froehlich": bench_froehlich, "adjoint": bench_adjoint}[args.workload](args)
+        out = {"batch": bench_batch, "froehlich": bench_froehlich, "tabulated": bench_tabulated,
+               "adjoint": bench_adjoint}[args.workload](args)
         print(json.dumps(out))
         return
 

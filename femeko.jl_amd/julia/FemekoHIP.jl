@@ -27,7 +27,7 @@ module FemekoHIP
 using SparseArrays
 
 export stiffnessMatrix, localStiffnessMatrix, tangentialStiffnessMatrix, lagrange,
-       solveDirichlet, magneticField, fieldObjective
+       solveDirichlet, magneticField, fieldObjective, newtonSpline
 
 const LIB = joinpath(@__DIR__, "..", "lib", "libfemeko_hip.so")
 
@@ -155,6 +155,38 @@ function solveDirichlet(mesh, mu::Vector{Float64}, fixed::AbstractVector{Bool}, 
     u = Vector{Float64}(undef, mesh.nv)
     check(dm.ctx, ccall((:fk_potential, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), dm.handle, u), "fk_potential")
     return u, (iterations = iters[], outer = outer[], relres = relres[])
+end
+
+# The mu(|H|) loop of Magnetostatics/NewtonRaphson.jl:148-212 on the GPU with
+# the reference's own splines (spl = Spline1D(HofM, mu), spl_dmu =
+# Spline1D(HofM, dmu), Dierckx.jl: fields t and c are FITPACK's knots and
+# coefficients) on the elements listed in mesh.InsideElements, Dirichlet data
+# (fixed, g). Returns (u, mu, (outer, cg_iterations, change)).
+function newtonSpline(mesh, spl, spl_dmu, fixed::AbstractVector{Bool}, g::Vector{Float64};
+                      rtol = 1e-10, maxDeviation = 1e-4, maxAtt = 100, switch_tol = Inf)
+    dm = device_mesh(mesh)
+    region = zeros(UInt8, mesh.nt)
+    region[mesh.InsideElements] .= 0x01
+    mask = UInt8.(fixed)
+    check(dm.ctx, ccall((:fk_set_dirichlet, LIB), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Ptr{Float64}), dm.handle, mask, g),
+          "fk_set_dirichlet")
+    nc_mu = length(spl.t) - 4
+    nc_dmu = length(spl_dmu.t) - 4
+    outer = Ref{Int32}(0)
+    cg = Ref{Int64}(0)
+    change = Ref{Float64}(0.0)
+    check(dm.ctx, ccall((:fk_nonlinear_spline, LIB), Cint,
+                        (Ptr{Cvoid}, Ptr{UInt8}, Ptr{Float64}, Int32, Ptr{Float64}, Ptr{Float64}, Int32,
+                         Ptr{Float64}, Float64, Float64, Int32, Float64, Ref{Int32}, Ref{Int64}, Ref{Float64}),
+                        dm.handle, region, spl.t, Int32(length(spl.t)), spl.c[1:nc_mu], spl_dmu.t,
+                        Int32(length(spl_dmu.t)), spl_dmu.c[1:nc_dmu], rtol, maxDeviation, Int32(maxAtt),
+                        switch_tol, outer, cg, change), "fk_nonlinear_spline")
+    u = Vector{Float64}(undef, mesh.nv)
+    check(dm.ctx, ccall((:fk_potential, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), dm.handle, u), "fk_potential")
+    mu = Vector{Float64}(undef, mesh.nt)
+    check(dm.ctx, ccall((:fk_get_coefficient, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), dm.handle, mu),
+          "fk_get_coefficient")
+    return u, mu, (outer = outer[], cg_iterations = cg[], change = change[])
 end
 
 # H (3 x nt) = -grad(u) per element
