@@ -283,6 +283,13 @@ inline void values_changed(fk_mesh *m) {
     m->sell32_valid = false;
 }
 CgState run_pcg_f64(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref);
+struct NeumannResult {
+    double multiplier;
+    int64_t iters;
+    double relres;
+};
+// the projected Neumann solve of b in w64.r (device); u returned in w64.x
+NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter);
 void phi_axpy_f64(fk_mesh *m, const double *x, double s);
 void k_init_phi_dev(fk_mesh *m, int keep_free);
 

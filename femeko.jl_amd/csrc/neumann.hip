@@ -134,12 +134,12 @@ static void sums3(fk_mesh *m, const double *a, const double *c, double out[3]) {
     }
 }
 
-void solve_neumann(fk_mesh *m, const double *b_host, double rtol, int64_t max_iter, double *u_host,
-                   double *mult_out, int64_t *iters_out, double *relres_out) {
+// The saddle solve with b already in w64.r on the device; leaves u (Lag' u =
+// 0) in w64.x.
+NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter) {
     if (!m->have_values) fail(FK_ERR_STATE, "matrix not assembled");
     hipStream_t st = m->ctx->stream;
     const int64_t nv = m->nv;
-    ensure_work_f64(m);
     PcgWork<double> &w = m->w64;
     // every node free
     {
@@ -163,7 +163,6 @@ void solve_neumann(fk_mesh *m, const double *b_host, double rtol, int64_t max_it
                                                              lag.get(), e0, e1);
     }
     FK_HIP(hipGetLastError());
-    FK_HIP(hipMemcpyAsync(w.r.get(), b_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
     double s[3];
     sums3(m, w.r.get(), lag.get(), s);  // sum b, sum Lag
     const double mult = s[1] != 0.0 ? s[0] / s[1] : 0.0;
@@ -176,11 +175,21 @@ void solve_neumann(fk_mesh *m, const double *b_host, double rtol, int64_t max_it
     sums3(m, w.x.get(), lag.get(), s);
     k_shift<<<grid_for(nv, 256), 256, 0, st>>>(w.x.get(), s[2] / s[1], nv);
     FK_HIP(hipGetLastError());
-    FK_HIP(hipMemcpyAsync(u_host, w.x.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    return NeumannResult{mult, cs.iters, cs.bb > 0.0 ? std::sqrt(cs.rr / cs.bb) : 0.0};
+}
+
+void solve_neumann(fk_mesh *m, const double *b_host, double rtol, int64_t max_iter, double *u_host,
+                   double *mult_out, int64_t *iters_out, double *relres_out) {
+    if (!m->have_values) fail(FK_ERR_STATE, "matrix not assembled");
+    hipStream_t st = m->ctx->stream;
+    ensure_work_f64(m);
+    FK_HIP(hipMemcpyAsync(m->w64.r.get(), b_host, m->nv * sizeof(double), hipMemcpyHostToDevice, st));
+    const NeumannResult res = solve_neumann_dev(m, rtol, max_iter);
+    FK_HIP(hipMemcpyAsync(u_host, m->w64.x.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
-    if (mult_out) *mult_out = mult;
-    if (iters_out) *iters_out = cs.iters;
-    if (relres_out) *relres_out = cs.bb > 0.0 ? std::sqrt(cs.rr / cs.bb) : 0.0;
+    if (mult_out) *mult_out = res.multiplier;
+    if (iters_out) *iters_out = res.iters;
+    if (relres_out) *relres_out = res.relres;
 }
 
 }  // namespace fk

@@ -181,6 +181,19 @@ int fk_boundary_integral(fk_mesh *mesh, int64_t ns, const int32_t *surfaceT, con
 int fk_solve_neumann(fk_mesh *mesh, const double *b, double rtol, int64_t max_iter, double *u,
                      double *multiplier, int64_t *iterations, double *relres);
 
+/* Demagnetizing field of Micromagnetics/LL.jl:162-209 (magnetostaticField),
+ * called once per Landau-Lifshitz step with the matrix fixed: with the
+ * assembled A (stiffnessMatrix(mesh), permalloy.jl:99, no fixed node), the
+ * load RHS_i = sum over elements k with inside[k] != 0 (caller order) of
+ * VE_k grad(lambda_i) . mean(M over the 4 nodes of k), u from the projected
+ * Neumann Jacobi-PCG from zero to rtol relative to ||RHS|| (the reference's
+ * cg(A, RHS) defaults to sqrt(eps)), H_k = -grad(u), and Hd_nodes[:, i] =
+ * sum over inside elements k at node i of VE_k H_k / volumes[i] (NaN where
+ * volumes[i] == 0, as the reference's 0/0). M, Hd_nodes: 3 x nv column-major
+ * (M[3 i + c]); volumes: nv (the driver's per-node sums of inside VE). */
+int fk_demag_field(fk_mesh *mesh, const uint8_t *inside, const double *M, const double *volumes, double rtol,
+                   int64_t max_iter, double *Hd_nodes, int64_t *iterations, double *relres);
+
 /* ---- nonlinear loop and line search ----------------------------------------
  * Picard/Newton mu(|H|) loop of Magnetostatics/NewtonRaphson.jl:148-212 with
  * the Froehlich law mu = mu0 (1 + Ms/(a+|H|)) on elements with region != 0

@@ -495,3 +495,50 @@ def saddle_solve(A: CSC, Lag: np.ndarray, b: np.ndarray):
     K = sp.bmat([[S, L], [L.T, None]], format="csc")
     sol = sla.spsolve(K, np.concatenate([b, [0.0]]))
     return sol[:-1], sol[-1]
+
+
+# --------------------------------------------------------------------------
+# Micromagnetics: demagnetizing field (Micromagnetics/LL.jl:162-209)
+def node_volumes(mesh, inside: np.ndarray) -> np.ndarray:
+    """Micromagnetics/permalloy.jl:92-96: sum of VE over the inside elements
+    at each node."""
+    vol = np.zeros(mesh.nv)
+    t0 = np.asarray(mesh.t, dtype=np.int64) - 1
+    for k in np.nonzero(inside)[0]:
+        vol[t0[:, k]] += mesh.VE[k]
+    return vol
+
+
+def demag_load(mesh, inside: np.ndarray, M: np.ndarray) -> np.ndarray:
+    """The load vector of magnetostaticField (LL.jl:170-183): over inside
+    elements k, RHS[nds[i]] += VE_k (b_ik, c_ik, d_ik) . mean(M[:, nds])."""
+    G = element_gradients(mesh)
+    t0 = np.asarray(mesh.t, dtype=np.int64) - 1
+    rhs = np.zeros(mesh.nv)
+    for k in np.nonzero(inside)[0]:
+        nds = t0[:, k]
+        f = M[:, nds].sum(axis=1) / 4.0
+        for i in range(4):
+            rhs[nds[i]] += mesh.VE[k] * float(G[k, i] @ f)
+    return rhs
+
+
+def magnetostatic_field(mesh, inside: np.ndarray, M: np.ndarray, volumes: np.ndarray):
+    """magnetostaticField (LL.jl:162-209) with A = stiffnessMatrix(mesh)
+    (permalloy.jl:99). The reference's cg(A, RHS) converges on the consistent
+    singular system to a solution defined up to a constant; here u comes from
+    the direct bordered solve with Lag' u = 0, which has the same gradient.
+    Returns (Hd_nodes (3, nv), u, RHS); nodes with volumes == 0 hold NaN
+    (0/0, as the reference)."""
+    inside = np.asarray(inside).astype(bool)
+    A = stiffnessMatrix(mesh)
+    rhs = demag_load(mesh, inside, M)
+    u, _mult = saddle_solve(A, lagrange(mesh), rhs)
+    H, _Hn = field(mesh, u)
+    t0 = np.asarray(mesh.t, dtype=np.int64) - 1
+    Hd = np.zeros((3, mesh.nv))
+    for k in np.nonzero(inside)[0]:
+        Hd[:, t0[:, k]] += mesh.VE[k] * H[:, k:k + 1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        Hd = Hd / volumes
+    return Hd, u, rhs

@@ -506,3 +506,38 @@ def test_newton_spline_reproduces_froehlich_loop(oracle):
                                tol_H=1e-12, max_outer=60, switch=1e-5)
     assert np.linalg.norm(out["phi"] - ref["phi"]) / np.linalg.norm(ref["phi"]) < 1e-7
     np.testing.assert_allclose(out["mu"], ref["mu"], rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# Demagnetizing field (Micromagnetics/LL.jl:162-209)
+def test_demag_uniform_m_everywhere_is_exact(oracle):
+    # uniform M over the whole mesh: u = M . x solves A u = RHS exactly (a P1
+    # function), so H = -M on every element and every node
+    pr = W.linear(5)
+    m = pr.mesh
+    inside = np.ones(m.nt, bool)
+    Mvec = np.array([0.3, -1.2, 0.7])
+    M = np.repeat(Mvec[:, None], m.nv, axis=1)
+    vol = oracle.node_volumes(m, inside)
+    Hd, u, rhs = oracle.magnetostatic_field(m, inside, M, vol)
+    assert abs(rhs.sum()) < 1e-12 * np.abs(rhs).sum()
+    np.testing.assert_allclose(Hd, -np.repeat(Mvec[:, None], m.nv, axis=1), rtol=0, atol=1e-10)
+    lin = Mvec @ m.p
+    np.testing.assert_allclose(u - u.mean(), lin - lin.mean(), rtol=0, atol=1e-10)
+
+
+def test_demag_energy_identity_and_nan_outside(oracle):
+    # partial magnet, random M: RHS . u = u' A u (the load and the operator
+    # agree in sign and scale), and the reference's 0/0 outside the magnet
+    pr = W.linear(6)
+    m = pr.mesh
+    inside = pr.magnet
+    rng = np.random.default_rng(4)
+    M = rng.normal(size=(3, m.nv))
+    vol = oracle.node_volumes(m, inside)
+    Hd, u, rhs = oracle.magnetostatic_field(m, inside, M, vol)
+    A = oracle.stiffnessMatrix(m).to_scipy()
+    assert rhs @ u == pytest.approx(u @ (A @ u), rel=1e-10)
+    out = vol == 0
+    assert out.any() and np.all(np.isnan(Hd[:, out])) and np.all(np.isfinite(Hd[:, ~out]))
+    assert abs(rhs.sum()) < 1e-12 * np.abs(rhs).sum()
