@@ -246,6 +246,58 @@ def bench_tabulated(args):
     return res
 
 
+def bench_demag(args):
+    """Repeated-RHS Poisson solves of the micromagnetics driver
+    (Micromagnetics/LL.jl:162-209): n=100 Kuhn box, magnet [-0.25,0.25]^3
+    with a unit magnetization that rotates a little each step, A =
+    stiffnessMatrix(mesh) resident, one fk_demag_field per step at the
+    reference cg's tolerance sqrt(eps). A step = load vector + FP64 PCG +
+    nodal field, host copies of M and Hd included."""
+    from femeko_hip import workloads as W
+    from femeko_hip.device import DeviceMesh, context
+    pr = W.linear(100, seed=102)
+    m = pr.mesh
+    inside = pr.magnet
+    t0i = np.asarray(m.t, dtype=np.int64) - 1
+    vol = np.zeros(m.nv)
+    np.add.at(vol, t0i[:, inside].ravel(), np.repeat(m.VE[inside][None, :], 4, axis=0).ravel())
+    dm = DeviceMesh(m, context(0))
+    dm.set_coefficient(np.ones(m.nt))
+    dm.assemble()
+    ins = inside.astype(np.uint8)
+    innodes = vol > 0
+
+    def M_at(k):
+        th = 0.05 * k
+        M = np.zeros((3, m.nv))
+        M[0, innodes] = np.cos(th)
+        M[1, innodes] = np.sin(th)
+        return M
+
+    def run(warm):
+        for k in range(args.warmup):
+            dm.demag_field(ins, M_at(k), vol, warm=warm)
+        its = []
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            _Hd, it, _rel = dm.demag_field(ins, M_at(args.warmup + k), vol, warm=warm)
+            its.append(it)
+        return (time.perf_counter() - t0) / args.steps, its
+
+    t_cold, it_cold = run(False)
+    t, iters = run(True)
+    log(f"demag: {t*1e3:.1f} ms per step starting from the previous potential (PCG iterations {iters}); "
+        f"{t_cold*1e3:.1f} ms from zero as the reference's cg ({it_cold})")
+    res = _common(args, "demagnetizing-field solves per second (fixed matrix, new right-hand side each step, FP64)",
+                  1.0 / t, "solves/s", t * 1e3, "f64",
+                  {"workload": "demag100: Kuhn n=100, magnet [-0.25,0.25]^3, unit M rotating 0.05 rad per step, "
+                               "rtol sqrt(eps), PCG started from the previous step's potential", "nodes": m.nv,
+                   "tets": m.nt})
+    res.update({"pcg_iterations": iters, "from_zero": {"ms_per_step": t_cold * 1e3, "pcg_iterations": it_cold}})
+    dm.close()
+    return res
+
+
 def bench_adjoint(args):
     """n=80 adjoint gradient of the field-target objective; central differences
     (step 1e-6) on 32 seeded elements."""
@@ -315,7 +367,8 @@ def main():
                     help="renumber the nodes (reverse Cuthill-McKee or random) to measure a general, "
                          "non-stencil numbering of the same mesh")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="boxes", choices=["boxes", "batch", "froehlich", "tabulated", "adjoint"],
+    ap.add_argument("--workload", default="boxes",
+                    choices=["boxes", "batch", "froehlich", "tabulated", "demag", "adjoint"],
                     help="boxes: the headline n=160 mixed-precision line (default); the others measure the "
                          "batched, nonlinear and adjoint configurations")
     args = ap.parse_args()
@@ -325,7 +378,7 @@ def main():
         sys.exit(2)
     if args.workload != "boxes":
         out = {"batch": bench_batch, "froehlich": bench_froehlich, "tabulated": bench_tabulated,
-               "adjoint": bench_adjoint}[args.workload](args)
+               "demag": bench_demag, "adjoint": bench_adjoint}[args.workload](args)
         print(json.dumps(out))
         return
 

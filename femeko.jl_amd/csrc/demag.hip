@@ -12,7 +12,10 @@
 //   k_div_volumes: division by the caller's nodal volumes (0/0 = NaN on
 //       nodes outside the magnet, as the reference).
 // The matrix, its solver copy and the Jacobi diagonal stay resident across
-// calls; each call moves M in and the nodal field out.
+// calls; each call moves M in and the nodal field out. With warm != 0 the
+// PCG starts from the previous call's potential (kept on the device): the
+// field of the next time step is close to the last one's, and the stopping
+// test stays relative to ||RHS||, as for the start from zero.
 #include <cmath>
 
 #include "device.h"
@@ -74,7 +77,7 @@ __global__ void k_div_volumes(double *__restrict__ Hd, const double *__restrict_
 }
 
 void demag_field(fk_mesh *m, const uint8_t *inside_host, const double *M_host, const double *vol_host, double rtol,
-                 int64_t max_iter, double *Hd_host, int64_t *iters_out, double *relres_out) {
+                 int64_t max_iter, int warm, double *Hd_host, int64_t *iters_out, double *relres_out) {
     if (!m->have_values) fail(FK_ERR_STATE, "matrix not assembled (A = stiffnessMatrix(mesh))");
     const int64_t nv = m->nv, nt = m->nt;
     for (int64_t i = 0; i < 3 * nv; ++i)
@@ -98,7 +101,11 @@ void demag_field(fk_mesh *m, const uint8_t *inside_host, const double *M_host, c
                                                              inside.get(), M.get(), rhs, e0, e1);
     }
     FK_HIP(hipGetLastError());
-    const NeumannResult res = solve_neumann_dev(m, rtol, max_iter);
+    const bool from_prev = warm && m->have_demag_u;
+    const NeumannResult res = solve_neumann_dev(m, rtol, max_iter, from_prev ? m->demag_u.get() : nullptr);
+    m->demag_u.ensure(nv);
+    FK_HIP(hipMemcpyAsync(m->demag_u.get(), m->w64.x.get(), nv * sizeof(double), hipMemcpyDeviceToDevice, st));
+    m->have_demag_u = true;
     FK_HIP(hipMemsetAsync(Hd.get(), 0, 3 * nv * sizeof(double), st));
     for (int c = 0; c < m->ncolours; ++c) {
         const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
@@ -119,11 +126,11 @@ void demag_field(fk_mesh *m, const uint8_t *inside_host, const double *M_host, c
 extern "C" {
 
 int fk_demag_field(fk_mesh *m, const uint8_t *inside, const double *M, const double *volumes, double rtol,
-                   int64_t max_iter, double *Hd_nodes, int64_t *iterations, double *relres) {
+                   int64_t max_iter, int32_t warm, double *Hd_nodes, int64_t *iterations, double *relres) {
     if (!m || !inside || !M || !volumes || !Hd_nodes || !(rtol > 0.0) || max_iter < 1) return FK_ERR_INVALID_ARG;
     return fk::guard(m->ctx, "fk_demag_field", [&] {
         fk::set_device(m->ctx);
-        fk::demag_field(m, inside, M, volumes, rtol, max_iter, Hd_nodes, iterations, relres);
+        fk::demag_field(m, inside, M, volumes, rtol, max_iter, warm, Hd_nodes, iterations, relres);
     });
 }
 

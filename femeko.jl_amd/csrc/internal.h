@@ -181,6 +181,8 @@ struct fk_mesh {
     bool have_coef = false;
     fk::DevBuf<double> dmu;           // d mu / d|H| per element (colour order), nonlinear loop
     fk::DevBuf<double> hn_prev;       // |H| of the previous step (colour order), tabulated-law loop
+    fk::DevBuf<double> demag_u;       // potential of the last demagnetizing-field call
+    bool have_demag_u = false;
     fk::DevBuf<uint8_t> region;       // nonlinear material region (colour order)
     fk::DevBuf<uint8_t> target;       // objective region of the adjoint (colour order)
     fk::DevBuf<double> val64;
@@ -288,8 +290,9 @@ struct NeumannResult {
     int64_t iters;
     double relres;
 };
-// the projected Neumann solve of b in w64.r (device); u returned in w64.x
-NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter);
+// the projected Neumann solve of b in w64.r (device); u returned in w64.x;
+// u0: start vector (device) or nullptr for a start from zero
+NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter, const double *u0);
 void phi_axpy_f64(fk_mesh *m, const double *x, double s);
 void k_init_phi_dev(fk_mesh *m, int keep_free);
 

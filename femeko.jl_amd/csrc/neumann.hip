@@ -73,6 +73,27 @@ __global__ void k_axpy_inplace(double *__restrict__ y, const double *__restrict_
     if (i < n) y[i] += a * x[i];
 }
 
+// r -= A u over the FP64 CSR values (one row per thread group of 16 lanes)
+__global__ __launch_bounds__(kBlock) void k_sub_spmv_csr(const int32_t *__restrict__ rowptr,
+                                                         const int32_t *__restrict__ col,
+                                                         const double *__restrict__ val,
+                                                         const double *__restrict__ u, double *__restrict__ r,
+                                                         int64_t n) {
+    constexpr int L = 16, RPB = kBlock / L;
+    const int sub = threadIdx.x / L, lane = threadIdx.x % L;
+    for (int64_t rb = static_cast<int64_t>(blockIdx.x) * RPB; rb < n; rb += static_cast<int64_t>(gridDim.x) * RPB) {
+        const int64_t row = rb + sub;
+        double s = 0.0;
+        if (row < n) {
+            const int32_t a0 = rowptr[row], a1 = rowptr[row + 1];
+            for (int32_t k = a0 + lane; k < a1; k += L) s += val[k] * u[col[k]];
+        }
+#pragma unroll
+        for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (row < n && lane == 0) r[row] -= s;
+    }
+}
+
 __global__ void k_shift(double *__restrict__ y, double c, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) y[i] -= c;
@@ -135,8 +156,10 @@ static void sums3(fk_mesh *m, const double *a, const double *c, double out[3]) {
 }
 
 // The saddle solve with b already in w64.r on the device; leaves u (Lag' u =
-// 0) in w64.x.
-NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter) {
+// 0) in w64.x. With u0 (device, nv) the PCG starts from u0: it solves for
+// the correction of the residual b' - A u0 and still stops at rtol relative
+// to ||b'||, the test of a start from zero.
+NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter, const double *u0) {
     if (!m->have_values) fail(FK_ERR_STATE, "matrix not assembled");
     hipStream_t st = m->ctx->stream;
     const int64_t nv = m->nv;
@@ -169,8 +192,20 @@ NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter) {
     // b' = b - m Lag, consistent with the null space of A
     k_axpy_inplace<<<grid_for(nv, 256), 256, 0, st>>>(w.r.get(), lag.get(), -mult, nv);
     FK_HIP(hipGetLastError());
-    const CgState cs = run_pcg_f64(m, rtol, max_iter, -1.0);
+    double bb_ref = -1.0;
+    if (u0) {
+        sums3(m, w.r.get(), w.r.get(), s);
+        bb_ref = s[2];
+        k_sub_spmv_csr<<<std::max(1, m->nblocks), kBlock, 0, st>>>(m->rowptr.get(), m->colidx.get(),
+                                                                    m->val64.get(), u0, w.r.get(), nv);
+        FK_HIP(hipGetLastError());
+    }
+    const CgState cs = run_pcg_f64(m, rtol, max_iter, bb_ref);
     if (cs.status != FK_OK) fail(cs.status, std::string("Neumann solve: ") + fk_status_string(cs.status));
+    if (u0) {
+        k_axpy_inplace<<<grid_for(nv, 256), 256, 0, st>>>(w.x.get(), u0, 1.0, nv);
+        FK_HIP(hipGetLastError());
+    }
     // project: Lag' u = 0
     sums3(m, w.x.get(), lag.get(), s);
     k_shift<<<grid_for(nv, 256), 256, 0, st>>>(w.x.get(), s[2] / s[1], nv);
@@ -184,7 +219,7 @@ void solve_neumann(fk_mesh *m, const double *b_host, double rtol, int64_t max_it
     hipStream_t st = m->ctx->stream;
     ensure_work_f64(m);
     FK_HIP(hipMemcpyAsync(m->w64.r.get(), b_host, m->nv * sizeof(double), hipMemcpyHostToDevice, st));
-    const NeumannResult res = solve_neumann_dev(m, rtol, max_iter);
+    const NeumannResult res = solve_neumann_dev(m, rtol, max_iter, nullptr);
     FK_HIP(hipMemcpyAsync(u_host, m->w64.x.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
     if (mult_out) *mult_out = res.multiplier;

@@ -82,7 +82,7 @@ SIGNATURES = {
                                           _c.c_int32, _c.c_double, _ip, _i64p, _dp]),
     "fk_nonlinear_spline": (_c.c_int, [_vp, _up, _dp, _c.c_int32, _dp, _dp, _c.c_int32, _dp, _c.c_double,
                                        _c.c_double, _c.c_int32, _c.c_double, _ip, _i64p, _dp]),
-    "fk_demag_field": (_c.c_int, [_vp, _up, _dp, _dp, _c.c_double, _c.c_int64, _dp, _i64p, _dp]),
+    "fk_demag_field": (_c.c_int, [_vp, _up, _dp, _dp, _c.c_double, _c.c_int64, _c.c_int32, _dp, _i64p, _dp]),
     "fk_line_search": (_c.c_int, [_vp, _dp, _dp, _dp, _c.c_double, _dp]),
     "fk_get_coefficient": (_c.c_int, [_vp, _dp]),
     "fk_adjoint_field_target": (_c.c_int, [_vp, _up, _dp, _c.c_double, _dp, _dp, _i64p]),

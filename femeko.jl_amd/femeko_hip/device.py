@@ -308,11 +308,13 @@ class DeviceMesh:
             self.ctx.check(s, "fk_nonlinear_spline")
         return {"outer": int(outer.value), "cg_iterations": int(cg.value), "change": float(ch.value), "status": s}
 
-    def demag_field(self, inside, M, volumes, rtol: float = 1.4901161193847656e-08, max_iter: int = 100000):
+    def demag_field(self, inside, M, volumes, rtol: float = 1.4901161193847656e-08, max_iter: int = 100000,
+                    warm: bool = False):
         """magnetostaticField (Micromagnetics/LL.jl:162-209) with the resident
         matrix A = stiffnessMatrix(mesh): M is 3 x nv, volumes nv; returns
         (Hd_nodes 3 x nv, PCG iterations, relative residual). rtol defaults to
-        the reference cg's sqrt(eps)."""
+        the reference cg's sqrt(eps); warm starts the PCG from the previous
+        call's potential."""
         ins = np.ascontiguousarray(inside, dtype=np.uint8)
         if ins.size != self.nt:
             raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "inside size")
@@ -324,7 +326,8 @@ class DeviceMesh:
         it = ctypes.c_int64(0)
         rel = ctypes.c_double(0.0)
         self.ctx.check(self._L.fk_demag_field(self.handle, uptr(ins), dptr(Mf.reshape(-1, order="F")), dptr(vol),
-                                              float(rtol), int(max_iter), dptr(Hd.reshape(-1, order="F")),
+                                              float(rtol), int(max_iter), 1 if warm else 0,
+                                              dptr(Hd.reshape(-1, order="F")),
                                               ctypes.byref(it), ctypes.byref(rel)), "fk_demag_field")
         return Hd, int(it.value), float(rel.value)
 

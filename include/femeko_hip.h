@@ -190,9 +190,11 @@ int fk_solve_neumann(fk_mesh *mesh, const double *b, double rtol, int64_t max_it
  * cg(A, RHS) defaults to sqrt(eps)), H_k = -grad(u), and Hd_nodes[:, i] =
  * sum over inside elements k at node i of VE_k H_k / volumes[i] (NaN where
  * volumes[i] == 0, as the reference's 0/0). M, Hd_nodes: 3 x nv column-major
- * (M[3 i + c]); volumes: nv (the driver's per-node sums of inside VE). */
+ * (M[3 i + c]); volumes: nv (the driver's per-node sums of inside VE).
+ * warm != 0: the PCG starts from the previous call's potential on this mesh
+ * (kept on the device) instead of zero, same stopping test. */
 int fk_demag_field(fk_mesh *mesh, const uint8_t *inside, const double *M, const double *volumes, double rtol,
-                   int64_t max_iter, double *Hd_nodes, int64_t *iterations, double *relres);
+                   int64_t max_iter, int32_t warm, double *Hd_nodes, int64_t *iterations, double *relres);
 
 /* ---- nonlinear loop and line search ----------------------------------------
  * Picard/Newton mu(|H|) loop of Magnetostatics/NewtonRaphson.jl:148-212 with

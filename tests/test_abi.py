@@ -60,7 +60,7 @@ def test_null_arguments_rejected_without_gpu():
     assert L.fk_nonlinear_spline(None, None, None, 8, None, None, 8, None, 1e-10, 1e-4, 10, 1.0, None, None,
                                  None) == _lib.FK_ERR_INVALID_ARG
     assert L.fk_status_string(_lib.FK_ERR_DOMAIN) == b"argument outside the table range"
-    assert L.fk_demag_field(None, None, None, None, 1e-8, 100, None, None, None) == _lib.FK_ERR_INVALID_ARG
+    assert L.fk_demag_field(None, None, None, None, 1e-8, 100, 0, None, None, None) == _lib.FK_ERR_INVALID_ARG
     assert L.fk_batch_solver_format(None, None) == _lib.FK_ERR_INVALID_ARG
 
 

@@ -58,3 +58,35 @@ def test_demag_uniform_magnetization_patch_test(oracle):
         np.testing.assert_allclose(Hd, -M, rtol=0, atol=1e-9)
     finally:
         dm.close()
+
+
+def test_demag_warm_start_same_field_fewer_iterations(oracle):
+    # slowly rotating magnetization, one call per step: the start from the
+    # previous potential reaches the same field in fewer iterations
+    pr = W.linear(10)
+    m = pr.mesh
+    inside = pr.magnet
+    vol = oracle.node_volumes(m, inside)
+    innodes = vol > 0
+    dm = _device_A(m)
+    try:
+        cold, warm = [], []
+        for k in range(4):
+            th = 0.05 * k
+            M = np.zeros((3, m.nv))
+            M[0, innodes], M[1, innodes] = np.cos(th), np.sin(th)
+            ref, _u, _rhs = oracle.magnetostatic_field(m, inside, M, vol)
+            ok = ~np.isnan(ref)
+            scale = np.abs(ref[ok]).max()
+            Hc, itc, _ = dm.demag_field(inside.astype(np.uint8), M, vol, rtol=1e-12)
+            Hw, itw, relw = dm.demag_field(inside.astype(np.uint8), M, vol, rtol=1e-12, warm=True)
+            assert relw < 1e-12
+            for H in (Hc, Hw):
+                assert np.array_equal(np.isnan(H), ~ok)
+                assert np.max(np.abs(H[ok] - ref[ok])) < 1e-9 * scale
+            cold.append(itc)
+            warm.append(itw)
+        # the warm call of each step starts from that step's own cold solution
+        assert max(warm) < min(cold)
+    finally:
+        dm.close()
