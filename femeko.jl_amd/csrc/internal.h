@@ -92,7 +92,8 @@ struct CgState {
     double bb;          // ||b||^2
     double rr;          // ||r||^2 at stop
     int64_t max_iter;   // iteration limit of this run
-    double pad[2];
+    double alpha;       // step length of the current iteration (0 after a breakdown)
+    double pad;
 };
 
 // State of the FP32 solve's residual replacements. The decisions are taken

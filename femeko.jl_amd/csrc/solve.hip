@@ -109,10 +109,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const int32_t *__restrict__ 
 // reduction (a band of 4096 rows is 4 vectors per thread in FP32).
 constexpr int kPrefetch = 4;
 
-// K2: x += alpha p; r -= alpha q; partials of r.z and r.r
+// K2: r -= alpha q; partials of r.z and r.r. alpha (every block computes the
+// same value from the same partials) is left in st->alpha for K3, which
+// applies x += alpha p while it reads p anyway: 12 bytes per row less than
+// updating x here.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *__restrict__ r,
-                                                         const T *__restrict__ p, const T *__restrict__ q,
+__global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ r, const T *__restrict__ q,
                                                          const T *__restrict__ dinv,
                                                          const double *__restrict__ part_pq, int npq,
                                                          const double *__restrict__ part_rz_cur,
@@ -129,15 +131,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *_
     using V = Vec16<T>;
     const int64_t v1 = r1 > r0 ? r1 / VW : r0 / VW;
     const int64_t j0 = r0 / VW + threadIdx.x;
-    V D[kPrefetch], P[kPrefetch], Q[kPrefetch], X[kPrefetch], R[kPrefetch];
+    V D[kPrefetch], Q[kPrefetch], R[kPrefetch];
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
         const int64_t j = j0 + u * kBlock;
         if (j < v1) {
             D[u] = reinterpret_cast<const V *>(dinv)[j];
-            P[u] = reinterpret_cast<const V *>(p)[j];
             Q[u] = reinterpret_cast<const V *>(q)[j];
-            X[u] = reinterpret_cast<const V *>(x)[j];
             R[u] = reinterpret_cast<const V *>(r)[j];
         }
     }
@@ -145,19 +145,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *_
     const double pq = sum_partials<kBlock>(part_pq, npq, sh);
     if (!(pq > 0.0) || !isfinite(pq)) {
         // breakdown: A not positive definite on the Krylov space (every
-        // block sees the same pq and skips the update)
+        // block sees the same pq and skips the update; K3 then adds nothing
+        // to x)
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->status = FK_ERR_SINGULAR;
             st->done_at[parity ^ 1] = 1;
+            st->alpha = 0.0;
         }
         return;
     }
     const T alpha = static_cast<T>(rho / pq);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = static_cast<double>(alpha);
     double rz = 0.0, rr = 0.0;
-    auto step = [&](V &Dv, const V &Pv, const V &Qv, V &Xv, V &Rv) {
+    auto step = [&](const V &Dv, const V &Qv, V &Rv) {
 #pragma unroll
         for (int e = 0; e < VW; ++e) {
-            Xv.v[e] += alpha * Pv.v[e];
             const T ri = Dv.v[e] != T(0) ? Rv.v[e] - alpha * Qv.v[e] : T(0);
             Rv.v[e] = ri;
             rz += static_cast<double>(ri) * static_cast<double>(Dv.v[e] * ri);
@@ -168,24 +170,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *_
     for (int u = 0; u < kPrefetch; ++u) {
         const int64_t j = j0 + u * kBlock;
         if (j < v1) {
-            step(D[u], P[u], Q[u], X[u], R[u]);
-            reinterpret_cast<V *>(x)[j] = X[u];
+            step(D[u], Q[u], R[u]);
             reinterpret_cast<V *>(r)[j] = R[u];
         }
     }
     for (int64_t j = j0 + kPrefetch * kBlock; j < v1; j += kBlock) {
-        V Dv = reinterpret_cast<const V *>(dinv)[j];
-        const V Pv = reinterpret_cast<const V *>(p)[j];
+        const V Dv = reinterpret_cast<const V *>(dinv)[j];
         const V Qv = reinterpret_cast<const V *>(q)[j];
-        V Xv = reinterpret_cast<const V *>(x)[j];
         V Rv = reinterpret_cast<const V *>(r)[j];
-        step(Dv, Pv, Qv, Xv, Rv);
-        reinterpret_cast<V *>(x)[j] = Xv;
+        step(Dv, Qv, Rv);
         reinterpret_cast<V *>(r)[j] = Rv;
     }
     for (int64_t i = max(r0, v1 * VW) + threadIdx.x; i < r1; i += kBlock) {
         const T d = dinv[i];
-        x[i] += alpha * p[i];
         const T ri = d != T(0) ? r[i] - alpha * q[i] : T(0);
         r[i] = ri;
         const T z = d * ri;
@@ -200,10 +197,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ x, T *_
     }
 }
 
-// K3: convergence test, p = D^-1 r + beta p
+// K3: convergence test, x += alpha p (alpha of K2; 0 after a breakdown),
+// p = D^-1 r + beta p
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const T *__restrict__ r,
-                                                        const T *__restrict__ dinv,
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ x, T *__restrict__ p,
+                                                        const T *__restrict__ r, const T *__restrict__ dinv,
                                                         const double *__restrict__ part_rz_next,
                                                         const double *__restrict__ part_rz_cur,
                                                         const double *__restrict__ part_rr, int64_t n,
@@ -218,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
     using V = Vec16<T>;
     const int64_t v1 = r1 > r0 ? r1 / VW : r0 / VW;
     const int64_t j0 = r0 / VW + threadIdx.x;
-    V D[kPrefetch], R[kPrefetch], P[kPrefetch];
+    V D[kPrefetch], R[kPrefetch], P[kPrefetch], X[kPrefetch];
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
         const int64_t j = j0 + u * kBlock;
@@ -226,8 +224,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
             D[u] = reinterpret_cast<const V *>(dinv)[j];
             R[u] = reinterpret_cast<const V *>(r)[j];
             P[u] = reinterpret_cast<const V *>(p)[j];
+            X[u] = reinterpret_cast<const V *>(x)[j];
         }
     }
+    const T alpha = static_cast<T>(st->alpha);
     const double rho_new = sum_partials<kBlock>(part_rz_next, nblocks, sh);
     const double rho = sum_partials<kBlock>(part_rz_cur, nblocks, sh);
     if (blockIdx.x == 0) {
@@ -252,10 +252,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
         }
     }
     const T beta = static_cast<T>(rho_new / rho);
+    const bool addx = alpha != T(0);
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
         const int64_t j = j0 + u * kBlock;
         if (j < v1) {
+            if (addx) {
+#pragma unroll
+                for (int e = 0; e < VW; ++e) X[u].v[e] += alpha * P[u].v[e];
+                reinterpret_cast<V *>(x)[j] = X[u];
+            }
 #pragma unroll
             for (int e = 0; e < VW; ++e) P[u].v[e] = D[u].v[e] * R[u].v[e] + beta * P[u].v[e];
             reinterpret_cast<V *>(p)[j] = P[u];
@@ -265,11 +271,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ p, const
         const V Dv = reinterpret_cast<const V *>(dinv)[j];
         const V Rv = reinterpret_cast<const V *>(r)[j];
         V Pv = reinterpret_cast<const V *>(p)[j];
+        if (addx) {
+            V Xv = reinterpret_cast<const V *>(x)[j];
+#pragma unroll
+            for (int e = 0; e < VW; ++e) Xv.v[e] += alpha * Pv.v[e];
+            reinterpret_cast<V *>(x)[j] = Xv;
+        }
 #pragma unroll
         for (int e = 0; e < VW; ++e) Pv.v[e] = Dv.v[e] * Rv.v[e] + beta * Pv.v[e];
         reinterpret_cast<V *>(p)[j] = Pv;
     }
-    for (int64_t i = max(r0, v1 * VW) + threadIdx.x; i < r1; i += kBlock) p[i] = dinv[i] * r[i] + beta * p[i];
+    for (int64_t i = max(r0, v1 * VW) + threadIdx.x; i < r1; i += kBlock) {
+        const T pi = p[i];
+        if (addx) x[i] += alpha * pi;
+        p[i] = dinv[i] * r[i] + beta * pi;
+    }
 }
 
 // ---- SELL-64x4 solver copy of the matrix: off-diagonal entries only (the
@@ -645,6 +661,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(const double *__restrict__ 
         st->rr = rr;
         st->status = FK_OK;
         st->max_iter = max_iter;
+        st->alpha = 0.0;
         int stop = 0;
         if (!isfinite(rr)) {
             st->status = FK_ERR_NONFINITE;
@@ -1283,10 +1300,11 @@ static void enqueue_iteration(fk_mesh *m, PcgWork<T> &w, int parity, bool timed)
         FK_HIP(hipEventRecord(te.ev[te.used + 1], st));
         te.used += 2;
     }
-    k_cg_update_xr<T><<<nb, kBlock, 0, st>>>(w.x.get(), w.r.get(), w.p.get(), w.q.get(), w.dinv.get(),
+    k_cg_update_xr<T><<<nb, kBlock, 0, st>>>(w.r.get(), w.q.get(), w.dinv.get(),
                                              w.part_pq.get(), spmv_blocks<T>(m), rz_cur, rz_next, w.part_rr.get(),
                                              m->nv, m->chunk, nb, w.state.get(), parity);
-    k_cg_update_p<T><<<nb, kBlock, 0, st>>>(w.p.get(), w.r.get(), w.dinv.get(), rz_next, rz_cur, w.part_rr.get(),
+    k_cg_update_p<T><<<nb, kBlock, 0, st>>>(w.x.get(), w.p.get(), w.r.get(), w.dinv.get(), rz_next, rz_cur,
+                                            w.part_rr.get(),
                                             m->nv, m->chunk, nb, w.state.get(), parity);
 }
 
