@@ -65,6 +65,7 @@ struct fk_batch {
     int precision = FK_PREC_F32_REFINED;
     fk::DevBuf<int32_t> ru_act, ru_mode, ru_fin, ru_nrep, ru_restarts, ru_status, nfin;
     fk::DevBuf<double> rr32, rr64, rr_last, thresh64, part_res;
+    fk::DevBuf<float> alpha;    // per system: step length of the current iteration, for the p-update kernel
     hipGraphExec_t ru_graph = nullptr;
     int ru_graph_cfg = -1;
     double ru_delta = -1.0;     // baked into the captured chunk
@@ -249,33 +250,37 @@ __global__ void kb_stencil_fill(const double *__restrict__ val64, const int32_t 
     out[t] = k >= 0 ? static_cast<float>(val64[k]) : 0.f;
 }
 
-// K2: per system alpha, x += alpha p, r -= alpha q, partials of r.z, r.r.
-// NR right-hand sides per node (contiguous). The block's first row is loaded
-// before the per-mesh reductions so the loads overlap them.
+// K2: per system alpha, r -= alpha q, partials of r.z, r.r; the mesh's first
+// block leaves alpha in alphas[] for K3, which applies x += alpha p while it
+// reads p anyway. NR right-hand sides per node (contiguous). The block's
+// first row is loaded before the per-mesh reductions so the loads overlap
+// them.
 template <int BLOCK, int NR>
-__global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__restrict__ x, float *__restrict__ r,
-                                                      const float *__restrict__ p, const float *__restrict__ q,
-                                                      const float *__restrict__ dinv,
+__global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__restrict__ r,
+                                                      const float *__restrict__ q, const float *__restrict__ dinv,
                                                       const double *__restrict__ part_pq,
                                                       const int32_t *__restrict__ pq_b0,
                                                       const int32_t *__restrict__ pq_b1,
                                                       const double *__restrict__ rz_cur, double *__restrict__ rz_next,
                                                       double *__restrict__ part_rr, int32_t *__restrict__ done,
-                                                      int32_t *__restrict__ status, int32_t *__restrict__ ndone) {
+                                                      int32_t *__restrict__ status, int32_t *__restrict__ ndone,
+                                                      float *__restrict__ alphas) {
     __shared__ double sh[BLOCK / 64 + 1];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
-    if (mesh_all_done(done, mesh, NR)) return;
+    if (mesh_all_done(done, mesh, NR)) {
+        // K3 of this iteration then leaves the mesh alone
+        if (lb == ba.mesh_b0[mesh] && threadIdx.x < NR) alphas[mesh * NR + threadIdx.x] = 0.f;
+        return;
+    }
     const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
     const int64_t i0 = r0 + threadIdx.x;
-    float d0 = 0.f, x0[NR], r0v[NR], p0[NR], q0[NR];
+    float d0 = 0.f, r0v[NR], q0[NR];
     if (i0 < r1) {
         d0 = dinv[i0];
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            x0[j] = x[i0 * NR + j];
             r0v[j] = r[i0 * NR + j];
-            p0[j] = p[i0 * NR + j];
             q0[j] = q[i0 * NR + j];
         }
     }
@@ -289,6 +294,7 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
         const bool live = !done[mesh * NR + j];
         act[j] = live && pq > 0.0 && isfinite(pq);
         alpha[j] = act[j] ? static_cast<float>(rho / pq) : 0.f;
+        if (lb == b0 && threadIdx.x == 0) alphas[mesh * NR + j] = alpha[j];
         if (live && !act[j] && lb == b0 && threadIdx.x == 0) {
             // breakdown (every block of the mesh sees the same pq): stop the
             // system, its caller restarts or reports it
@@ -300,12 +306,11 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
     double rz[NR], rr[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) rz[j] = rr[j] = 0.0;
-    auto row = [&](int64_t i, float d, float *xv, float *rv, const float *pv, const float *qv) {
+    auto row = [&](int64_t i, float d, float *rv, const float *qv) {
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             float ri = rv[j];
             if (act[j]) {
-                x[i * NR + j] = xv[j] + alpha[j] * pv[j];
                 ri = d != 0.f ? ri - alpha[j] * qv[j] : 0.f;
                 r[i * NR + j] = ri;
             }
@@ -314,18 +319,16 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
             rr[j] += static_cast<double>(ri) * static_cast<double>(ri);
         }
     };
-    if (i0 < r1) row(i0, d0, x0, r0v, p0, q0);
+    if (i0 < r1) row(i0, d0, r0v, q0);
     for (int64_t i = i0 + BLOCK; i < r1; i += BLOCK) {
-        float xv[NR], rv[NR], pv[NR], qv[NR];
+        float rv[NR], qv[NR];
         const float d = dinv[i];
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            xv[j] = x[i * NR + j];
             rv[j] = r[i * NR + j];
-            pv[j] = p[i * NR + j];
             qv[j] = q[i * NR + j];
         }
-        row(i, d, xv, rv, pv, qv);
+        row(i, d, rv, qv);
     }
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
@@ -338,9 +341,15 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
     }
 }
 
-// K3: per system convergence test (first block of the mesh), p = D^-1 r + beta p
+// K3: per system convergence test (first block of the mesh), x += alpha p,
+// p = D^-1 r + beta p for the systems K2 stepped (alpha != 0). The mesh's
+// first block marks converged systems done during this launch, so the other
+// blocks decide from alpha, which no block of this launch writes, and all
+// rows of a system finish the step together.
 template <int BLOCK, int NR>
-__global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__restrict__ p, const float *__restrict__ r,
+__global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__restrict__ x, float *__restrict__ p,
+                                                     const float *__restrict__ r,
+                                                     const float *__restrict__ alphas,
                                                      const float *__restrict__ dinv,
                                                      const double *__restrict__ rz_next,
                                                      const double *__restrict__ rz_cur,
@@ -351,31 +360,39 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
     __shared__ double sh[BLOCK / 64 + 1];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
-    if (mesh_all_done(done, mesh, NR)) return;
+    float alpha[NR];
+    bool upd[NR];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        alpha[j] = alphas[mesh * NR + j];
+        upd[j] = alpha[j] != 0.f;
+        any |= upd[j];
+    }
+    if (!any) return;
     const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
     const int64_t i0 = r0 + threadIdx.x;
-    float d0 = 0.f, rv0[NR], pv0[NR];
+    float d0 = 0.f, rv0[NR], pv0[NR], xv0[NR];
     if (i0 < r1) {
         d0 = dinv[i0];
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             rv0[j] = r[i0 * NR + j];
             pv0[j] = p[i0 * NR + j];
+            xv0[j] = x[i0 * NR + j];
         }
     }
     const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
     float beta[NR];
-    bool act[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
         const int s = mesh * NR + j;
-        act[j] = !done[s];
         const double rho_new = seg_sum<BLOCK>(rz_next, b0, b1, j, NR, sh);
         const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, NR, sh);
-        beta[j] = act[j] ? static_cast<float>(rho_new / rho) : 0.f;
+        beta[j] = upd[j] ? static_cast<float>(rho_new / rho) : 0.f;
         if (lb == b0) {
             const double rr = seg_sum<BLOCK>(part_rr, b0, b1, j, NR, sh);
-            if (threadIdx.x == 0 && act[j]) {
+            if (threadIdx.x == 0 && upd[j]) {
                 const int64_t it = iters[s] + 1;
                 iters[s] = it;
                 rr32[s] = rr;
@@ -399,13 +416,20 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
     if (i0 < r1) {
 #pragma unroll
         for (int j = 0; j < NR; ++j)
-            if (act[j]) p[i0 * NR + j] = d0 * rv0[j] + beta[j] * pv0[j];
+            if (upd[j]) {
+                x[i0 * NR + j] = xv0[j] + alpha[j] * pv0[j];
+                p[i0 * NR + j] = d0 * rv0[j] + beta[j] * pv0[j];
+            }
     }
     for (int64_t i = i0 + BLOCK; i < r1; i += BLOCK) {
         const float d = dinv[i];
 #pragma unroll
         for (int j = 0; j < NR; ++j)
-            if (act[j]) p[i * NR + j] = d * r[i * NR + j] + beta[j] * p[i * NR + j];
+            if (upd[j]) {
+                const float pj = p[i * NR + j];
+                x[i * NR + j] += alpha[j] * pj;
+                p[i * NR + j] = d * r[i * NR + j] + beta[j] * pj;
+            }
     }
 }
 
@@ -883,12 +907,14 @@ static void iteration_tail(fk_batch *b, int parity, int64_t max_iter) {
     double *rz_cur = b->part_rz.get() + (parity ? nb * nr : 0);
     double *rz_next = b->part_rz.get() + (parity ? 0 : nb * nr);
     fk_mesh *m = b->mesh;
-    kb_update_xr<BLOCK, NR><<<nb, BLOCK, 0, st>>>(args_of(b), b->x.get(), b->r.get(), b->p.get(), b->q.get(),
+    kb_update_xr<BLOCK, NR><<<nb, BLOCK, 0, st>>>(args_of(b), b->r.get(), b->q.get(),
                                               m->w32.dinv.get(), b->part_pq.get(),
                                               b->stencil == 1 ? b->sp_mesh_b0.get() : b->mesh_b0.get(),
                                               b->stencil == 1 ? b->sp_mesh_b1.get() : b->mesh_b1.get(), rz_cur, rz_next,
-                                              b->part_rr.get(), b->done.get(), b->status.get(), b->ndone.get());
-    kb_update_p<BLOCK, NR><<<nb, BLOCK, 0, st>>>(args_of(b), b->p.get(), b->r.get(), m->w32.dinv.get(), rz_next, rz_cur,
+                                              b->part_rr.get(), b->done.get(), b->status.get(), b->ndone.get(),
+                                              b->alpha.get());
+    kb_update_p<BLOCK, NR><<<nb, BLOCK, 0, st>>>(args_of(b), b->x.get(), b->p.get(), b->r.get(), b->alpha.get(),
+                                             m->w32.dinv.get(), rz_next, rz_cur,
                                              b->part_rr.get(), b->done.get(), b->status.get(), b->iters.get(),
                                              b->thresh.get(), max_iter, b->ndone.get(), b->rr32.get());
 }
@@ -1382,6 +1408,7 @@ int fk_batch_create(fk_context *ctx, int32_t nmesh, const int64_t *nv, const int
         b->ru_act.alloc(S); b->ru_mode.alloc(S); b->ru_fin.alloc(S); b->ru_nrep.alloc(S);
         b->ru_restarts.alloc(S); b->ru_status.alloc(S); b->nfin.alloc(1);
         b->rr32.alloc(S); b->rr64.alloc(S); b->rr_last.alloc(S); b->thresh64.alloc(S); b->part_res.alloc(np);
+        b->alpha.alloc(S);
         b->hpart.alloc(np + 16);
         b->hcount.alloc(4);
     });

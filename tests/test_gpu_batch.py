@@ -7,6 +7,7 @@ import pytest
 
 from femeko_hip import workloads as W
 from femeko_hip.batch import Batch
+from femeko_hip.device import DeviceMesh, context
 from femeko_hip.workloads import dirichlet_values
 
 pytestmark = pytest.mark.gpu
@@ -172,5 +173,37 @@ def test_batch_stencil_mixed_offset_counts(oracle):
         assert np.all(b.solve(rtol=1e-10).relres < 1e-10)
         assert b.solver_format() == "stencil"
         _check_oracle(oracle, b, probs, gs)
+    finally:
+        b.close()
+
+
+def test_batch_iterations_close_to_single_mesh_solves():
+    # every row of a system must take each step: with several blocks per mesh
+    # a block that skipped a step would leave x inconsistent and cost many
+    # extra iterations after the replacements. The batch's count stays near
+    # the single-mesh FP32 solver's on the same systems.
+    probs = W.batch(count=3, nmin=18, nmax=22, seed=31)
+    gs = _fields(probs)
+    b = Batch([p.mesh for p in probs], nrhs=3)
+    try:
+        b.set_coefficient([p.mu for p in probs])
+        b.assemble()
+        b.set_dirichlet([p.mask for p in probs], gs)
+        rep = b.solve(rtol=1e-10)
+        assert np.all(rep.relres < 1e-10)
+        single = []
+        for i, p in enumerate(probs):
+            dm = DeviceMesh(p.mesh, context(0))
+            try:
+                dm.set_coefficient(p.mu)
+                dm.assemble()
+                for j in range(3):
+                    dm.set_dirichlet(p.mask, gs[i][j])
+                    single.append(dm.solve("f32", rtol=1e-10, inner_rtol=0.9).iterations)
+            finally:
+                dm.close()
+        single = np.array(single)
+        its = np.asarray(rep.iterations).reshape(-1)  # system s = mesh * 3 + field
+        assert np.all(its <= 1.5 * single + 64), (its, single)
     finally:
         b.close()
