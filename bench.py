@@ -528,6 +528,13 @@ def main():
         "outer_refinements": rep.outer,
         "relres": rep.relres,
         "ms_assembly": t_asm * 1e3,
+        # value assembly against its compulsory FP64 bytes (t 16 nt, mu 8 nt,
+        # coordinates 24 nv, values 8 nnz written)
+        "assembly_roofline": {"bound": "hbm", "kernel": "k_assemble_gather",
+                              "bytes": 16 * m.nt + 8 * m.nt + 24 * m.nv + 8 * dm.nnz,
+                              "achieved": (16 * m.nt + 8 * m.nt + 24 * m.nv + 8 * dm.nnz) / t_asm / 1e9,
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": (16 * m.nt + 8 * m.nt + 24 * m.nv + 8 * dm.nnz) / t_asm / 1e9 / HBM_PEAK_GBS},
         "fp64": {"ms_per_step": t_step64 * 1e3, "iterations": rep64.iterations, "relres": rep64.relres,
                  "phi_rel_l2_vs_rtol1e-13": err_phi64,
                  "dof_per_s": m.nv / t_step64},

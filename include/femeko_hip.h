@@ -134,13 +134,16 @@ int fk_set_dirichlet(fk_mesh *mesh, const uint8_t *mask, const double *g);
  * run.jl:141) on the assembled matrix, stopping at ||r||_2 < rtol ||b||_2.
  * precision FK_PREC_F64: one FP64 PCG to rtol.
  * precision FK_PREC_F32_REFINED: FP32 values and PCG recursion (difference-
- * form SpMV on the SELL formats); at the end of each chunk of iterations in
+ * form SpMV on the stencil and SELL copies); at the end of each chunk of iterations in
  * which the FP32 recursive residual has fallen by the factor inner_rtol
  * (0 < inner_rtol < 1; other values mean 0.9) since the last update, the
  * FP32 increments are folded into the FP64 potential and the residual is
  * replaced by the FP64 true residual -mask(A64 phi), keeping the search
  * direction ("reliable updates"). The decisions are taken on the device;
- * the solve stops on the FP64 residual (max_outer unused).
+ * the solve stops on the FP64 residual (max_outer unused). At the same
+ * rtol its potential can be further from the exact solution than the FP64
+ * solve's (measured at 4.2M nodes, rtol 1e-10: up to 10x, the remaining
+ * error in smooth modes); half the rtol brought it level.
  * precision FK_PREC_F32_RESTART: FP32 PCG to inner_rtol restarted inside an
  * FP64 residual/update loop (at most max_outer restarts).
  * warm_start != 0 starts from the resident potential. The potential stays
