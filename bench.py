@@ -21,7 +21,9 @@ events around every SpMV launch of an instrumented solve), and the CPU oracle
 timed on a bounded sample of the same workload.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160] [--no-cpu]
-                [--shape 1024,1] [--delta 0.9] [--format auto|stencil|sell64|sell64-rowidx|csr] [--workload boxes|batch|froehlich|adjoint]
+                [--shape 1024,1] [--delta 0.95] [--rtol 5e-11] [--rtol64 1e-10]
+                [--format auto|stencil|sell64|sell64-rowidx|csr]
+                [--workload boxes|batch|froehlich|tabulated|demag|adjoint]
 """
 import argparse
 import json
@@ -359,7 +361,7 @@ def main():
                          "ranged 1.4e-10 to 1.06e-9)")
     ap.add_argument("--rtol64", type=float, default=1e-10,
                     help="relative residual the plain FP64 solve stops at (the configured 1e-10)")
-    ap.add_argument("--delta", type=float, default=0.9,
+    ap.add_argument("--delta", type=float, default=0.95,
                     help="FP32 solve: residual drop that triggers an FP64 residual replacement")
     ap.add_argument("--shape", default="1024,1",
                     help="SELL SpMV launch shape: threads per block, slices in flight per wave")

@@ -1827,8 +1827,8 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
     } else if (precision == FK_PREC_F32_REFINED) {
         double rr = 0.0;
         // inner_rtol is the reliable-update trigger: replace once the FP32
-        // residual fell by this factor (values outside (0,1) mean 0.9)
-        const double delta = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 0.9;
+        // residual fell by this factor (values outside (0,1) mean 0.95)
+        const double delta = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 0.95;
         status = solve_f32_replace(m, rtol, max_iter, bb, delta, &total, &nout, &rr);
         rel = std::sqrt(rr / bb);
     } else {

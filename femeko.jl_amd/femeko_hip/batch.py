@@ -91,10 +91,10 @@ class Batch:
 
     def solve(self, rtol: float = 1e-10, inner_rtol: float | None = None, max_outer: int = 50,
               max_iter: int = 1_000_000) -> BatchReport:
-        """inner_rtol: replacement trigger (default 0.9) with 'f32', inner
+        """inner_rtol: replacement trigger (default 0.95) with 'f32', inner
         tolerance (default 1e-5) with 'f32-restart'."""
         if inner_rtol is None:
-            inner_rtol = 1e-5 if getattr(self, "precision", "f32") == "f32-restart" else 0.9
+            inner_rtol = 1e-5 if getattr(self, "precision", "f32") == "f32-restart" else 0.95
         S = self.nmesh * self.nrhs
         it = np.zeros(S, np.int64)
         outer = np.zeros(S, np.int32)

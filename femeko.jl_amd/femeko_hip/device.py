@@ -200,10 +200,10 @@ class DeviceMesh:
               inner_rtol: float | None = None, max_outer: int = 50, warm_start: bool = False,
               raise_on_error: bool = True) -> SolveReport:
         """precision 'f64'; 'f32' (FP32 PCG with FP64 reliable updates, inner_rtol =
-        residual drop that triggers an update, default 0.9); 'f32-restart'
+        residual drop that triggers an update, default 0.95); 'f32-restart'
         (restarted FP32 PCG to inner_rtol, default 1e-5, inside FP64 refinement)."""
         if inner_rtol is None:
-            inner_rtol = 1e-5 if precision == "f32-restart" else 0.9
+            inner_rtol = 1e-5 if precision == "f32-restart" else 0.95
         it = ctypes.c_int64(0)
         outer = ctypes.c_int32(0)
         rel = ctypes.c_double(0.0)

@@ -138,7 +138,7 @@ end
 
 # Assemble A(mu), fix u = g where fixed[i], Jacobi-PCG for the free nodes.
 function solveDirichlet(mesh, mu::Vector{Float64}, fixed::AbstractVector{Bool}, g::Vector{Float64};
-                        rtol = 1e-10, precision = :f64, inner_rtol = 0.9, maxiter = 1_000_000)
+                        rtol = 1e-10, precision = :f64, inner_rtol = 0.95, maxiter = 1_000_000)
     dm = device_mesh(mesh)
     check(dm.ctx, ccall((:fk_set_coefficient, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), dm.handle, mu),
           "fk_set_coefficient")

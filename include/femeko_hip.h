@@ -136,7 +136,7 @@ int fk_set_dirichlet(fk_mesh *mesh, const uint8_t *mask, const double *g);
  * precision FK_PREC_F32_REFINED: FP32 values and PCG recursion (difference-
  * form SpMV on the stencil and SELL copies); at the end of each chunk of iterations in
  * which the FP32 recursive residual has fallen by the factor inner_rtol
- * (0 < inner_rtol < 1; other values mean 0.9) since the last update, the
+ * (0 < inner_rtol < 1, e.g. 0.95; other values mean 0.95) since the last update, the
  * FP32 increments are folded into the FP64 potential and the residual is
  * replaced by the FP64 true residual -mask(A64 phi), keeping the search
  * direction ("reliable updates"). The decisions are taken on the device;
@@ -270,7 +270,7 @@ int fk_batch_set_dirichlet(fk_batch *batch, const uint8_t *const *mask, const do
 /* Solve every system to ||r|| < rtol ||b|| (FP64 true residual). With
  * FK_PREC_F32_REFINED (default): FP32 PCG per system with FP64 reliable
  * updates decided on the device, inner_rtol = the residual drop that triggers
- * a replacement (e.g. 0.9), max_outer unused. With FK_PREC_F32_RESTART: FP32
+ * a replacement (e.g. 0.95), max_outer unused. With FK_PREC_F32_RESTART: FP32
  * PCG to inner_rtol (e.g. 1e-5) restarted inside an FP64 residual-correction
  * loop of at most max_outer steps. Per system s = m*nrhs + j: iterations[s],
  * outer[s] (replacements or restarts), relres[s]. */
