@@ -396,14 +396,8 @@ def main():
         if args.renumber == "random":
             perm = np.random.default_rng(args.n).permutation(pr.mesh.nv)
         else:
-            import scipy.sparse as sp
-            from scipy.sparse.csgraph import reverse_cuthill_mckee
-            tmp = DeviceMesh(pr.mesh, ctx)
-            rp, ci = tmp.pattern(base=0)
-            tmp.close()
-            G = sp.csr_matrix((np.ones(ci.size, np.int8), ci, rp), shape=(pr.mesh.nv, pr.mesh.nv))
-            perm = reverse_cuthill_mckee(G, symmetric_mode=True)
-            del G, rp, ci
+            from femeko_hip.mesh import rcm_permutation
+            perm = rcm_permutation(pr.mesh)
         pr = W.renumbered(pr, perm, args.renumber)
         log(f"renumbered nodes ({args.renumber})")
     m = pr.mesh

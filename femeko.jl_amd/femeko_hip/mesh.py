@@ -248,6 +248,44 @@ def set_inside(mesh: MESH, inside: np.ndarray) -> None:
     mesh.InsideNodes = np.unique(mesh.t[:, inside].ravel()).astype(np.int32)
 
 
+def rcm_permutation(mesh: MESH) -> np.ndarray:
+    """Reverse Cuthill-McKee order of the mesh's node graph (two nodes
+    adjacent when they share an element): perm[k] = the old number of new
+    node k (0-based). Renumbering an unstructured mesh this way before upload
+    keeps a row's neighbours near it in memory, which the solver's SELL copy
+    needs for cache reuse (the renumbering Gmsh offers at export)."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    t0 = np.asarray(mesh.t, dtype=np.int64) - 1
+    pairs = [(a, b) for a in range(4) for b in range(4) if a != b]
+    rows = np.concatenate([t0[a] for a, _ in pairs])
+    cols = np.concatenate([t0[b] for _, b in pairs])
+    G = sp.csr_matrix((np.ones(rows.size, np.int8), (rows, cols)), shape=(mesh.nv, mesh.nv))
+    return np.asarray(reverse_cuthill_mckee(G, symmetric_mode=True), dtype=np.int64)
+
+
+def renumber_nodes(mesh: MESH, perm: np.ndarray) -> MESH:
+    """The same mesh with node k = old node perm[k] (0-based): p's columns
+    permuted, t and surfaceT mapped, elements and their order unchanged,
+    InsideNodes mapped. Node data of the caller maps the same way
+    (new = old[perm])."""
+    perm = np.asarray(perm, dtype=np.int64)
+    if perm.shape != (mesh.nv,) or not np.array_equal(np.sort(perm), np.arange(mesh.nv)):
+        raise ValueError("perm must be a permutation of the nodes")
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    t = np.asfortranarray((inv[np.asarray(mesh.t, dtype=np.int64) - 1] + 1).astype(np.int32))
+    out = MESH(p=np.asfortranarray(mesh.p[:, perm]), t=t, VE=mesh.VE,
+               InsideElements=mesh.InsideElements.copy(),
+               InsideNodes=np.sort((inv[np.asarray(mesh.InsideNodes, dtype=np.int64) - 1] + 1).astype(np.int32)),
+               normal=mesh.normal, AE=mesh.AE, shell_id=mesh.shell_id, order=mesh.order, n_cells=0)
+    if mesh.surfaceT is not None:
+        sT = np.array(mesh.surfaceT, dtype=np.int32, order="F", copy=True)
+        sT[:3] = inv[sT[:3].astype(np.int64) - 1] + 1
+        out.surfaceT = sT
+    return out
+
+
 def kuhn_counts(n: int) -> dict:
     """Closed-form sizes of the Kuhn grid: nt, nv, CSR nnz, free and boundary nodes."""
     n1 = n + 1

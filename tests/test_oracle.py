@@ -541,3 +541,26 @@ def test_demag_energy_identity_and_nan_outside(oracle):
     out = vol == 0
     assert out.any() and np.all(np.isnan(Hd[:, out])) and np.all(np.isfinite(Hd[:, ~out]))
     assert abs(rhs.sum()) < 1e-12 * np.abs(rhs).sum()
+
+
+def test_rcm_renumbering_permutes_the_matrix(oracle):
+    # a randomly numbered mesh renumbered by RCM: the bandwidth drops to the
+    # structured numbering's order and the stiffness matrix is the same matrix
+    # with rows and columns permuted (A' = P A P^T, bitwise)
+    from femeko_hip.mesh import rcm_permutation, renumber_nodes
+    pr = W.linear(6)
+    rng = np.random.default_rng(5)
+    rnd = renumber_nodes(pr.mesh, rng.permutation(pr.mesh.nv))
+    perm = rcm_permutation(rnd)
+    m2 = renumber_nodes(rnd, perm)
+
+    def bandwidth(m):
+        A = oracle.stiffnessMatrix(m, pr.mu).to_scipy().tocoo()
+        return int(np.max(np.abs(A.row - A.col)))
+
+    assert bandwidth(m2) < bandwidth(rnd) / 4
+    A1 = oracle.stiffnessMatrix(rnd, pr.mu).to_scipy().toarray()
+    A2 = oracle.stiffnessMatrix(m2, pr.mu).to_scipy().toarray()
+    np.testing.assert_array_equal(A2, A1[np.ix_(perm, perm)])
+    with pytest.raises(ValueError):
+        renumber_nodes(rnd, np.zeros(rnd.nv, np.int64))
