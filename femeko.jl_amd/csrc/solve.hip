@@ -807,10 +807,21 @@ static int stencil_blocks(const fk_mesh *m) {
     return static_cast<int>(8 * std::max<int64_t>(1, (runs + 1) / 2));
 }
 
+// Blocks of the FP32 SELL SpMV: one 64-row slice per wave, so a block is as
+// short-lived as the stencil kernel's and the blocks of an XCD sweep its
+// band in dispatch order.
+static int sell_blocks(const fk_mesh *m) {
+    const int64_t ns = (m->nv + 63) / 64;
+    const int64_t per = std::max(1, m->sell_block / 64);
+    return static_cast<int>((((ns + per - 1) / per) + 7) / 8 * 8);
+}
+
 // blocks of the SpMV kernel, which writes one p.q partial per block
 template <typename T>
 static int spmv_blocks(const fk_mesh *m) {
-    return sizeof(T) == 4 && m->kind == FK_SPMV_STENCIL ? stencil_blocks(m) : m->nblocks;
+    if (sizeof(T) == 4 && m->kind == FK_SPMV_STENCIL) return stencil_blocks(m);
+    if (sizeof(T) == 4 && m->kind == FK_SPMV_SELL64) return sell_blocks(m);
+    return m->nblocks;
 }
 
 template <typename T>
@@ -818,7 +829,7 @@ static void ensure_work(fk_mesh *m) {
     PcgWork<T> &w = work<T>(m);
     const size_t nv = static_cast<size_t>(m->nv), nb = static_cast<size_t>(m->nblocks);
     w.x.ensure(nv); w.r.ensure(nv); w.p.ensure(nv); w.q.ensure(nv); w.dinv.ensure(nv);
-    w.part_pq.ensure(std::max<size_t>(nb, static_cast<size_t>(stencil_blocks(m)))); w.part_rz.ensure(2 * nb); w.part_rr.ensure(nb); w.part_bb.ensure(nb);
+    w.part_pq.ensure(std::max<size_t>({nb, static_cast<size_t>(stencil_blocks(m)), static_cast<size_t>(sell_blocks(m))})); w.part_rz.ensure(2 * nb); w.part_rr.ensure(nb); w.part_bb.ensure(nb);
     w.state.ensure(1);
 }
 
@@ -1199,9 +1210,11 @@ static Sell<T, IDX> sell_view(fk_mesh *m, const IDX *col) {
 
 template <typename T, typename IDX, int BLOCK, int U>
 static void launch_sell(fk_mesh *m, PcgWork<T> &w, const IDX *col, int parity) {
-    k_cg_spmv_sell<T, IDX, BLOCK, U><<<m->nblocks, BLOCK, 0, m->ctx->stream>>>(
-        sell_view<T, IDX>(m, col), w.p.get(), w.q.get(), w.part_pq.get(), m->nslices, m->sell_chunk, m->nblocks,
-        w.state.get(), parity);
+    const int nb = spmv_blocks<T>(m);
+    const int64_t chunk = nb == m->nblocks ? m->sell_chunk : (m->nslices + nb - 1) / nb;
+    k_cg_spmv_sell<T, IDX, BLOCK, U><<<nb, BLOCK, 0, m->ctx->stream>>>(
+        sell_view<T, IDX>(m, col), w.p.get(), w.q.get(), w.part_pq.get(), m->nslices, chunk, nb, w.state.get(),
+        parity);
 }
 
 template <typename T, typename IDX>
