@@ -80,6 +80,7 @@ struct fk_batch {
     fk::DevBuf<int32_t> st_off, st_K, st_map;
     fk::DevBuf<int64_t> st_vbase;
     fk::DevBuf<float> st_val32;
+    fk::DevBuf<double> st_val64, st_diag64;  // FP64 copy and diagonal for the true residuals
     ~fk_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (ru_graph) (void)hipGraphExecDestroy(ru_graph);
@@ -488,6 +489,80 @@ __global__ __launch_bounds__(BLOCK) void kb_neg_residual(BatchArgs ba, const int
     for (int j = 0; j < nrhs; ++j) {
         const double t = block_sum<BLOCK>(acc[j], sh);
         if (threadIdx.x == 0) part[lb * nrhs + j] = t;
+    }
+}
+
+// r64 = -mask(A64 phi) from the per-mesh FP64 stencil copy (standard form
+// with the diagonal), partials of |r|^2 per [block][rhs] on the batch's main
+// block table, gate as kb_neg_residual. Loads guarded by the mesh's rows.
+template <int BLOCK, int NR, int KC>
+__global__ __launch_bounds__(BLOCK) void kb_neg_residual_stencil(
+    BatchArgs ba, const int64_t *__restrict__ node_off, const int32_t *__restrict__ Ks,
+    const int32_t *__restrict__ offs, const int64_t *__restrict__ vbase, const double *__restrict__ val,
+    const double *__restrict__ diag, const double *__restrict__ phi, const uint8_t *__restrict__ mask,
+    double *__restrict__ r, double *__restrict__ part, const int32_t *__restrict__ gate) {
+    __shared__ double sh[BLOCK / 64 + 1];
+    const int lb = batch_logical_block(ba.nblocks);
+    const int mesh = ba.blk_mesh[lb];
+    if (gate) {
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < NR; ++j) any |= gate[mesh * NR + j] == 1;
+        if (!any) return;
+    }
+    const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
+    const int64_t a = node_off[mesh], n = node_off[mesh + 1] - a;
+    const int K = KC > 0 ? KC : Ks[mesh];
+    const double *__restrict__ v0 = val + vbase[mesh];
+    double acc[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) acc[j] = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        const int64_t li = i - a;
+        double y[NR];
+        const double d = diag[i];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) y[j] = d * phi[i * NR + j];
+        for (int k = 0; k < K; ++k) {
+            const int64_t o = offs[16 * mesh + k];
+            const double *__restrict__ v = v0 + static_cast<int64_t>(k) * n;
+            if (li + o < n) {
+                const double w = v[li];
+#pragma unroll
+                for (int j = 0; j < NR; ++j) y[j] += w * phi[(i + o) * NR + j];
+            }
+            if (li >= o) {
+                const double w = v[li - o];
+#pragma unroll
+                for (int j = 0; j < NR; ++j) y[j] += w * phi[(i - o) * NR + j];
+            }
+        }
+        const bool fixed = mask[i] != 0;
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const double rv = fixed ? 0.0 : -y[j];
+            r[i * NR + j] = rv;
+            acc[j] += rv * rv;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const double t = block_sum<BLOCK>(acc[j], sh);
+        if (threadIdx.x == 0) part[lb * NR + j] = t;
+    }
+}
+
+__global__ void kb_stencil_fill64(const double *__restrict__ val64, const int32_t *__restrict__ map,
+                                  const int32_t *__restrict__ diag_pos, int64_t len, int64_t nv,
+                                  double *__restrict__ out, double *__restrict__ diag) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < len) {
+        const int32_t k = map[t];
+        out[t] = k >= 0 ? val64[k] : 0.0;
+    }
+    if (t < nv) {
+        const int32_t k = diag_pos[t];
+        diag[t] = k >= 0 ? val64[k] : 0.0;
     }
 }
 
@@ -938,11 +1013,38 @@ static void enqueue_batch_iteration(fk_batch *b, int parity, int64_t max_iter) {
     }
 }
 
+template <int NR, int KC>
+static void launch_neg_residual_stencil(fk_batch *b, const double *phi, double *part, const int32_t *gate) {
+    kb_neg_residual_stencil<256, NR, KC><<<b->nblocks, 256, 0, b->ctx->stream>>>(
+        args_of(b), b->node_off_dev.get(), b->st_K.get(), b->st_off.get(), b->st_vbase.get(), b->st_val64.get(),
+        b->st_diag64.get(), phi, b->mesh->mask.get(), b->r64.get(), part, gate);
+}
+
+template <int NR>
+static void launch_neg_residual_stencil_nr(fk_batch *b, const double *phi, double *part, const int32_t *gate) {
+    if (b->st_Kc == 7)
+        launch_neg_residual_stencil<NR, 7>(b, phi, part, gate);
+    else
+        launch_neg_residual_stencil<NR, 0>(b, phi, part, gate);
+}
+
+// r64 = -mask(A64 phi): the FP64 stencil copy when the batch has one (filled
+// with the FP32 copy after each assembly), else the CSR values
 static void neg_residual_batch(fk_batch *b, const double *phi, double *part = nullptr, const int32_t *gate = nullptr) {
     fk_mesh *m = b->mesh;
-    kb_neg_residual<256><<<b->nblocks, 256, 0, b->ctx->stream>>>(args_of(b), m->rowptr.get(), m->colidx.get(),
-                                                                m->val64.get(), phi, m->mask.get(), b->r64.get(),
-                                                                part ? part : b->part_rr.get(), gate);
+    double *pt = part ? part : b->part_rr.get();
+    if (b->stencil == 1 && b->st_valid) {
+        switch (b->nrhs) {
+            case 1: launch_neg_residual_stencil_nr<1>(b, phi, pt, gate); break;
+            case 2: launch_neg_residual_stencil_nr<2>(b, phi, pt, gate); break;
+            case 3: launch_neg_residual_stencil_nr<3>(b, phi, pt, gate); break;
+            default: launch_neg_residual_stencil_nr<4>(b, phi, pt, gate); break;
+        }
+    } else {
+        kb_neg_residual<256><<<b->nblocks, 256, 0, b->ctx->stream>>>(args_of(b), m->rowptr.get(), m->colidx.get(),
+                                                                    m->val64.get(), phi, m->mask.get(),
+                                                                    b->r64.get(), pt, gate);
+    }
     FK_HIP(hipGetLastError());
 }
 
@@ -1240,6 +1342,8 @@ static void build_batch_stencil(fk_batch *b) {
     b->st_vbase.alloc(M);
     b->st_map.alloc(len);
     b->st_val32.alloc(len);
+    b->st_val64.alloc(len);
+    b->st_diag64.alloc(nv);
     b->node_off_dev.alloc(M + 1);
     FK_HIP(hipMemcpyAsync(b->st_K.get(), K.data(), M * sizeof(int32_t), hipMemcpyHostToDevice, st));
     FK_HIP(hipMemcpyAsync(b->st_off.get(), offs.data(), offs.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -1272,6 +1376,9 @@ static void prepare_spmm(fk_batch *b) {
     if (b->stencil == 1 && !b->st_valid) {
         kb_stencil_fill<<<grid_for(b->st_len, 256), 256, 0, b->ctx->stream>>>(m->val64.get(), b->st_map.get(),
                                                                              b->st_len, b->st_val32.get());
+        kb_stencil_fill64<<<grid_for(std::max(b->st_len, m->nv), 256), 256, 0, b->ctx->stream>>>(
+            m->val64.get(), b->st_map.get(), m->diag.get(), b->st_len, m->nv, b->st_val64.get(),
+            b->st_diag64.get());
         FK_HIP(hipGetLastError());
         b->st_valid = true;
     }
