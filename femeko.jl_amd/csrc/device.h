@@ -117,6 +117,29 @@ __device__ __forceinline__ double sum_partials(const double *__restrict__ part, 
     return block_sum<BLOCK>((s0 + s1) + (s2 + s3), sh);
 }
 
+// The same for a long partial array (16-byte aligned): pairs of partials per
+// load, four pairs in flight per thread.
+template <int BLOCK>
+__device__ __forceinline__ double sum_partials_wide(const double *__restrict__ part, int n, double *sh) {
+    const double2 *__restrict__ p2 = reinterpret_cast<const double2 *>(part);
+    const int n2 = n >> 1;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * BLOCK < n2; i += 4 * BLOCK) {
+        const double2 a = p2[i], b = p2[i + BLOCK], c = p2[i + 2 * BLOCK], d = p2[i + 3 * BLOCK];
+        s0 += a.x + a.y;
+        s1 += b.x + b.y;
+        s2 += c.x + c.y;
+        s3 += d.x + d.y;
+    }
+    for (; i < n2; i += BLOCK) {
+        const double2 a = p2[i];
+        s0 += a.x + a.y;
+    }
+    if ((n & 1) && threadIdx.x == 0) s1 += part[n - 1];
+    return block_sum<BLOCK>((s0 + s1) + (s2 + s3), sh);
+}
+
 // Blocks are handed to the 8 XCDs round-robin; give each XCD a contiguous
 // 1/8 of the logical chunks so its L2 holds one band of rows (and the x
 // entries those rows gather).

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ r, cons
         }
     }
     const double rho = sum_partials<kBlock>(part_rz_cur, nblocks, sh);
-    const double pq = sum_partials<kBlock>(part_pq, npq, sh);
+    const double pq = sum_partials_wide<kBlock>(part_pq, npq, sh);
     if (!(pq > 0.0) || !isfinite(pq)) {
         // breakdown: A not positive definite on the Krylov space (every
         // block sees the same pq and skips the update; K3 then adds nothing
@@ -800,7 +800,7 @@ static int solver_config(const fk_mesh *m) {
 // XCD's band. Measured at n=160 (4.17M rows): 26.7 us against 35.9 us for the
 // 1024-block grid of the other kernels, whose blocks sweep 8 runs each and
 // drift apart, so the rows one grid plane back have left L2 more often.
-constexpr int kStencilBlock = 512;
+constexpr int kStencilBlock = 256;
 static int stencil_blocks(const fk_mesh *m) {
     const int64_t band = ((m->nv + 7) / 8 + kStencilBlock - 1) / kStencilBlock * kStencilBlock;
     const int64_t runs = band / kStencilBlock;
@@ -1272,7 +1272,12 @@ static void launch_stencil(fk_mesh *m, PcgWork<T> &w, int parity) {
 template <typename T>
 static void launch_spmv_lanes(fk_mesh *m, PcgWork<T> &w, int parity) {
     if (m->kind == FK_SPMV_STENCIL) {
-        launch_stencil<T, kStencilBlock>(m, w, parity);  // measured best of 256/512/1024 at n=160
+        // FP32: the short-lived blocks of stencil_blocks; FP64: 1024 blocks
+        // of 512 threads (8 waves per SIMD for its guarded loads)
+        if (sizeof(T) == 4)
+            launch_stencil<T, kStencilBlock>(m, w, parity);
+        else
+            launch_stencil<T, 512>(m, w, parity);
         return;
     }
     if (m->format != FK_SPMV_CSR) {
