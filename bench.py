@@ -540,7 +540,7 @@ def main():
             "bound": "hbm",
             "kernel": {"csr": "k_cg_spmv<float,%d>" % args.lanes,
                        "sell64": "k_cg_spmv_sell<float,int16,%d,%d>" % shape,
-                       "stencil": "k_cg_spmv_stencil<float,512>"}[fmt_name],
+                       "stencil": "k_cg_spmv_stencil<float,256>"}[fmt_name],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -796,10 +796,11 @@ static int solver_config(const fk_mesh *m) {
     return m->lanes + 100 * m->format + 1000 * m->sell_unroll + 10000 * m->sell_block;
 }
 
-// Blocks of the FP32 stencil SpMV: two runs of 512 rows per block in each
-// XCD's band. Measured at n=160 (4.17M rows): 26.7 us against 35.9 us for the
-// 1024-block grid of the other kernels, whose blocks sweep 8 runs each and
-// drift apart, so the rows one grid plane back have left L2 more often.
+// Blocks of the FP32 stencil SpMV: two runs of 256 rows per block in each
+// XCD's band (8152 blocks at n=160, 4.17M rows). Measured there: 26.2 us,
+// against 27.8 us with runs of 512 rows and 35.9 us for the 1024-block grid
+// of the other kernels, whose blocks sweep 8 runs each and drift apart, so
+// the rows one grid plane back have left L2 more often.
 constexpr int kStencilBlock = 256;
 static int stencil_blocks(const fk_mesh *m) {
     const int64_t band = ((m->nv + 7) / 8 + kStencilBlock - 1) / kStencilBlock * kStencilBlock;
