@@ -103,8 +103,10 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
 // Sum of n block partials, same order in every block. Four independent
 // accumulators per thread keep four loads in flight (measured 2 us less per
 // launch than one accumulator for 4080 partials).
+// One thread's share of a fixed-order sum of block partials: four
+// accumulators over a stride of BLOCK, combined as (s0 + s1) + (s2 + s3).
 template <int BLOCK>
-__device__ __forceinline__ double sum_partials(const double *__restrict__ part, int n, double *sh) {
+__device__ __forceinline__ double thread_partials(const double *__restrict__ part, int n) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int i = threadIdx.x;
     for (; i + 3 * BLOCK < n; i += 4 * BLOCK) {
@@ -114,13 +116,13 @@ __device__ __forceinline__ double sum_partials(const double *__restrict__ part, 
         s3 += part[i + 3 * BLOCK];
     }
     for (; i < n; i += BLOCK) s0 += part[i];
-    return block_sum<BLOCK>((s0 + s1) + (s2 + s3), sh);
+    return (s0 + s1) + (s2 + s3);
 }
 
 // The same for a long partial array (16-byte aligned): pairs of partials per
 // load, four pairs in flight per thread.
 template <int BLOCK>
-__device__ __forceinline__ double sum_partials_wide(const double *__restrict__ part, int n, double *sh) {
+__device__ __forceinline__ double thread_partials_wide(const double *__restrict__ part, int n) {
     const double2 *__restrict__ p2 = reinterpret_cast<const double2 *>(part);
     const int n2 = n >> 1;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -137,7 +139,40 @@ __device__ __forceinline__ double sum_partials_wide(const double *__restrict__ p
         s0 += a.x + a.y;
     }
     if ((n & 1) && threadIdx.x == 0) s1 += part[n - 1];
-    return block_sum<BLOCK>((s0 + s1) + (s2 + s3), sh);
+    return (s0 + s1) + (s2 + s3);
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double sum_partials(const double *__restrict__ part, int n, double *sh) {
+    return block_sum<BLOCK>(thread_partials<BLOCK>(part, n), sh);
+}
+
+// block_sum of N values in one LDS round: each value is reduced in the very
+// order block_sum uses (wave sums, then thread k adds the BLOCK/64 wave sums
+// of value k from 0.0), so the results are bit-identical to N block_sum
+// calls. sh holds N * (BLOCK/64 + 1) doubles.
+template <int BLOCK, int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double *sh) {
+    constexpr int W = BLOCK / 64;
+    static_assert(N <= 64, "one thread per value");
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double s = wave_sum(v[k]);
+        if (l == 0) sh[k * (W + 1) + w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        double *row = sh + threadIdx.x * (W + 1);
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) s += row[i];
+        row[W] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = sh[k * (W + 1) + W];
+    __syncthreads();
 }
 
 // Blocks are handed to the 8 XCDs round-robin; give each XCD a contiguous

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ r, cons
                                                          double *__restrict__ part_rr, int64_t n, int64_t chunk,
                                                          int nblocks, CgState *__restrict__ st, int parity) {
     if (st->done_at[parity]) return;
-    __shared__ double sh[kBlock / 64 + 1];
+    __shared__ double sh[2 * (kBlock / 64 + 1)];
     const int lb = logical_block(nblocks);
     const int64_t r0 = static_cast<int64_t>(lb) * chunk;  // chunk is a multiple of 64
     const int64_t r1 = min(n, r0 + chunk);
@@ -141,8 +141,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ r, cons
             R[u] = reinterpret_cast<const V *>(r)[j];
         }
     }
-    const double rho = sum_partials<kBlock>(part_rz_cur, nblocks, sh);
-    const double pq = sum_partials_wide<kBlock>(part_pq, npq, sh);
+    // both reductions in one pass: their loads in flight together
+    double red[2] = {thread_partials<kBlock>(part_rz_cur, nblocks), thread_partials_wide<kBlock>(part_pq, npq)};
+    block_sum_n<kBlock, 2>(red, sh);
+    const double rho = red[0], pq = red[1];
     if (!(pq > 0.0) || !isfinite(pq)) {
         // breakdown: A not positive definite on the Krylov space (every
         // block sees the same pq and skips the update; K3 then adds nothing
@@ -189,11 +191,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ r, cons
         rz += static_cast<double>(ri) * static_cast<double>(z);
         rr += static_cast<double>(ri) * static_cast<double>(ri);
     }
-    const double trz = block_sum<kBlock>(rz, sh);
-    const double trr = block_sum<kBlock>(rr, sh);
+    double tot[2] = {rz, rr};
+    block_sum_n<kBlock, 2>(tot, sh);
     if (threadIdx.x == 0) {
-        part_rz_next[blockIdx.x] = trz;
-        part_rr[blockIdx.x] = trr;
+        part_rz_next[blockIdx.x] = tot[0];
+        part_rr[blockIdx.x] = tot[1];
     }
 }
 
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ x, T *__
                                                         int64_t chunk, int nblocks, CgState *__restrict__ st,
                                                         int parity) {
     if (st->done_at[parity]) return;
-    __shared__ double sh[kBlock / 64 + 1];
+    __shared__ double sh[3 * (kBlock / 64 + 1)];
     const int lb = logical_block(nblocks);
     const int64_t r0 = static_cast<int64_t>(lb) * chunk;  // chunk is a multiple of 64
     const int64_t r1 = min(n, r0 + chunk);
@@ -228,10 +230,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ x, T *__
         }
     }
     const T alpha = static_cast<T>(st->alpha);
-    const double rho_new = sum_partials<kBlock>(part_rz_next, nblocks, sh);
-    const double rho = sum_partials<kBlock>(part_rz_cur, nblocks, sh);
+    // the three reductions in one pass (r.r only in block 0, which keeps
+    // the convergence state)
+    double red[3] = {thread_partials<kBlock>(part_rz_next, nblocks), thread_partials<kBlock>(part_rz_cur, nblocks),
+                     blockIdx.x == 0 ? thread_partials<kBlock>(part_rr, nblocks) : 0.0};
+    block_sum_n<kBlock, 3>(red, sh);
+    const double rho_new = red[0], rho = red[1];
     if (blockIdx.x == 0) {
-        const double rr = sum_partials<kBlock>(part_rr, nblocks, sh);
+        const double rr = red[2];
         if (threadIdx.x == 0) {
             const int64_t it = st->iters + 1;
             st->iters = it;
