@@ -148,6 +148,77 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_gather(
     for (int32_t k = threadIdx.x; k < seg; k += kGatherRows) val[base + k] = acc[k];
 }
 
+// Newton-step assembly of the nonlinear loop in row-gather form (the
+// incidence lists of k_assemble_gather): row i adds, over its elements e
+// (local index la), K_e(mu)[la][b] = VE mu_e g_la.g_b and, on a Newton
+// step, the tangent VE mu'_e/|H_e| (H_e.g_la)(H_e.g_b) of
+// src/FEM3D.jl:79-119, with H_e = -grad(phi); and sums the element residual
+// -sum_b K_e(mu)[la][b] phi_b into r[i]. Values and residual are written
+// once (no zero fill); g_la.g_b and (H.g_la)(H.g_b) are formed in an order
+// that does not depend on which of the two rows computes them, so the matrix
+// is exactly symmetric. mu and dmu are in caller element order.
+__global__ __launch_bounds__(kGatherRows) void k_assemble_newton_gather(
+    const int32_t *__restrict__ rowptr, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
+    const uint32_t *__restrict__ inc, const uint32_t *__restrict__ off, const int4 *__restrict__ tet,
+    const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+    const double *__restrict__ mu, const double *__restrict__ dmu, const double *__restrict__ phi, int newton,
+    int64_t nv, double *__restrict__ val, double *__restrict__ r) {
+    extern __shared__ double acc[];
+    const int64_t i0 = static_cast<int64_t>(logical_block(static_cast<int>(gridDim.x))) * kGatherRows;
+    const int64_t i = i0 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t s = i >> 6;  // uniform over the wave
+    const int32_t base = rowptr[i0];
+    const int32_t seg = rowptr[min(i0 + kGatherRows, nv)] - base;
+    for (int32_t k = threadIdx.x; k < seg; k += kGatherRows) acc[k] = 0.0;
+    __syncthreads();
+    double res = 0.0;
+    if (s * 64 < nv) {
+        double *a = acc + (rowptr[min(i, nv)] - base);
+        const int64_t sb = sptr[s] + lane;
+        const int W = width[s];
+        for (int k = 0; k < W; ++k) {
+            const uint32_t w = inc[sb + 64 * k];
+            if (w == kNoInc) continue;
+            const uint32_t o = off[sb + 64 * k];
+            const int64_t e = w >> 2;
+            const int la = static_cast<int>(w & 3u);
+            const int4 nd = tet[e];
+            const Grad G = element_gradients(X, Y, Z, nd);
+            const double vol = fabs(G.det) * (1.0 / 6.0);
+            const double wk = vol * mu[e];
+            const double u[4] = {phi[nd.x], phi[nd.y], phi[nd.z], phi[nd.w]};
+            double gl[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                gl[c] = la == 0 ? G.g[0][c] : la == 1 ? G.g[1][c] : la == 2 ? G.g[2][c] : G.g[3][c];
+            double kv[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) kv[b] = wk * dot3(gl, G.g[b]);
+            res -= kv[0] * u[0] + kv[1] * u[1] + kv[2] * u[2] + kv[3] * u[3];
+            if (newton) {
+                double h[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    h[c] = -(u[0] * G.g[0][c] + u[1] * G.g[1][c] + u[2] * G.g[2][c] + u[3] * G.g[3][c]);
+                const double hn = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+                const double wt = hn > 0.0 ? vol * dmu[e] / hn : 0.0;
+                double hg[4];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) hg[b] = dot3(h, G.g[b]);
+                const double hl = la == 0 ? hg[0] : la == 1 ? hg[1] : la == 2 ? hg[2] : hg[3];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) kv[b] += wt * (hl * hg[b]);
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) a[(o >> (8 * b)) & 255u] += kv[b];
+        }
+    }
+    __syncthreads();
+    for (int32_t k = threadIdx.x; k < seg; k += kGatherRows) val[base + k] = acc[k];
+    if (i < nv) r[i] = res;
+}
+
 // coef_nat[perm[e]] = coef[e] (colour order -> caller order)
 __global__ void k_coef_to_caller(const double *__restrict__ coef, const int32_t *__restrict__ perm, int64_t nt,
                                  double *__restrict__ coef_nat) {
@@ -334,6 +405,31 @@ void build_gather(fk_mesh *m) {
     FK_HIP(hipStreamSynchronize(st));
     m->gather_lds = static_cast<int32_t>(seg_max);
     m->gather_ok = true;
+}
+
+// Enqueue the nonlinear loop's Newton-step assembly in row-gather form into
+// val64 and r (mu, dmu in colour order, converted here). Returns false, with
+// nothing enqueued, when the mesh's rows are too long for the LDS segment;
+// the caller then runs the coloured element kernel.
+bool assemble_newton_gather(fk_mesh *m, const double *mu, const double *dmu, const double *phi, int newton,
+                            double *r) {
+    build_gather(m);
+    if (!m->gather_ok) return false;
+    hipStream_t st = m->ctx->stream;
+    m->coef_nat.ensure(m->nt);
+    m->dmu_nat.ensure(m->nt);
+    k_coef_to_caller<<<grid_for(m->nt, 256), 256, 0, st>>>(mu, m->perm.get(), m->nt, m->coef_nat.get());
+    if (newton)
+        k_coef_to_caller<<<grid_for(m->nt, 256), 256, 0, st>>>(dmu, m->perm.get(), m->nt, m->dmu_nat.get());
+    m->coef_nat_valid = false;  // the loop rewrites coef right after this assembly
+    const int64_t nb = (m->nv + kGatherRows - 1) / kGatherRows;
+    k_assemble_newton_gather<<<static_cast<unsigned>(nb), kGatherRows,
+                               static_cast<size_t>(std::max(m->gather_lds, 1)) * sizeof(double), st>>>(
+        m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->inc_off.get(), m->tet_nat.get(),
+        m->X.get(), m->Y.get(), m->Z.get(), m->coef_nat.get(), m->dmu_nat.get(), phi, newton, m->nv,
+        m->val64.get(), r);
+    FK_HIP(hipGetLastError());
+    return true;
 }
 
 // Enqueue the value assembly on the context stream (no host sync).

@@ -242,6 +242,7 @@ struct fk_mesh {
     fk::DevBuf<uint32_t> inc, inc_off;
     fk::DevBuf<int4> tet_nat;
     fk::DevBuf<double> coef_nat;      // coefficient in caller order
+    fk::DevBuf<double> dmu_nat;       // dmu in caller order (gather Newton assembly)
     bool coef_nat_valid = false;
 
     int lanes = 16;                   // lanes per CSR row in SpMV
@@ -261,6 +262,8 @@ void set_device(fk_context *ctx);
 void build_mesh(fk_mesh *m, const double *p, const int32_t *t, int32_t base, const double *VE);
 void element_volumes(fk_mesh *m, double *VE_host);
 void assemble(fk_mesh *m, int method);
+bool assemble_newton_gather(fk_mesh *m, const double *mu, const double *dmu, const double *phi, int newton,
+                            double *r);
 void make_val32(fk_mesh *m);
 void local_stiffness(fk_mesh *m, const double *f, double *Ak_host);
 void tangential(fk_mesh *m, const double *H, const double *dmu, double *nzval_host);

@@ -333,14 +333,16 @@ static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const La
     PcgWork<double> &w = m->w64;
     for (it = 1; it <= max_outer; ++it) {
         const int newton = dev < switch_tol ? 1 : 0;
-        k_fill_f64b<<<grid_for(m->nnz, 256), 256, 0, st>>>(m->val64.get(), m->nnz, 0.0);
-        k_fill_f64b<<<grid_for(nv, 256), 256, 0, st>>>(w.r.get(), nv, 0.0);
-        for (int c = 0; c < m->ncolours; ++c) {
-            const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
-            if (e1 <= e0) continue;
-            k_assemble_newton<<<grid_for(e1 - e0, 256), 256, 0, st>>>(
-                m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->coef.get(), m->dmu.get(), m->phi.get(),
-                m->slot.get(), m->val64.get(), w.r.get(), newton, e0, e1);
+        if (!assemble_newton_gather(m, m->coef.get(), m->dmu.get(), m->phi.get(), newton, w.r.get())) {
+            k_fill_f64b<<<grid_for(m->nnz, 256), 256, 0, st>>>(m->val64.get(), m->nnz, 0.0);
+            k_fill_f64b<<<grid_for(nv, 256), 256, 0, st>>>(w.r.get(), nv, 0.0);
+            for (int c = 0; c < m->ncolours; ++c) {
+                const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
+                if (e1 <= e0) continue;
+                k_assemble_newton<<<grid_for(e1 - e0, 256), 256, 0, st>>>(
+                    m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->coef.get(), m->dmu.get(), m->phi.get(),
+                    m->slot.get(), m->val64.get(), w.r.get(), newton, e0, e1);
+            }
         }
         FK_HIP(hipGetLastError());
         values_changed(m);
