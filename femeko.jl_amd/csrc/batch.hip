@@ -105,10 +105,11 @@ __device__ __forceinline__ bool mesh_all_done(const int32_t *__restrict__ done, 
     return all;
 }
 
-// sum over the logical blocks [b0, b1) of part[b*nrhs + k], fixed order
+// sum over the logical blocks [b0, b1) of part[b*nrhs + k], fixed order:
+// one thread's share (two accumulators over a stride of BLOCK), then the
+// block sum
 template <int BLOCK>
-__device__ __forceinline__ double seg_sum(const double *__restrict__ part, int b0, int b1, int k, int nrhs,
-                                          double *sh) {
+__device__ __forceinline__ double seg_thread(const double *__restrict__ part, int b0, int b1, int k, int nrhs) {
     double s0 = 0.0, s1 = 0.0;
     int b = b0 + static_cast<int>(threadIdx.x);
     for (; b + BLOCK < b1; b += 2 * BLOCK) {
@@ -116,7 +117,12 @@ __device__ __forceinline__ double seg_sum(const double *__restrict__ part, int b
         s1 += part[(b + BLOCK) * nrhs + k];
     }
     for (; b < b1; b += BLOCK) s0 += part[b * nrhs + k];
-    return block_sum<BLOCK>(s0 + s1, sh);
+    return s0 + s1;
+}
+template <int BLOCK>
+__device__ __forceinline__ double seg_sum(const double *__restrict__ part, int b0, int b1, int k, int nrhs,
+                                          double *sh) {
+    return block_sum<BLOCK>(seg_thread<BLOCK>(part, b0, b1, k, nrhs), sh);
 }
 
 // K1: q = A p for all right-hand sides of the block's mesh, partials of p.q.
@@ -266,7 +272,7 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
                                                       double *__restrict__ part_rr, int32_t *__restrict__ done,
                                                       int32_t *__restrict__ status, int32_t *__restrict__ ndone,
                                                       float *__restrict__ alphas) {
-    __shared__ double sh[BLOCK / 64 + 1];
+    __shared__ double sh[2 * NR * (BLOCK / 64 + 1)];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
     if (mesh_all_done(done, mesh, NR)) {
@@ -286,12 +292,20 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
         }
     }
     const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
+    // the 2 NR per-mesh reductions in one pass (same order per value)
+    double red[2 * NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        red[j] = seg_thread<BLOCK>(rz_cur, b0, b1, j, NR);
+        red[NR + j] = seg_thread<BLOCK>(part_pq, pq_b0[mesh], pq_b1[mesh], j, NR);
+    }
+    block_sum_n<BLOCK, 2 * NR>(red, sh);
     float alpha[NR];
     bool act[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
-        const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, NR, sh);
-        const double pq = seg_sum<BLOCK>(part_pq, pq_b0[mesh], pq_b1[mesh], j, NR, sh);
+        const double rho = red[j];
+        const double pq = red[NR + j];
         const bool live = !done[mesh * NR + j];
         act[j] = live && pq > 0.0 && isfinite(pq);
         alpha[j] = act[j] ? static_cast<float>(rho / pq) : 0.f;
@@ -331,13 +345,18 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
         }
         row(i, d, rv, qv);
     }
+    double tot[2 * NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
-        const double a = block_sum<BLOCK>(rz[j], sh);
-        const double c = block_sum<BLOCK>(rr[j], sh);
-        if (threadIdx.x == 0) {
-            rz_next[lb * NR + j] = a;
-            part_rr[lb * NR + j] = c;
+        tot[j] = rz[j];
+        tot[NR + j] = rr[j];
+    }
+    block_sum_n<BLOCK, 2 * NR>(tot, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            rz_next[lb * NR + j] = tot[j];
+            part_rr[lb * NR + j] = tot[NR + j];
         }
     }
 }
@@ -358,7 +377,7 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
                                                      int32_t *__restrict__ status, int64_t *__restrict__ iters,
                                                      const double *__restrict__ thresh, int64_t max_iter,
                                                      int32_t *__restrict__ ndone, double *__restrict__ rr32) {
-    __shared__ double sh[BLOCK / 64 + 1];
+    __shared__ double sh[3 * NR * (BLOCK / 64 + 1)];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
     float alpha[NR];
@@ -384,15 +403,25 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
         }
     }
     const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
+    // the per-mesh reductions in one pass (r.r only in the mesh's first
+    // block, which keeps the convergence state)
+    double red[3 * NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        red[j] = seg_thread<BLOCK>(rz_next, b0, b1, j, NR);
+        red[NR + j] = seg_thread<BLOCK>(rz_cur, b0, b1, j, NR);
+        red[2 * NR + j] = lb == b0 ? seg_thread<BLOCK>(part_rr, b0, b1, j, NR) : 0.0;
+    }
+    block_sum_n<BLOCK, 3 * NR>(red, sh);
     float beta[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
         const int s = mesh * NR + j;
-        const double rho_new = seg_sum<BLOCK>(rz_next, b0, b1, j, NR, sh);
-        const double rho = seg_sum<BLOCK>(rz_cur, b0, b1, j, NR, sh);
+        const double rho_new = red[j];
+        const double rho = red[NR + j];
         beta[j] = upd[j] ? static_cast<float>(rho_new / rho) : 0.f;
         if (lb == b0) {
-            const double rr = seg_sum<BLOCK>(part_rr, b0, b1, j, NR, sh);
+            const double rr = red[2 * NR + j];
             if (threadIdx.x == 0 && upd[j]) {
                 const int64_t it = iters[s] + 1;
                 iters[s] = it;
