@@ -321,18 +321,20 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
     double rz[NR], rr[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) rz[j] = rr[j] = 0.0;
+    // all NR values of a node are stored together (one vector store; a
+    // system K2 does not step stores its value unchanged)
     auto row = [&](int64_t i, float d, float *rv, const float *qv) {
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             float ri = rv[j];
-            if (act[j]) {
-                ri = d != 0.f ? ri - alpha[j] * qv[j] : 0.f;
-                r[i * NR + j] = ri;
-            }
+            if (act[j]) ri = d != 0.f ? ri - alpha[j] * qv[j] : 0.f;
+            rv[j] = ri;
             const float z = d * ri;
             rz[j] += static_cast<double>(ri) * static_cast<double>(z);
             rr[j] += static_cast<double>(ri) * static_cast<double>(ri);
         }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) r[i * NR + j] = rv[j];
     };
     if (i0 < r1) row(i0, d0, r0v, q0);
     for (int64_t i = i0 + BLOCK; i < r1; i += BLOCK) {
@@ -443,23 +445,32 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
             }
         }
     }
-    if (i0 < r1) {
+    // all NR values of a node are loaded and stored together (vector
+    // accesses; a system K2 did not step keeps its values)
+    auto row = [&](int64_t i, float d, const float *rv, float *pv, float *xv) {
 #pragma unroll
         for (int j = 0; j < NR; ++j)
             if (upd[j]) {
-                x[i0 * NR + j] = xv0[j] + alpha[j] * pv0[j];
-                p[i0 * NR + j] = d0 * rv0[j] + beta[j] * pv0[j];
+                xv[j] += alpha[j] * pv[j];
+                pv[j] = d * rv[j] + beta[j] * pv[j];
             }
-    }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            x[i * NR + j] = xv[j];
+            p[i * NR + j] = pv[j];
+        }
+    };
+    if (i0 < r1) row(i0, d0, rv0, pv0, xv0);
     for (int64_t i = i0 + BLOCK; i < r1; i += BLOCK) {
         const float d = dinv[i];
+        float rv[NR], pv[NR], xv[NR];
 #pragma unroll
-        for (int j = 0; j < NR; ++j)
-            if (upd[j]) {
-                const float pj = p[i * NR + j];
-                x[i * NR + j] += alpha[j] * pj;
-                p[i * NR + j] = d * r[i * NR + j] + beta[j] * pj;
-            }
+        for (int j = 0; j < NR; ++j) {
+            rv[j] = r[i * NR + j];
+            pv[j] = p[i * NR + j];
+            xv[j] = x[i * NR + j];
+        }
+        row(i, d, rv, pv, xv);
     }
 }
 
