@@ -196,7 +196,7 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm_stencil(BatchArgs ba, const int
                                                          const float *__restrict__ val, const float *__restrict__ p,
                                                          float *__restrict__ q, double *__restrict__ part,
                                                          const int32_t *__restrict__ done) {
-    __shared__ double sh[BLOCK / 64 + 1];
+    __shared__ double sh[NR * (BLOCK / 64 + 1)];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
     if (mesh_all_done(done, mesh, NR)) return;
@@ -242,10 +242,10 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm_stencil(BatchArgs ba, const int
             acc[j] += static_cast<double>(xi[j]) * static_cast<double>(s[j]);
         }
     }
+    block_sum_n<BLOCK, NR>(acc, sh);  // the NR sums in one LDS round
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int j = 0; j < NR; ++j) {
-        const double t = block_sum<BLOCK>(acc[j], sh);
-        if (threadIdx.x == 0) part[lb * NR + j] = t;
+        for (int j = 0; j < NR; ++j) part[lb * NR + j] = acc[j];
     }
 }
 
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(BLOCK) void kb_neg_residual_stencil(
     const int32_t *__restrict__ offs, const int64_t *__restrict__ vbase, const double *__restrict__ val,
     const double *__restrict__ diag, const double *__restrict__ phi, const uint8_t *__restrict__ mask,
     double *__restrict__ r, double *__restrict__ part, const int32_t *__restrict__ gate) {
-    __shared__ double sh[BLOCK / 64 + 1];
+    __shared__ double sh[NR * (BLOCK / 64 + 1)];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
     if (gate) {
@@ -585,10 +585,10 @@ __global__ __launch_bounds__(BLOCK) void kb_neg_residual_stencil(
             acc[j] += rv * rv;
         }
     }
+    block_sum_n<BLOCK, NR>(acc, sh);  // the NR sums in one LDS round
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int j = 0; j < NR; ++j) {
-        const double t = block_sum<BLOCK>(acc[j], sh);
-        if (threadIdx.x == 0) part[lb * NR + j] = t;
+        for (int j = 0; j < NR; ++j) part[lb * NR + j] = acc[j];
     }
 }
 
