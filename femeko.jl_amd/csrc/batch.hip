@@ -210,25 +210,34 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm_stencil(BatchArgs ba, const int
     double acc[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
-        const int32_t li = static_cast<int32_t>(i - a);
-        float xi[NR], s[NR];
+    // two rows per thread and step (the second clamped to the first past the
+    // block's range, so both rows' loads are issued together)
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 2 * BLOCK) {
+        const bool two = i + BLOCK < r1;
+        const int64_t ib = two ? i + BLOCK : i;
+        const int32_t li[2] = {static_cast<int32_t>(i - a), static_cast<int32_t>(ib - a)};
+        float xi[2][NR], s[2][NR];
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            xi[j] = p[i * NR + j];
-            s[j] = 0.f;
+            xi[0][j] = p[i * NR + j];
+            xi[1][j] = p[ib * NR + j];
+            s[0][j] = 0.f;
+            s[1][j] = 0.f;
         }
         auto term = [&](int k, int32_t o) {
             const float *__restrict__ v = v0 + static_cast<int64_t>(k) * n;
-            const float wu = v[li];
-            const int32_t lu = min(li + o, static_cast<int32_t>(n - 1));
-            const bool lo_ok = li >= o;
-            const float wl = lo_ok ? v[lo_ok ? li - o : 0] : 0.f;
-            const int32_t ll = lo_ok ? li - o : 0;
-            const float *pu = p + (a + lu) * NR;
-            const float *pl = p + (a + ll) * NR;
 #pragma unroll
-            for (int j = 0; j < NR; ++j) s[j] += wu * (pu[j] - xi[j]) + wl * (pl[j] - xi[j]);
+            for (int h = 0; h < 2; ++h) {
+                const float wu = v[li[h]];
+                const int32_t lu = min(li[h] + o, static_cast<int32_t>(n - 1));
+                const bool lo_ok = li[h] >= o;
+                const float wl = lo_ok ? v[lo_ok ? li[h] - o : 0] : 0.f;
+                const int32_t ll = lo_ok ? li[h] - o : 0;
+                const float *pu = p + (a + lu) * NR;
+                const float *pl = p + (a + ll) * NR;
+#pragma unroll
+                for (int j = 0; j < NR; ++j) s[h][j] += wu * (pu[j] - xi[h][j]) + wl * (pl[j] - xi[h][j]);
+            }
         };
         if constexpr (KC > 0) {
 #pragma unroll
@@ -238,8 +247,15 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm_stencil(BatchArgs ba, const int
         }
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            q[i * NR + j] = s[j];
-            acc[j] += static_cast<double>(xi[j]) * static_cast<double>(s[j]);
+            q[i * NR + j] = s[0][j];
+            acc[j] += static_cast<double>(xi[0][j]) * static_cast<double>(s[0][j]);
+        }
+        if (two) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                q[ib * NR + j] = s[1][j];
+                acc[j] += static_cast<double>(xi[1][j]) * static_cast<double>(s[1][j]);
+            }
         }
     }
     block_sum_n<BLOCK, NR>(acc, sh);  // the NR sums in one LDS round
