@@ -579,18 +579,37 @@ __global__ __launch_bounds__(BLOCK) void kb_neg_residual_stencil(
         const double d = diag[i];
 #pragma unroll
         for (int j = 0; j < NR; ++j) y[j] = d * phi[i * NR + j];
-        for (int k = 0; k < K; ++k) {
-            const int64_t o = offs[16 * mesh + k];
-            const double *__restrict__ v = v0 + static_cast<int64_t>(k) * n;
-            if (li + o < n) {
-                const double w = v[li];
+        if constexpr (KC > 0) {
+            // unconditional loads (indices clamped into the mesh): column k
+            // holds 0 where li + o is past the mesh and a backward term out
+            // of range is weighted 0, so each y gains exactly the guarded
+            // form's terms in the same order
 #pragma unroll
-                for (int j = 0; j < NR; ++j) y[j] += w * phi[(i + o) * NR + j];
+            for (int k = 0; k < KC; ++k) {
+                const int64_t o = offs[16 * mesh + k];
+                const double *__restrict__ v = v0 + static_cast<int64_t>(k) * n;
+                const int64_t lu = min(li + o, n - 1), ll = max(li - o, int64_t(0));
+                const double wu = v[li];
+                const double wl = li >= o ? v[ll] : 0.0;
+#pragma unroll
+                for (int j = 0; j < NR; ++j) y[j] += wu * phi[(a + lu) * NR + j];
+#pragma unroll
+                for (int j = 0; j < NR; ++j) y[j] += wl * phi[(a + ll) * NR + j];
             }
-            if (li >= o) {
-                const double w = v[li - o];
+        } else {
+            for (int k = 0; k < K; ++k) {
+                const int64_t o = offs[16 * mesh + k];
+                const double *__restrict__ v = v0 + static_cast<int64_t>(k) * n;
+                if (li + o < n) {
+                    const double w = v[li];
 #pragma unroll
-                for (int j = 0; j < NR; ++j) y[j] += w * phi[(i - o) * NR + j];
+                    for (int j = 0; j < NR; ++j) y[j] += w * phi[(i + o) * NR + j];
+                }
+                if (li >= o) {
+                    const double w = v[li - o];
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) y[j] += w * phi[(i - o) * NR + j];
+                }
             }
         }
         const bool fixed = mask[i] != 0;
