@@ -76,21 +76,43 @@ __global__ void k_div_volumes(double *__restrict__ Hd, const double *__restrict_
     if (t < 3 * nv) Hd[t] = Hd[t] / vol[t / 3];
 }
 
+// input checks on the device: bad[0] counts non-finite magnetization
+// entries, bad[1] non-finite or negative nodal volumes
+__global__ void k_demag_check(const double *__restrict__ M, const double *__restrict__ vol, int64_t nv,
+                              int32_t *__restrict__ bad) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < 3 * nv && !isfinite(M[t])) atomicAdd(bad, 1);
+    if (t < nv && !(isfinite(vol[t]) && vol[t] >= 0.0)) atomicAdd(bad + 1, 1);
+}
+
 void demag_field(fk_mesh *m, const uint8_t *inside_host, const double *M_host, const double *vol_host, double rtol,
                  int64_t max_iter, int warm, double *Hd_host, int64_t *iters_out, double *relres_out) {
     if (!m->have_values) fail(FK_ERR_STATE, "matrix not assembled (A = stiffnessMatrix(mesh))");
     const int64_t nv = m->nv, nt = m->nt;
-    for (int64_t i = 0; i < 3 * nv; ++i)
-        if (!std::isfinite(M_host[i])) fail(FK_ERR_NONFINITE, "non-finite magnetization");
-    for (int64_t i = 0; i < nv; ++i)
-        if (!std::isfinite(vol_host[i]) || vol_host[i] < 0.0) fail(FK_ERR_INVALID_ARG, "nodal volumes");
     hipStream_t st = m->ctx->stream;
     ensure_work_f64(m);
-    DevBuf<uint8_t> in_caller(nt), inside(nt);
-    DevBuf<double> M(3 * nv), vol(nv), Hd(3 * nv);
+    m->demag_in_caller.ensure(nt);
+    m->demag_inside.ensure(nt);
+    m->demag_M.ensure(3 * nv);
+    m->demag_vol.ensure(nv);
+    m->demag_Hd.ensure(3 * nv);
+    DevBuf<uint8_t> &in_caller = m->demag_in_caller, &inside = m->demag_inside;
+    DevBuf<double> &M = m->demag_M, &vol = m->demag_vol, &Hd = m->demag_Hd;
     FK_HIP(hipMemcpyAsync(in_caller.get(), inside_host, nt, hipMemcpyHostToDevice, st));
     FK_HIP(hipMemcpyAsync(M.get(), M_host, 3 * nv * sizeof(double), hipMemcpyHostToDevice, st));
     FK_HIP(hipMemcpyAsync(vol.get(), vol_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
+    {
+        // checked before anything the call keeps (the previous potential) changes
+        DevBuf<int32_t> bad(2);
+        FK_HIP(hipMemsetAsync(bad.get(), 0, 2 * sizeof(int32_t), st));
+        k_demag_check<<<grid_for(3 * nv, 256), 256, 0, st>>>(M.get(), vol.get(), nv, bad.get());
+        FK_HIP(hipGetLastError());
+        int32_t h[2] = {0, 0};
+        FK_HIP(hipMemcpyAsync(h, bad.get(), 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        if (h[0]) fail(FK_ERR_NONFINITE, "non-finite magnetization");
+        if (h[1]) fail(FK_ERR_INVALID_ARG, "nodal volumes");
+    }
     k_mask_to_colour<<<grid_for(nt, 256), 256, 0, st>>>(in_caller.get(), m->perm.get(), nt, inside.get());
     double *rhs = m->w64.r.get();
     FK_HIP(hipMemsetAsync(rhs, 0, nv * sizeof(double), st));

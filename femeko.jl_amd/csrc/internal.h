@@ -184,6 +184,11 @@ struct fk_mesh {
     fk::DevBuf<double> hn_prev;       // |H| of the previous step (colour order), tabulated-law loop
     fk::DevBuf<double> demag_u;       // potential of the last demagnetizing-field call
     bool have_demag_u = false;
+    // buffers of the demagnetizing-field call, kept across calls
+    fk::DevBuf<uint8_t> demag_in_caller, demag_inside;
+    fk::DevBuf<double> demag_M, demag_vol, demag_Hd;
+    fk::DevBuf<double> lag;           // Lagrange vector C_i = sum VE/4 (geometry only)
+    bool have_lag = false;
     fk::DevBuf<uint8_t> region;       // nonlinear material region (colour order)
     fk::DevBuf<uint8_t> target;       // objective region of the adjoint (colour order)
     fk::DevBuf<double> val64;

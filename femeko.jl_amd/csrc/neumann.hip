@@ -176,16 +176,20 @@ NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter, const
         m->dinv64_valid = false;  // the Dirichlet solves recompute theirs
         if (h) fail(FK_ERR_SINGULAR, std::to_string(h) + " node(s) without a positive diagonal");
     }
-    // Lag and b
-    DevBuf<double> lag(nv);
-    FK_HIP(hipMemsetAsync(lag.get(), 0, nv * sizeof(double), st));
-    for (int c = 0; c < m->ncolours; ++c) {
-        const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
-        if (e1 <= e0) continue;
-        k_lagrange_n<<<grid_for(e1 - e0, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
-                                                             lag.get(), e0, e1);
+    // Lag (geometry only: built once per mesh and kept) and b
+    if (!m->have_lag) {
+        m->lag.ensure(nv);
+        FK_HIP(hipMemsetAsync(m->lag.get(), 0, nv * sizeof(double), st));
+        for (int c = 0; c < m->ncolours; ++c) {
+            const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
+            if (e1 <= e0) continue;
+            k_lagrange_n<<<grid_for(e1 - e0, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
+                                                                 m->lag.get(), e0, e1);
+        }
+        FK_HIP(hipGetLastError());
+        m->have_lag = true;
     }
-    FK_HIP(hipGetLastError());
+    const DevBuf<double> &lag = m->lag;
     double s[3];
     sums3(m, w.r.get(), lag.get(), s);  // sum b, sum Lag
     const double mult = s[1] != 0.0 ? s[0] / s[1] : 0.0;

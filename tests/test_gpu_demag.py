@@ -90,3 +90,42 @@ def test_demag_warm_start_same_field_fewer_iterations(oracle):
         assert max(warm) < min(cold)
     finally:
         dm.close()
+
+
+def test_demag_rejects_bad_input_and_keeps_state(oracle):
+    # fk_demag_field checks its inputs (on the device, after the upload)
+    # before anything it keeps changes: a non-finite M is FK_ERR_NONFINITE, a
+    # negative or non-finite nodal volume FK_ERR_INVALID_ARG; the warm start
+    # of the next good call then still starts from the last good potential
+    from femeko_hip import _lib
+    from femeko_hip._lib import FemekoHIPError
+    pr = W.linear(8)
+    m = pr.mesh
+    inside = pr.magnet
+    vol = oracle.node_volumes(m, inside)
+    M = np.zeros((3, m.nv))
+    M[2, vol > 0] = 1.0
+    dm = _device_A(m)
+    try:
+        Hd0, _its, _rel = dm.demag_field(inside.astype(np.uint8), M, vol, rtol=1e-12)
+        bad = M.copy()
+        bad[1, 5] = np.nan
+        with pytest.raises(FemekoHIPError) as e:
+            dm.demag_field(inside.astype(np.uint8), bad, vol)
+        assert e.value.status == _lib.FK_ERR_NONFINITE
+        bad[1, 5] = np.inf
+        with pytest.raises(FemekoHIPError) as e:
+            dm.demag_field(inside.astype(np.uint8), bad, vol)
+        assert e.value.status == _lib.FK_ERR_NONFINITE
+        for v in (-1.0, np.nan):
+            vb = vol.copy()
+            vb[3] = v
+            with pytest.raises(FemekoHIPError) as e:
+                dm.demag_field(inside.astype(np.uint8), M, vb)
+            assert e.value.status == _lib.FK_ERR_INVALID_ARG
+        Hd1, its1, _rel = dm.demag_field(inside.astype(np.uint8), M, vol, rtol=1e-12, warm=True)
+        assert its1 <= 2  # started from the potential of the same M
+        ok = ~np.isnan(Hd0)
+        np.testing.assert_allclose(Hd1[ok], Hd0[ok], rtol=0, atol=1e-9 * np.abs(Hd0[ok]).max())
+    finally:
+        dm.close()
