@@ -113,6 +113,7 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_gather(
     // XCD-aware: each XCD takes a contiguous eighth of the rows, so the
     // elements shared by rows one grid line or plane apart stay in its L2
     const int64_t i0 = static_cast<int64_t>(logical_block(static_cast<int>(gridDim.x))) * kGatherRows;
+    if (i0 >= nv) return;  // padding of the grid to a multiple of 8 (whole block)
     const int64_t i = i0 + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t s = i >> 6;  // uniform over the wave
@@ -165,6 +166,7 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_newton_gather(
     int64_t nv, double *__restrict__ val, double *__restrict__ r) {
     extern __shared__ double acc[];
     const int64_t i0 = static_cast<int64_t>(logical_block(static_cast<int>(gridDim.x))) * kGatherRows;
+    if (i0 >= nv) return;  // padding of the grid to a multiple of 8 (whole block)
     const int64_t i = i0 + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t s = i >> 6;  // uniform over the wave
@@ -217,6 +219,14 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_newton_gather(
     __syncthreads();
     for (int32_t k = threadIdx.x; k < seg; k += kGatherRows) val[base + k] = acc[k];
     if (i < nv) r[i] = res;
+}
+
+// Blocks of the row-gather kernels: one per kGatherRows rows, padded to a
+// multiple of 8 so that logical_block gives each XCD a contiguous eighth of
+// the rows (the padding blocks return at once).
+static int64_t gather_blocks(int64_t nv) {
+    const int64_t nb = (nv + kGatherRows - 1) / kGatherRows;
+    return nb < 8 ? nb : (nb + 7) / 8 * 8;
 }
 
 // coef_nat[perm[e]] = coef[e] (colour order -> caller order)
@@ -422,7 +432,7 @@ bool assemble_newton_gather(fk_mesh *m, const double *mu, const double *dmu, con
     if (newton)
         k_coef_to_caller<<<grid_for(m->nt, 256), 256, 0, st>>>(dmu, m->perm.get(), m->nt, m->dmu_nat.get());
     m->coef_nat_valid = false;  // the loop rewrites coef right after this assembly
-    const int64_t nb = (m->nv + kGatherRows - 1) / kGatherRows;
+    const int64_t nb = gather_blocks(m->nv);
     k_assemble_newton_gather<<<static_cast<unsigned>(nb), kGatherRows,
                                static_cast<size_t>(std::max(m->gather_lds, 1)) * sizeof(double), st>>>(
         m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->inc_off.get(), m->tet_nat.get(),
@@ -446,7 +456,7 @@ void assemble(fk_mesh *m, int method) {
                                                                       m->coef_nat.get());
                 m->coef_nat_valid = true;
             }
-            const int64_t nb = (m->nv + kGatherRows - 1) / kGatherRows;
+            const int64_t nb = gather_blocks(m->nv);
             k_assemble_gather<<<static_cast<unsigned>(nb), kGatherRows,
                                 static_cast<size_t>(std::max(m->gather_lds, 1)) * sizeof(double), st>>>(
                 m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->inc_off.get(),
