@@ -957,7 +957,9 @@ static void upload_table(const TableHost &t, int nmesh, fk::DevBuf<int32_t> &bm,
     FK_HIP(hipMemcpy(b1.get(), t.b1.data(), nmesh * sizeof(int32_t), hipMemcpyHostToDevice));
 }
 
-// the table of every kernel but the stencil SpMM: ~4 blocks per CU overall
+// the table of every kernel but the stencil SpMM: ~4 blocks per CU overall.
+// Not padded to a multiple of 8, so logical_block keeps dispatch order: with
+// XCD bands (padded) the batch step measured 80.4 ms against 76.9 ms.
 static void build_block_table(fk_batch *b) {
     const int64_t total = b->node_off.back();
     const int64_t target_blocks = std::max<int64_t>(b->nmesh, 4 * static_cast<int64_t>(b->ctx->num_cus));
