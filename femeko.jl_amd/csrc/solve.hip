@@ -1025,19 +1025,29 @@ static void build_sell(fk_mesh *m) {
 // Stencil values from the FP64 CSR values, one thread per row: the row's
 // entries lie in one or two cache lines, read once for all K columns (and the
 // diagonal for the FP64 standard form).
+#ifndef FK_FILL_BLOCKS
+#define FK_FILL_BLOCKS 2048
+#endif
+constexpr int kFillBlocks = FK_FILL_BLOCKS;
 template <typename T>
 __global__ void k_stencil_fill(const double *__restrict__ val64, const int32_t *__restrict__ map,
                                const int32_t *__restrict__ diagpos, int K, int64_t ld, int64_t n, T *__restrict__ out,
                                double *__restrict__ diag64) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= ld) return;
-    for (int k = 0; k < K; ++k) {
-        const int32_t pos = map[k * ld + i];
-        out[k * ld + i] = pos >= 0 ? static_cast<T>(val64[pos]) : T(0);
-    }
-    if (diag64 && i < n) {
-        const int32_t d = diagpos[i];
-        diag64[i] = d >= 0 ? val64[d] : 0.0;
+    // bounded grid (a multiple of 8): the blocks of XCD x sweep its eighth of
+    // the rows together, so the CSR lines of the rows in flight fit its L2
+    const int x = blockIdx.x & 7, bx = blockIdx.x >> 3, nbx = gridDim.x >> 3;
+    const int64_t band = ((ld + 7) / 8 + 255) / 256 * 256;
+    const int64_t end = min(ld, (x + 1) * band);
+    for (int64_t i = x * band + static_cast<int64_t>(bx) * blockDim.x + threadIdx.x; i < end;
+         i += static_cast<int64_t>(nbx) * blockDim.x) {
+        for (int k = 0; k < K; ++k) {
+            const int32_t pos = map[k * ld + i];
+            out[k * ld + i] = pos >= 0 ? static_cast<T>(val64[pos]) : T(0);
+        }
+        if (diag64 && i < n) {
+            const int32_t d = diagpos[i];
+            diag64[i] = d >= 0 ? val64[d] : 0.0;
+        }
     }
 }
 
@@ -1148,7 +1158,7 @@ void sync_solver_values<double>(fk_mesh *m) {
         const int64_t len = m->st_K * m->st_ld;
         m->st_val64.ensure(len);
         m->sell_diag64.ensure(m->nv);
-        k_stencil_fill<double><<<grid_for(m->st_ld, 256), 256, 0, m->ctx->stream>>>(
+        k_stencil_fill<double><<<kFillBlocks, 256, 0, m->ctx->stream>>>(
             m->val64.get(), m->st_map.get(), m->diag.get(), m->st_K, m->st_ld, m->nv, m->st_val64.get(),
             m->sell_diag64.get());
         FK_HIP(hipGetLastError());
@@ -1172,7 +1182,7 @@ void sync_solver_values<float>(fk_mesh *m) {
     if (solver_kind(m) == FK_SPMV_STENCIL) {
         const int64_t len = m->st_K * m->st_ld;
         m->st_val32.ensure(len);
-        k_stencil_fill<float><<<grid_for(m->st_ld, 256), 256, 0, m->ctx->stream>>>(
+        k_stencil_fill<float><<<kFillBlocks, 256, 0, m->ctx->stream>>>(
             m->val64.get(), m->st_map.get(), m->diag.get(), m->st_K, m->st_ld, m->nv, m->st_val32.get(), nullptr);
         FK_HIP(hipGetLastError());
         m->sell32_valid = true;
