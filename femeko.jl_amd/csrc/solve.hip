@@ -1025,10 +1025,11 @@ static void build_sell(fk_mesh *m) {
 // Stencil values from the FP64 CSR values, one thread per row: the row's
 // entries lie in one or two cache lines, read once for all K columns (and the
 // diagonal for the FP64 standard form).
-#ifndef FK_FILL_BLOCKS
-#define FK_FILL_BLOCKS 2048
-#endif
-constexpr int kFillBlocks = FK_FILL_BLOCKS;
+// Grid of the stencil fill: 512 blocks of 256 threads (two per CU). Measured
+// at n=160: the step 0.6 ms shorter than with one thread per row over the
+// whole grid, whose in-flight CSR lines overflow L2 and are fetched again for
+// each column; 2048 blocks gained nothing.
+constexpr int kFillBlocks = 512;
 template <typename T>
 __global__ void k_stencil_fill(const double *__restrict__ val64, const int32_t *__restrict__ map,
                                const int32_t *__restrict__ diagpos, int K, int64_t ld, int64_t n, T *__restrict__ out,
