@@ -5,8 +5,10 @@
 // lineSearch (src/FEM3D.jl:123-152).
 //
 // Per outer iteration, everything stays on the device:
-//   k_assemble_newton (one launch per colour): gradients, H = -grad(phi) of
-//       the current potential, K_e(mu) plus (Newton step) the tangent
+//   k_assemble_newton_gather (row-gather form, assemble.hip; the coloured
+//       k_assemble_newton below, one launch per colour, when the rows are too
+//       long for its LDS segment): gradients, H = -grad(phi) of the current
+//       potential, K_e(mu) plus (Newton step) the tangent
 //       VE mu'/|H| (H.g_i)(H.g_j) of src/FEM3D.jl:79-119 into the CSR values,
 //       and the element residual -K_e(mu) phi_e into r -- one pass, no
 //       separate SpMV for the residual;
