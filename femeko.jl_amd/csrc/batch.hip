@@ -759,28 +759,43 @@ __global__ void kb_ru_check(BatchRu u) {
     u.act[s] = 2;  // drop the increment, restart from the last true residual
 }
 
-// phi += scale x (act 1), x = 0 (act 1 and 2)
-template <int BLOCK>
+// phi += scale x (act 1), x = 0 (act 1 and 2); a node's NR values are
+// loaded and stored together (a system that does not fold keeps its values)
+template <int BLOCK, int NR>
 __global__ __launch_bounds__(BLOCK) void kb_ru_fold(BatchArgs ba, BatchRu u, double *__restrict__ phi,
                                                     float *__restrict__ x) {
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
-    const int nrhs = ba.nrhs;
-    int a[4] = {0, 0, 0, 0};
+    int a[NR];
+    double sc[NR];
     bool any = false;
-    for (int j = 0; j < nrhs; ++j) {
-        a[j] = u.act[mesh * nrhs + j];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        a[j] = u.act[mesh * NR + j];
+        sc[j] = u.scale[mesh * NR + j];
         any |= a[j] != 0;
     }
     if (!any) return;
     const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK)
-        for (int j = 0; j < nrhs; ++j) {
-            if (!a[j]) continue;
-            const int64_t o = i * nrhs + j;
-            if (a[j] == 1) phi[o] += u.scale[mesh * nrhs + j] * static_cast<double>(x[o]);
-            x[o] = 0.f;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        double ph[NR];
+        float xv[NR];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            ph[j] = phi[i * NR + j];
+            xv[j] = x[i * NR + j];
         }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            if (a[j] == 1) ph[j] += sc[j] * static_cast<double>(xv[j]);
+            if (a[j]) xv[j] = 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            phi[i * NR + j] = ph[j];
+            x[i * NR + j] = xv[j];
+        }
+    }
 }
 
 // one thread per system: the FP64 residual decides (block sums in order)
@@ -828,43 +843,70 @@ __global__ void kb_ru_decide(BatchArgs ba, BatchRu u, const double *__restrict__
 }
 
 // r32 = r64 / scale, p = D^-1 r on a restart, rho partials into the parity-0
-// slot, for the replaced systems only
-template <int BLOCK>
+// slot, for the replaced systems only. When every system of the block's mesh
+// is replaced (and restarts), a node's values are stored together.
+template <int BLOCK, int NR>
 __global__ __launch_bounds__(BLOCK) void kb_ru_replace(BatchArgs ba, BatchRu u, const double *__restrict__ r64,
                                                        const float *__restrict__ dinv, float *__restrict__ r,
                                                        float *__restrict__ p, double *__restrict__ part_rz) {
-    __shared__ double sh[BLOCK / 64 + 1];
+    __shared__ double sh[NR * (BLOCK / 64 + 1)];
     const int lb = batch_logical_block(ba.nblocks);
     const int mesh = ba.blk_mesh[lb];
-    const int nrhs = ba.nrhs;
-    bool rep[4] = {false, false, false, false}, restart[4] = {false, false, false, false};
-    bool any = false;
-    for (int j = 0; j < nrhs; ++j) {
-        const int s = mesh * nrhs + j;
+    bool rep[NR], restart[NR];
+    bool any = false, all_rep = true, all_restart = true;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const int s = mesh * NR + j;
         rep[j] = u.act[s] != 0 && !u.fin[s];
         restart[j] = rep[j] && u.mode[s] == 2;
         any |= rep[j];
+        all_rep &= rep[j];
+        all_restart &= restart[j];
     }
     if (!any) return;
-    double dinvs[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int j = 0; j < nrhs; ++j) dinvs[j] = 1.0 / u.scale[mesh * nrhs + j];
+    double dinvs[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) dinvs[j] = 1.0 / u.scale[mesh * NR + j];
     const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
-    double rz[4] = {0.0, 0.0, 0.0, 0.0};
+    double rz[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) rz[j] = 0.0;
     for (int64_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
         const float d = dinv[i];
-        for (int j = 0; j < nrhs; ++j) {
-            if (!rep[j]) continue;
-            const int64_t o = i * nrhs + j;
-            const float ri = d != 0.f ? static_cast<float>(r64[o] * dinvs[j]) : 0.f;
-            r[o] = ri;
-            if (restart[j]) p[o] = d * ri;
-            rz[j] += static_cast<double>(ri) * static_cast<double>(d * ri);
+        double rd[NR];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) rd[j] = r64[i * NR + j];
+        float rv[NR];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            rv[j] = d != 0.f ? static_cast<float>(rd[j] * dinvs[j]) : 0.f;
+            if (rep[j]) rz[j] += static_cast<double>(rv[j]) * static_cast<double>(d * rv[j]);
+        }
+        if (all_rep) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) r[i * NR + j] = rv[j];
+            if (all_restart) {
+#pragma unroll
+                for (int j = 0; j < NR; ++j) p[i * NR + j] = d * rv[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < NR; ++j)
+                    if (restart[j]) p[i * NR + j] = d * rv[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NR; ++j)
+                if (rep[j]) {
+                    r[i * NR + j] = rv[j];
+                    if (restart[j]) p[i * NR + j] = d * rv[j];
+                }
         }
     }
-    for (int j = 0; j < nrhs; ++j) {
-        if (!rep[j]) continue;  // uniform over the block
-        const double t = block_sum<BLOCK>(rz[j], sh);
-        if (threadIdx.x == 0) part_rz[lb * nrhs + j] = t;
+    block_sum_n<BLOCK, NR>(rz, sh);  // the NR sums in one LDS round
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+            if (rep[j]) part_rz[lb * NR + j] = rz[j];
     }
 }
 
@@ -1191,11 +1233,24 @@ static void enqueue_ru_chunk(fk_batch *b, const BatchRu &u, int G) {
     for (int k = 0; k < G; ++k) enqueue_batch_iteration(b, k & 1, int64_t(1) << 40);
     const unsigned gs = static_cast<unsigned>(grid_for(b->nsys, 64));
     kb_ru_check<<<gs, 64, 0, st>>>(u);
-    kb_ru_fold<256><<<b->nblocks, 256, 0, st>>>(args_of(b), u, b->phi.get(), b->x.get());
+    switch (b->nrhs) {
+        case 1: kb_ru_fold<256, 1><<<b->nblocks, 256, 0, st>>>(args_of(b), u, b->phi.get(), b->x.get()); break;
+        case 2: kb_ru_fold<256, 2><<<b->nblocks, 256, 0, st>>>(args_of(b), u, b->phi.get(), b->x.get()); break;
+        case 3: kb_ru_fold<256, 3><<<b->nblocks, 256, 0, st>>>(args_of(b), u, b->phi.get(), b->x.get()); break;
+        default: kb_ru_fold<256, 4><<<b->nblocks, 256, 0, st>>>(args_of(b), u, b->phi.get(), b->x.get()); break;
+    }
     neg_residual_batch(b, b->phi.get(), b->part_res.get(), b->ru_act.get());
     kb_ru_decide<<<gs, 64, 0, st>>>(args_of(b), u, b->part_res.get());
-    kb_ru_replace<256><<<b->nblocks, 256, 0, st>>>(args_of(b), u, b->r64.get(), m->w32.dinv.get(), b->r.get(),
-                                                   b->p.get(), b->part_rz.get());
+    switch (b->nrhs) {
+#define FK_RU_REPLACE(NR)                                                                                          \
+    kb_ru_replace<256, NR><<<b->nblocks, 256, 0, st>>>(args_of(b), u, b->r64.get(), m->w32.dinv.get(), b->r.get(), \
+                                                       b->p.get(), b->part_rz.get())
+        case 1: FK_RU_REPLACE(1); break;
+        case 2: FK_RU_REPLACE(2); break;
+        case 3: FK_RU_REPLACE(3); break;
+        default: FK_RU_REPLACE(4); break;
+#undef FK_RU_REPLACE
+    }
     kb_ru_resume<<<gs, 64, 0, st>>>(u);
     FK_HIP(hipGetLastError());
 }
