@@ -1,4 +1,4 @@
-"""Batched solves (many meshes x 3 applied fields) against per-system oracle
+"""Batched solves (many meshes x 1 to 4 applied fields) against per-system oracle
 solves. Bar: every system's potential within 1e-9 relative of the oracle's
 direct-accuracy solution (GPU FP32-refined to rtol 1e-10), FP64 true residual
 below 1e-10."""
@@ -205,5 +205,36 @@ def test_batch_iterations_close_to_single_mesh_solves():
         single = np.array(single)
         its = np.asarray(rep.iterations).reshape(-1)  # system s = mesh * 3 + field
         assert np.all(its <= 1.5 * single + 64), (its, single)
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("nrhs", [1, 2, 4])
+@pytest.mark.parametrize("fmt", ["auto", "csr"])
+def test_batch_other_field_counts(oracle, nrhs, fmt):
+    # the kernels are instantiated per number of right-hand sides (1..4);
+    # every count against the oracle's per-system solves
+    probs = small_batch(count=4, seed=11)
+    dirs = [np.array(v, float) for v in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1))][:nrhs]
+    gs = [np.stack([np.where(p.mask == 1, dirichlet_values(p.mesh, e), 0.0) for e in dirs]) for p in probs]
+    b = Batch([p.mesh for p in probs], nrhs=nrhs)
+    try:
+        b.set_format(fmt)
+        b.set_coefficient([p.mu for p in probs])
+        b.assemble()
+        b.set_dirichlet([p.mask for p in probs], gs)
+        rep = b.solve(rtol=1e-10)
+        assert rep.relres.shape == (len(probs), nrhs)
+        assert np.all(rep.relres < 1e-10)
+        assert b.solver_format() == ("stencil" if fmt == "auto" else "csr")
+        for i, p in enumerate(probs):
+            A = oracle.stiffnessMatrix(p.mesh, p.mu)
+            phi = b.potential(i)
+            assert phi.shape == (nrhs, p.mesh.nv)
+            for j in range(nrhs):
+                ref, _it, _rel, s = oracle.pcg_dirichlet(A, p.mask, gs[i][j], 1e-13)
+                assert s == 0
+                err = np.linalg.norm(phi[j] - ref) / np.linalg.norm(ref)
+                assert err < 1e-9, (nrhs, fmt, i, j, err)
     finally:
         b.close()
