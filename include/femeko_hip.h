@@ -195,7 +195,9 @@ int fk_solve_neumann(fk_mesh *mesh, const double *b, double rtol, int64_t max_it
  * volumes[i] == 0, as the reference's 0/0). M, Hd_nodes: 3 x nv column-major
  * (M[3 i + c]); volumes: nv (the driver's per-node sums of inside VE).
  * warm != 0: the PCG starts from the previous call's potential on this mesh
- * (kept on the device) instead of zero, same stopping test. */
+ * (kept on the device) instead of zero, same stopping test. A non-finite M
+ * entry returns FK_ERR_NONFINITE, a negative or non-finite volume
+ * FK_ERR_INVALID_ARG, both before the kept potential changes. */
 int fk_demag_field(fk_mesh *mesh, const uint8_t *inside, const double *M, const double *volumes, double rtol,
                    int64_t max_iter, int32_t warm, double *Hd_nodes, int64_t *iterations, double *relres);
 
