@@ -297,3 +297,124 @@ def kuhn_counts(n: int) -> dict:
         "free": (n - 1) ** 3,
         "dirichlet": n1 ** 3 - (n - 1) ** 3,
     }
+
+
+# ---- text dumps of a mesh -------------------------------------------------
+# Femeko's mesh builder can save a mesh as text files (src/mesh.jl:232-242):
+# one file per field, written by save2file (src/geometry.jl:236-243), which
+# prints each row of the matrix on one line with the entries joined by " , ".
+# A 3 x nv or 4 x nt matrix therefore takes 3 or 4 lines; a vector (VE, AE,
+# InsideElements, InsideNodes) takes one line per entry. These readers let a
+# mesh built by Femeko (Gmsh) be loaded here without Julia.
+
+_DUMP_MATRICES = {"p.txt": 3, "t.txt": 4, "surfaceT.txt": 4, "normal.txt": 3}
+_DUMP_VECTORS = ("VE.txt", "AE.txt", "InsideElements.txt", "InsideNodes.txt")
+
+
+def _read_rows(path, parse) -> list[list]:
+    rows = []
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line:
+                rows.append([parse(tok) for tok in line.split(",")])
+    return rows
+
+
+def _read_matrix(path, nrows: int, dtype) -> np.ndarray:
+    parse = float if dtype == np.float64 else int
+    rows = _read_rows(path, parse)
+    if not rows:
+        return np.zeros((nrows, 0), dtype=dtype, order="F")
+    if len(rows) != nrows or len({len(r) for r in rows}) != 1:
+        raise ValueError(f"{path}: expected {nrows} rows of equal length, got {[len(r) for r in rows]}")
+    return np.asfortranarray(np.array(rows, dtype=dtype))
+
+
+def _read_vector(path, dtype) -> np.ndarray:
+    parse = float if dtype == np.float64 else int
+    rows = _read_rows(path, parse)
+    if any(len(r) != 1 for r in rows):
+        raise ValueError(f"{path}: expected one value per line")
+    return np.array([r[0] for r in rows], dtype=dtype)
+
+
+def load_mesh_dump(folder) -> MESH:
+    """A MESH from Femeko's text dump (t.txt and p.txt required; VE.txt,
+    InsideElements.txt, InsideNodes.txt, surfaceT.txt, AE.txt, normal.txt
+    read when present; VE recomputed by elementVolume when absent). Only P1
+    meshes (4 nodes per element) are accepted; node numbers are 1-based as
+    written. Raises ValueError on a malformed file or an out-of-range index."""
+    import os
+    path = lambda name: os.path.join(folder, name)  # noqa: E731
+    rows_t = _dump_rows(path("t.txt"))
+    if rows_t != 4:
+        raise ValueError(f"t.txt: a P1 mesh has 4 node rows, found {rows_t} (second-order meshes are not supported)")
+    t = _read_matrix(path("t.txt"), 4, np.int32)
+    p = _read_matrix(path("p.txt"), 3, np.float64)
+    nv, nt = p.shape[1], t.shape[1]
+    if nt and (t.min() < 1 or t.max() > nv):
+        raise ValueError("t.txt: node numbers outside 1..nv")
+    if not np.all(np.isfinite(p)):
+        raise ValueError("p.txt: non-finite coordinates")
+    VE = _read_vector(path("VE.txt"), np.float64) if os.path.exists(path("VE.txt")) else element_volumes(p, t)
+    if VE.size != nt:
+        raise ValueError("VE.txt: one volume per element expected")
+    mesh = MESH(p=p, t=t, VE=VE)
+    if os.path.exists(path("InsideElements.txt")):
+        ie = _read_vector(path("InsideElements.txt"), np.int32)
+        if ie.size and (ie.min() < 1 or ie.max() > nt):
+            raise ValueError("InsideElements.txt: element numbers outside 1..nt")
+        mesh.InsideElements = ie
+    if os.path.exists(path("InsideNodes.txt")):
+        inn = _read_vector(path("InsideNodes.txt"), np.int32)
+        if inn.size and (inn.min() < 1 or inn.max() > nv):
+            raise ValueError("InsideNodes.txt: node numbers outside 1..nv")
+        mesh.InsideNodes = inn
+    if os.path.exists(path("surfaceT.txt")):
+        sT = _read_matrix(path("surfaceT.txt"), 4, np.int32)
+        if sT.shape[1] and (sT[:3].min() < 1 or sT[:3].max() > nv):
+            raise ValueError("surfaceT.txt: node numbers outside 1..nv")
+        mesh.surfaceT = sT
+        ns = sT.shape[1]
+        if os.path.exists(path("AE.txt")):
+            mesh.AE = _read_vector(path("AE.txt"), np.float64)
+            if mesh.AE.size != ns:
+                raise ValueError("AE.txt: one area per surface triangle expected")
+        if os.path.exists(path("normal.txt")):
+            mesh.normal = _read_matrix(path("normal.txt"), 3, np.float64)
+            if mesh.normal.shape[1] != ns:
+                raise ValueError("normal.txt: one normal per surface triangle expected")
+        mesh.shell_id = tuple(int(v) for v in np.unique(sT[3])) if ns else ()
+    return mesh
+
+
+def _dump_rows(path) -> int:
+    with open(path) as f:
+        return sum(1 for line in f if line.strip())
+
+
+def save_mesh_dump(mesh: MESH, folder) -> None:
+    """Write a MESH in the text-dump layout that load_mesh_dump and Femeko's
+    save2file use (rows on lines, entries joined by " , "; floats in their
+    shortest round-trip form, so a dump read back gives the same bits)."""
+    import os
+    os.makedirs(folder, exist_ok=True)
+
+    def put(name, rows):
+        with open(os.path.join(folder, name), "w") as f:
+            for row in rows:
+                f.write(" , ".join(repr(float(v)) if isinstance(v, (float, np.floating)) else str(int(v))
+                                   for v in row) + "\n")
+
+    put("p.txt", np.asarray(mesh.p, dtype=np.float64))
+    put("t.txt", np.asarray(mesh.t, dtype=np.int64))
+    put("VE.txt", np.asarray(mesh.VE, dtype=np.float64).reshape(-1, 1))
+    put("InsideElements.txt", np.asarray(mesh.InsideElements, dtype=np.int64).reshape(-1, 1))
+    put("InsideNodes.txt", np.asarray(mesh.InsideNodes, dtype=np.int64).reshape(-1, 1))
+    if mesh.surfaceT is not None:
+        put("surfaceT.txt", np.asarray(mesh.surfaceT, dtype=np.int64))
+        if mesh.AE is not None:
+            put("AE.txt", np.asarray(mesh.AE, dtype=np.float64).reshape(-1, 1))
+        if mesh.normal is not None:
+            put("normal.txt", np.asarray(mesh.normal, dtype=np.float64))

@@ -1,0 +1,74 @@
+"""Reading Femeko's text dump of a mesh (src/mesh.jl:232-242 through
+save2file, src/geometry.jl:236-243: one matrix row per line, entries joined
+by " , "; vectors one entry per line).
+
+tests/golden/mesh_dump_2tet holds a dump of a two-element mesh written in
+that layout by hand (Julia prints Float64 in shortest round-trip form, as
+below): node 5 = (1, 1, 1) on the far side of the face (2, 3, 4) of the
+reference tetrahedron, so the volumes are 1/6 and |det| / 6 = 2/6, the bottom
+face (1, 3, 2) with tag 1 has area 1/2 and outward normal -z."""
+import os
+
+import numpy as np
+import pytest
+
+from femeko_hip import mesh as M
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "mesh_dump_2tet")
+
+
+def test_golden_dump_reads_as_written():
+    m = M.load_mesh_dump(GOLDEN)
+    np.testing.assert_array_equal(m.p, [[0, 1, 0, 0, 1], [0, 0, 1, 0, 1], [0, 0, 0, 1, 1]])
+    assert m.p.dtype == np.float64 and m.p.flags.f_contiguous
+    np.testing.assert_array_equal(m.t, [[1, 2], [2, 3], [3, 4], [4, 5]])
+    assert m.t.dtype == np.int32 and (m.nv, m.nt) == (5, 2)
+    # the dumped volumes are the closed-form ones and elementVolume's
+    assert m.VE.tolist() == [1 / 6, 2 / 6]
+    np.testing.assert_allclose(M.element_volumes(m.p, m.t), m.VE, rtol=1e-15)
+    assert m.InsideElements.tolist() == [2] and m.InsideNodes.tolist() == [2, 3, 4, 5]
+    np.testing.assert_array_equal(m.surfaceT, [[1], [3], [2], [1]])
+    assert m.AE.tolist() == [0.5] and m.normal[:, 0].tolist() == [0.0, 0.0, -1.0]
+    assert m.shell_id == (1,)
+
+
+def test_dump_round_trip_is_bit_exact(tmp_path):
+    m = M.boundary_faces(M.kuhn_box_mesh(4, jitter=0.2, seed=9))
+    M.set_inside(m, np.arange(m.nt) % 3 == 0)
+    M.save_mesh_dump(m, tmp_path)
+    r = M.load_mesh_dump(tmp_path)
+    for name in ("p", "VE", "AE", "normal"):
+        a, b = np.asarray(getattr(m, name), np.float64), getattr(r, name)
+        assert np.array_equal(a.view(np.uint64), np.ascontiguousarray(b).view(np.uint64).reshape(a.shape)), name
+    for name in ("t", "surfaceT", "InsideElements", "InsideNodes"):
+        np.testing.assert_array_equal(getattr(m, name), getattr(r, name))
+    assert r.shell_id == (1,)
+
+
+def test_dump_without_volumes_recomputes_them(tmp_path):
+    m = M.kuhn_box_mesh(2, jitter=0.1, seed=3)
+    M.save_mesh_dump(m, tmp_path)
+    os.remove(tmp_path / "VE.txt")
+    r = M.load_mesh_dump(tmp_path)
+    np.testing.assert_allclose(r.VE, M.element_volumes(m.p, m.t), rtol=0, atol=0)
+
+
+def test_julia_float_spellings(tmp_path):
+    # Julia writes 1.0e-5 and -0.0 where Python writes 1e-05 and -0.0
+    (tmp_path / "AE.txt").write_text("1.0e-5\n-0.0\n2.5\n")
+    v = M._read_vector(str(tmp_path / "AE.txt"), np.float64)
+    assert v.tolist() == [1e-5, -0.0, 2.5] and np.signbit(v[1])
+
+
+@pytest.mark.parametrize("t_text,err", [
+    ("1 , 2\n2 , 3\n3 , 4\n", "4 node rows"),               # a row missing
+    ("1\n2\n3\n4\n5\n6\n7\n8\n9\n10\n", "second-order"),    # P2 (10 nodes)
+    ("1 , 2\n2 , 3\n3 , 4\n4 , 6\n", "outside 1..nv"),       # node 6 of 5
+    ("1 , 2\n2 , 3\n3\n4 , 5\n", "equal length"),             # ragged rows
+])
+def test_malformed_dumps_are_rejected(tmp_path, t_text, err):
+    for name in ("p.txt", "VE.txt"):
+        (tmp_path / name).write_text(open(os.path.join(GOLDEN, name)).read())
+    (tmp_path / "t.txt").write_text(t_text)
+    with pytest.raises(ValueError, match=err):
+        M.load_mesh_dump(tmp_path)
