@@ -72,3 +72,25 @@ def test_malformed_dumps_are_rejected(tmp_path, t_text, err):
     (tmp_path / "t.txt").write_text(t_text)
     with pytest.raises(ValueError, match=err):
         M.load_mesh_dump(tmp_path)
+
+
+@pytest.mark.gpu
+def test_dumped_mesh_assembles_on_the_gpu_as_the_oracle(oracle, tmp_path):
+    # a mesh taken through the text dump (save, then load as a Femeko user's
+    # saved mesh would be) assembles on the GPU to the oracle's matrix
+    from femeko_hip.device import DeviceMesh, context
+    m0 = M.boundary_faces(M.kuhn_box_mesh(6, jitter=0.25, seed=21))
+    M.save_mesh_dump(m0, tmp_path)
+    m = M.load_mesh_dump(tmp_path)
+    mu = 1.0 + np.arange(m.nt) % 5
+    A = oracle.stiffnessMatrix(m, mu)
+    dm = DeviceMesh(m, context(0))
+    try:
+        dm.set_coefficient(mu)
+        dm.assemble()
+        rowptr, colidx = dm.pattern(base=1)
+        assert np.array_equal(rowptr, A.colptr) and np.array_equal(colidx, A.rowval)
+        bound = oracle.term_scale(m, mu)
+        assert np.all(np.abs(dm.values() - A.nzval) <= 1e-12 * bound)
+    finally:
+        dm.close()
