@@ -411,6 +411,13 @@ def main():
     log(f"device mesh: nnz={dm.nnz} colours={dm.ncolours} ({time.perf_counter()-t0:.1f} s)")
     fmt_name, stencil_offsets = dm.solver_format()
     sell = dm.sell_info() if fmt_name == "sell64" else None
+    sell_index = None
+    if sell is not None:
+        # the SELL copy stores int16 offsets from the row when the node
+        # bandwidth max |j - i| (+ 64) fits, int32 columns otherwise
+        t = np.asarray(m.t, dtype=np.int64)
+        bw = max(int(np.abs(t[a_] - t[b_]).max()) for a_ in range(4) for b_ in range(a_ + 1, 4))
+        sell_index = "int16" if bw + 64 <= 32767 else "int32"
     log(f"solver format: {fmt_name} {stencil_offsets if stencil_offsets else ''}")
     if sell:
         log(f"SELL copy: {sell[0]} slices, {sell[1]} with shared offsets, {sell[2]} stored entries")
@@ -539,7 +546,7 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": {"csr": "k_cg_spmv<float,%d>" % args.lanes,
-                       "sell64": "k_cg_spmv_sell<float,int16,%d,%d>" % shape,
+                       "sell64": "k_cg_spmv_sell<float,%s,%d,%d>" % ((sell_index,) + shape),
                        "stencil": "k_cg_spmv_stencil<float,256>"}[fmt_name],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
