@@ -194,6 +194,7 @@ struct fk_mesh {
     fk::DevBuf<double> val64;
     fk::DevBuf<float> val32;
     bool have_values = false, have_val32 = false;
+    int64_t values_gen = 0;           // bumped whenever val64 changes or is dropped
     // Dirichlet data
     fk::DevBuf<uint8_t> mask;
     fk::DevBuf<double> g;
@@ -280,6 +281,8 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
 void solve_rhs(fk_mesh *m, const double *b, double rtol, int64_t max_iter, double *x, int64_t *iters,
                double *relres);
 void field(fk_mesh *m, double *H, double *Hn, double *energy);
+void boundary_integral(fk_mesh *m, int64_t ns, const int32_t *sT, const double *nrm, const double *AE,
+                       const double *F, const int32_t *ids, int nids, int32_t base, double *rhs_host);
 void setup_partition(fk_mesh *m);
 void ensure_work_f64(fk_mesh *m);
 void compute_dinv_f64(fk_mesh *m);
@@ -287,6 +290,7 @@ void compute_dinv_f32(fk_mesh *m);
 // call whenever val64 changed: drops every derived copy (FP32, SELL, D^-1)
 inline void values_changed(fk_mesh *m) {
     m->have_values = true;
+    m->values_gen += 1;
     m->have_val32 = false;
     m->dinv64_valid = false;
     m->dinv32_valid = false;

@@ -317,8 +317,18 @@ int fk_context_create(int device, fk_context **out) {
         hipDeviceProp_t prop;
         FK_HIP(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        // Iteration chunks are captured in hipGraphs unless FEMEKO_HIP_GRAPHS=0,
+        // or a rocprofiler-sdk tool (rocprofv3) is attached and
+        // FEMEKO_HIP_GRAPHS=1 does not insist: under rocprofv3 --kernel-trace
+        // a graph launch of the FP64 iteration chunk once ended the host
+        // process with SIGSEGV, while the host code around the launch (the
+        // pinned state copies, two of the four allocated, and the event
+        // waits) reads only memory the context owns and the same path runs
+        // clean without the tool; the kernels are the same either way.
         const char *g = std::getenv("FEMEKO_HIP_GRAPHS");
-        ctx->graphs = !(g && g[0] == '0');
+        const char *tool = std::getenv("ROCP_TOOL_LIBRARIES");
+        const bool profiled = tool && tool[0] != '\0';
+        ctx->graphs = g ? g[0] != '0' : !profiled;
         FK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     } catch (const fk::Error &e) {
         delete ctx;
@@ -393,6 +403,12 @@ int fk_mesh_info(const fk_mesh *m, int64_t *nnz, int32_t *ncolours, int64_t *unu
     if (nnz) *nnz = m->nnz;
     if (ncolours) *ncolours = m->ncolours;
     if (unused) *unused = m->unused_nodes;
+    return FK_OK;
+}
+
+int fk_values_generation(const fk_mesh *m, int64_t *generation) {
+    if (!m || !generation) return FK_ERR_INVALID_ARG;
+    *generation = m->values_gen;
     return FK_OK;
 }
 

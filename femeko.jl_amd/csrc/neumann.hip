@@ -11,6 +11,8 @@
 //   u <- u - (Lag'u / Lag'1) 1.
 #include <algorithm>
 #include <cmath>
+#include <string>
+#include <vector>
 
 #include "device.h"
 #include "internal.h"
@@ -91,6 +93,27 @@ __global__ __launch_bounds__(kBlock) void k_sub_spmv_csr(const int32_t *__restri
 #pragma unroll
         for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
         if (row < n && lane == 0) r[row] -= s;
+    }
+}
+
+// partials of (sum d_i u_i, sum d_i) with d_i = a_ii
+__global__ __launch_bounds__(kBlock) void k_diag_weighted(const double *__restrict__ val,
+                                                          const int32_t *__restrict__ diag,
+                                                          const double *__restrict__ u, int64_t n,
+                                                          double *__restrict__ part) {
+    __shared__ double sh[kBlock / 64 + 1];
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const double d = diag[i] >= 0 ? val[diag[i]] : 0.0;
+        s0 += d * u[i];
+        s1 += d;
+    }
+    s0 = block_sum<kBlock>(s0, sh);
+    s1 = block_sum<kBlock>(s1, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0;
+        part[2 * blockIdx.x + 1] = s1;
     }
 }
 
@@ -231,6 +254,74 @@ void solve_neumann(fk_mesh *m, const double *b_host, double rtol, int64_t max_it
     if (relres_out) *relres_out = res.relres;
 }
 
+// scalarPotential of Femeko_jl_C++/Magnetostatics/magnetostatics.cpp:49-93
+// (the C++ bridge run.jl:66-79 calls): RHS = BoundaryIntegral with the
+// triangle areas from p (src/FEMc.cpp:45-54, 170-200), A = stiffnessMatrix(p,
+// t, VE, mu) (FEMc.cpp:102-133), u solving A u = -RHS by Jacobi-PCG from zero
+// (Eigen's ConjugateGradient, magnetostatics.cpp:26-38). A is singular (A 1 =
+// 0) and -RHS consistent for a closed shell; PCG from zero with the diagonal
+// preconditioner keeps sum_i a_ii u_i = 0, so the constant is fixed that way.
+static void scalar_potential(double *u, const double *p, const int32_t *t, const int32_t *surfaceT,
+                             const double *normal, int64_t nv, int64_t nt, int64_t ns, const double *VE,
+                             const double *mu, const double *Hext, int32_t shell_id, double rtol) {
+    struct Ctx {
+        fk_context *c = nullptr;
+        ~Ctx() { if (c) fk_context_destroy(c); }
+    } ctx;
+    struct Mesh {
+        fk_mesh *m = nullptr;
+        ~Mesh() { if (m) fk_mesh_destroy(m); }
+    } mesh;
+    int s = fk_context_create(0, &ctx.c);
+    if (s != FK_OK) fail(s, "no GPU context on device 0");
+    s = fk_mesh_create(ctx.c, p, nv, t, nt, 0, VE, &mesh.m);
+    if (s != FK_OK) fail(s, std::string("mesh: ") + fk_context_last_error(ctx.c));
+    fk_mesh *m = mesh.m;
+    set_device(m->ctx);
+    // triangle areas |AB x AC| / 2 from the node coordinates
+    std::vector<double> AE(static_cast<size_t>(std::max<int64_t>(ns, 1)));
+    for (int64_t k = 0; k < ns; ++k) {
+        const int32_t *tri = surfaceT + 4 * k;
+        for (int i = 0; i < 3; ++i)
+            if (tri[i] < 0 || tri[i] >= nv) fail(FK_ERR_INDEX, "surface triangle node number out of range");
+        const double *a = p + 3 * static_cast<int64_t>(tri[0]), *b = p + 3 * static_cast<int64_t>(tri[1]),
+                     *c = p + 3 * static_cast<int64_t>(tri[2]);
+        const double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const double x = ab[1] * ac[2] - ab[2] * ac[1], y = ab[2] * ac[0] - ab[0] * ac[2],
+                     z = ab[0] * ac[1] - ab[1] * ac[0];
+        AE[k] = 0.5 * std::sqrt(x * x + y * y + z * z);
+    }
+    std::vector<double> rhs(static_cast<size_t>(nv));
+    boundary_integral(m, ns, surfaceT, normal, AE.data(), Hext, &shell_id, 1, 0, rhs.data());
+    for (auto &v : rhs) v = -v;
+    upload_coefficient(m, mu);
+    assemble(m, FK_ASM_GATHER);
+    ensure_work_f64(m);
+    hipStream_t st = m->ctx->stream;
+    FK_HIP(hipMemcpyAsync(m->w64.r.get(), rhs.data(), nv * sizeof(double), hipMemcpyHostToDevice, st));
+    solve_neumann_dev(m, rtol, std::max<int64_t>(1000, 20 * nv), nullptr);
+    // fix the constant: sum_i a_ii u_i = 0
+    const int nb = std::max(1, m->nblocks);
+    DevBuf<double> part(2 * nb);
+    k_diag_weighted<<<nb, kBlock, 0, st>>>(m->val64.get(), m->diag.get(), m->w64.x.get(), nv, part.get());
+    FK_HIP(hipGetLastError());
+    std::vector<double> h(2 * nb);
+    FK_HIP(hipMemcpyAsync(h.data(), part.get(), 2 * nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+    double du = 0.0, dd = 0.0;
+    for (int i = 0; i < nb; ++i) {
+        du += h[2 * i];
+        dd += h[2 * i + 1];
+    }
+    k_shift<<<grid_for(nv, 256), 256, 0, st>>>(m->w64.x.get(), dd > 0.0 ? du / dd : 0.0, nv);
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipMemcpyAsync(u, m->w64.x.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipStreamSynchronize(st));
+}
+
+thread_local std::string g_bridge_error;
+
 }  // namespace fk
 
 using fk::guard;
@@ -257,5 +348,30 @@ int fk_solve_neumann(fk_mesh *m, const double *b, double rtol, int64_t max_iter,
         fk::solve_neumann(m, b, rtol, max_iter, u, multiplier, iterations, relres);
     });
 }
+
+int fk_scalar_potential(double *u, const double *p, const int32_t *t, const int32_t *surfaceT, const double *normal,
+                        int32_t nv, int32_t nt, int32_t ns, const double *VE, const double *mu, const double *Hext,
+                        int32_t shell_id) {
+    fk::g_bridge_error.clear();
+    if (!u || !p || !t || nv < 1 || nt < 1 || ns < 0 || (ns > 0 && (!surfaceT || !normal)) || !mu || !Hext) {
+        fk::g_bridge_error = "fk_scalar_potential: null pointer or bad size";
+        return FK_ERR_INVALID_ARG;
+    }
+    try {
+        fk::scalar_potential(u, p, t, surfaceT, normal, nv, nt, ns, VE, mu, Hext, shell_id, 1e-10);
+        return FK_OK;
+    } catch (const fk::Error &e) {
+        fk::g_bridge_error = "fk_scalar_potential: " + e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        fk::g_bridge_error = "fk_scalar_potential: host allocation failed";
+        return FK_ERR_OOM;
+    } catch (...) {
+        fk::g_bridge_error = "fk_scalar_potential: unexpected error";
+        return FK_ERR_HIP;
+    }
+}
+
+const char *fk_scalar_potential_last_error(void) { return fk::g_bridge_error.c_str(); }
 
 }  // extern "C"

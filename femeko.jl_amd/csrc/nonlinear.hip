@@ -288,9 +288,11 @@ static void nonlinear_loop(fk_mesh *m, const uint8_t *region_host, const LawArgs
         nonlinear_loop_body(m, region_host, law, rtol, tol, max_outer, switch_tol, outer_out, cg_out, dev_out);
     } catch (...) {
         m->have_values = false;
+        m->values_gen += 1;
         throw;
     }
     m->have_values = false;
+    m->values_gen += 1;
 }
 
 static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const LawArgs &law, double rtol,

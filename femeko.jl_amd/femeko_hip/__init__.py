@@ -8,6 +8,7 @@ field-target gradient, mesh/workloads for the synthetic meshes.
 from ._lib import FemekoHIPError, LIB_PATH  # noqa: F401
 from .device import Context, DeviceMesh, SolveReport, context  # noqa: F401
 from .fem3d import (BoundaryIntegral, device_mesh, invalidate, lagrange, lineSearch,  # noqa: F401
-                    localStiffnessMatrix, magneticField, solveDirichlet, solveSaddle, stiffnessMatrix,
+                    localStiffnessMatrix, magneticField, scalarPotential, solveDirichlet, solveSaddle,
+                    stiffnessMatrix,
                     tangentialStiffnessMatrix)
 from .mesh import MESH, MU0, kuhn_box_mesh  # noqa: F401

@@ -63,6 +63,7 @@ SIGNATURES = {
     "fk_pattern": (_c.c_int, [_vp, _c.c_int32, _ip, _ip]),
     "fk_set_coefficient": (_c.c_int, [_vp, _dp]),
     "fk_assemble": (_c.c_int, [_vp, _c.c_int32]),
+    "fk_values_generation": (_c.c_int, [_vp, _i64p]),
     "fk_values": (_c.c_int, [_vp, _dp]),
     "fk_values_f32": (_c.c_int, [_vp, _fp]),
     "fk_stiffness_matrix": (_c.c_int, [_vp, _dp, _c.c_int32, _ip, _ip, _dp]),
@@ -77,6 +78,9 @@ SIGNATURES = {
     "fk_set_potential": (_c.c_int, [_vp, _dp]),
     "fk_field": (_c.c_int, [_vp, _dp, _dp, _dp]),
     "fk_boundary_integral": (_c.c_int, [_vp, _c.c_int64, _ip, _dp, _dp, _dp, _ip, _c.c_int32, _c.c_int32, _dp]),
+    "fk_scalar_potential": (_c.c_int, [_dp, _dp, _ip, _ip, _dp, _c.c_int32, _c.c_int32, _c.c_int32, _dp, _dp, _dp,
+                                       _c.c_int32]),
+    "fk_scalar_potential_last_error": (_c.c_char_p, []),
     "fk_solve_neumann": (_c.c_int, [_vp, _dp, _c.c_double, _c.c_int64, _dp, _dp, _i64p, _dp]),
     "fk_nonlinear_froehlich": (_c.c_int, [_vp, _up, _c.c_double, _c.c_double, _c.c_double, _c.c_double,
                                           _c.c_int32, _c.c_double, _ip, _i64p, _dp]),

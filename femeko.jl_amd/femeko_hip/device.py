@@ -154,6 +154,12 @@ class DeviceMesh:
     def assemble(self, method: str = "gather") -> None:
         self.ctx.check(self._L.fk_assemble(self.handle, METHODS[method]), "fk_assemble")
 
+    def values_generation(self) -> int:
+        """Counter of changes of the resident matrix values (fk_values_generation)."""
+        g = ctypes.c_int64(0)
+        self.ctx.check(self._L.fk_values_generation(self.handle, ctypes.byref(g)), "fk_values_generation")
+        return int(g.value)
+
     def values(self) -> np.ndarray:
         v = np.empty(self.nnz)
         self.ctx.check(self._L.fk_values(self.handle, dptr(v)), "fk_values")

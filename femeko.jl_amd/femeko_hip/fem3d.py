@@ -12,6 +12,8 @@ device copy of a MESH is made on first use and cached on the MESH object
   solveDirichlet(mesh, mu, mask, g, ...)   the `\\` / cg solve of the drivers
                                            with Dirichlet elimination
                                            (Fluids/fluids3D.jl:142-174)
+  scalarPotential(mesh, mu, Hext, shell)   the C++ bridge of
+                                           Femeko_jl_C++/Magnetostatics/run.jl:66-79
 
 Sparse results are scipy.sparse.csc_matrix with the arrays of Julia's
 SparseMatrixCSC (0-based here, as scipy stores them).
@@ -21,6 +23,8 @@ from __future__ import annotations
 import numpy as np
 import scipy.sparse as sp
 
+from . import _lib
+from ._lib import FK_ERR_STATE, FemekoHIPError, dptr, iptr
 from .device import DeviceMesh, context
 
 
@@ -45,7 +49,10 @@ def _csc(dm: DeviceMesh, values: np.ndarray, resident: bool = False) -> sp.csc_m
     # the pattern and the values are exactly symmetric: CSR arrays == CSC arrays
     A = sp.csc_matrix((values, colidx, rowptr), shape=(dm.nv, dm.nv))
     if resident:
-        A.fk_mesh = dm  # the device holds these values: lineSearch runs there
+        # the device holds these values: lineSearch runs there, as long as no
+        # later call on the mesh has replaced them
+        A.fk_mesh = dm
+        A.fk_generation = dm.values_generation()
     return A
 
 
@@ -53,8 +60,14 @@ def lineSearch(u, du, A, RHS, minStep: float = 1e-4) -> float:
     """src/FEM3D.jl:123-152 on the GPU: alpha <- 0.9 alpha until
     ||RHS - A(u + alpha du)|| <= ||RHS - A u||; mutates u. A is a matrix
     returned by stiffnessMatrix (its values are resident on the GPU) or a
-    MESH whose last assembled matrix is meant."""
+    MESH whose last assembled matrix is meant. A matrix whose resident values
+    a later call on the same mesh has replaced (another stiffnessMatrix,
+    solveSaddle, solveDirichlet, a nonlinear loop) raises FemekoHIPError
+    rather than searching along another matrix."""
     dm = getattr(A, "fk_mesh", None)
+    if dm is not None and dm.values_generation() != getattr(A, "fk_generation", None):
+        raise FemekoHIPError(FK_ERR_STATE, "lineSearch: the matrix's values on the device were replaced by a "
+                                           "later assembly on this mesh; call stiffnessMatrix again")
     if dm is None and hasattr(A, "p") and hasattr(A, "t"):
         dm = device_mesh(A)
     if dm is None:
@@ -119,3 +132,28 @@ def solveDirichlet(mesh, mu, mask, g, rtol: float = 1e-10, precision: str = "f64
     dm.set_dirichlet(mask, g)
     rep = dm.solve(precision=precision, rtol=rtol, inner_rtol=inner_rtol, max_iter=max_iter)
     return dm.potential(), rep
+
+
+def scalarPotential(mesh, mu, Hext, shell_id) -> np.ndarray:
+    """u of the reference's C++ bridge scalarPotential
+    (Femeko_jl_C++/Magnetostatics/magnetostatics.cpp:49-93) through
+    fk_scalar_potential, called as run.jl:47-79 does: 0-based copies of t and
+    surfaceT (node numbers and tag) and shell_id - 1; triangle areas from p.
+    Context-free: the mesh is uploaded, solved and freed in the call."""
+    L = _lib.load()
+    p = np.ascontiguousarray(np.asarray(mesh.p, dtype=np.float64).T)
+    t = np.ascontiguousarray((np.asarray(mesh.t, dtype=np.int32) - 1).T)
+    sT = np.ascontiguousarray((np.asarray(mesh.surfaceT, dtype=np.int32) - 1).T)
+    nrm = np.ascontiguousarray(np.asarray(mesh.normal, dtype=np.float64).T)
+    VE = np.ascontiguousarray(mesh.VE, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    H = np.ascontiguousarray(Hext, dtype=np.float64)
+    nv, nt, ns = p.shape[0], t.shape[0], sT.shape[0]
+    if mu.size != nt or H.size != 3 or VE.size != nt:
+        raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "scalarPotential: mu and VE need nt entries, Hext 3")
+    u = np.zeros(nv)
+    s = L.fk_scalar_potential(dptr(u), dptr(p), iptr(t), iptr(sT), dptr(nrm), nv, nt, ns, dptr(VE), dptr(mu),
+                              dptr(H), int(shell_id) - 1)
+    if s:
+        raise FemekoHIPError(s, L.fk_scalar_potential_last_error().decode())
+    return u

@@ -103,10 +103,17 @@ int fk_pattern(fk_mesh *mesh, int32_t index_base, int32_t *rowptr, int32_t *coli
  * Non-finite values return FK_ERR_NONFINITE. */
 int fk_set_coefficient(fk_mesh *mesh, const double *f);
 /* Assemble A(f) values on the GPU (FP64; FP32 copy made when a mixed solve
- * needs it). method: FK_ASM_GATHER (fastest: 3.1 ms vs 13.7 ms coloured at 24.6M
+ * needs it). method: FK_ASM_GATHER (fastest: 1.55 ms vs 13.6 ms coloured at 24.6M
  * tets), FK_ASM_COLOURED or FK_ASM_ATOMIC; all three agree to rounding. The
  * first FK_ASM_GATHER call on a mesh builds its incidence lists on the host. */
 int fk_assemble(fk_mesh *mesh, int32_t method);
+/* Counter of changes of the resident matrix values: it grows with every
+ * assembly and whenever a call leaves other values there (the nonlinear loops
+ * leave A + At). A caller that keeps a matrix object for later calls on the
+ * resident values (lineSearch, src/FEM3D.jl:123-152, takes A as an argument)
+ * stores the counter with it and compares before use. No counterpart in the
+ * reference, whose matrices are host objects. */
+int fk_values_generation(const fk_mesh *mesh, int64_t *generation);
 /* Copy the assembled FP64 values (nnz, CSR order == CSC order). */
 int fk_values(fk_mesh *mesh, double *values);
 int fk_values_f32(fk_mesh *mesh, float *values);
@@ -183,6 +190,27 @@ int fk_boundary_integral(fk_mesh *mesh, int64_t ns, const int32_t *surfaceT, con
  * projected out so that Lag' u = 0. u: nv. */
 int fk_solve_neumann(fk_mesh *mesh, const double *b, double rtol, int64_t max_iter, double *u,
                      double *multiplier, int64_t *iterations, double *relres);
+
+/* Drop-in for the reference's own native bridge scalarPotential
+ * (Femeko_jl_C++/Magnetostatics/magnetostatics.cpp:49-93, called by @ccall in
+ * Femeko_jl_C++/Magnetostatics/run.jl:66-79): the same argument list and
+ * meaning, so run.jl switches by changing the library and symbol name only.
+ * Context-free and synchronous like the reference: every call uploads the
+ * mesh to GPU 0, solves and frees everything. Arrays are 0-based as run.jl
+ * passes them (t, surfaceT and shell_id shifted by one, run.jl:50-52):
+ * p 3 x nv, t 4 x nt, surfaceT 4 x ns (row 4: surface tag), normal 3 x ns,
+ * VE and mu nt, Hext 3. RHS = BoundaryIntegral over the triangles tagged
+ * shell_id with areas computed from p (src/FEMc.cpp:45-54, 170-200); u (nv,
+ * caller-owned) solves stiffnessMatrix(p, t, VE, mu) u = -RHS by Jacobi-PCG
+ * from zero on the singular system (its constant fixed by sum a_ii u_i = 0,
+ * where PCG from zero stays) to relative residual 1e-10 (the reference's
+ * Eigen default tolerance eps is below what FP64 CG reaches, and its failure
+ * is thrown through extern "C", magnetostatics.cpp:29-37; here a status is
+ * returned and fk_scalar_potential_last_error() explains it). */
+int fk_scalar_potential(double *u, const double *p, const int32_t *t, const int32_t *surfaceT, const double *normal,
+                        int32_t nv, int32_t nt, int32_t ns, const double *VE, const double *mu, const double *Hext,
+                        int32_t shell_id);
+const char *fk_scalar_potential_last_error(void);
 
 /* Demagnetizing field of Micromagnetics/LL.jl:162-209 (magnetostaticField),
  * called once per Landau-Lifshitz step with the matrix fixed: with the

@@ -69,3 +69,13 @@ def test_header_cites_reference_for_each_entry_point():
     for ref in ("src/FEM3D.jl:40-56", "src/FEM3D.jl:59-76", "src/FEM3D.jl:79-119", "src/FEM3D.jl:274-281",
                 "magnetostatics.cpp", "Fluids/fluids3D.jl:142-174", "magneticField.jl:111-130"):
         assert ref in text
+
+
+def test_scalar_potential_bridge_validates_before_gpu():
+    """fk_scalar_potential (context-free, like the reference's scalarPotential)
+    rejects null arrays with a status and a message, before any HIP call."""
+    L = _lib.load()
+    u = np.zeros(4)
+    s = L.fk_scalar_potential(_lib.dptr(u), None, None, None, None, 4, 1, 0, None, None, None, 0)
+    assert s == _lib.FK_ERR_INVALID_ARG
+    assert b"null pointer" in L.fk_scalar_potential_last_error()

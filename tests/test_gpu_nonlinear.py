@@ -12,6 +12,8 @@ import femeko_hip as F
 from femeko_hip import workloads as W
 from femeko_hip.device import DeviceMesh, context
 
+import linesearch_cases as LS
+
 pytestmark = pytest.mark.gpu
 
 
@@ -54,6 +56,29 @@ def test_line_search_matches_oracle(oracle):
     F.invalidate(pr.mesh)
 
 
+@pytest.mark.parametrize("case", range(len(LS.CASES)))
+def test_line_search_closed_form(oracle, case):
+    """fk_line_search against the closed-form accepted step of
+    tests/linesearch_cases.py (alpha* = 2 r0.w/||w||^2, src/FEM3D.jl:123-152)."""
+    astar, min_step, k_expect = LS.CASES[case]
+    pr = W.linear(6)
+    A = oracle.stiffnessMatrix(pr.mesh, pr.mu)
+    S = A.to_scipy().tocsr()
+    rng = np.random.default_rng(100 + case)
+    u, du, RHS = LS.make_case(S, rng, astar, pr.mesh.nv)
+    k, _alpha, res_expect, u_expect = LS.expected(S, u, du, RHS, min_step)
+    assert k == k_expect
+    Ag = F.stiffnessMatrix(pr.mesh, pr.mu)
+    u0 = u.copy()
+    res = F.lineSearch(u, du, Ag, RHS, min_step)
+    assert res == pytest.approx(res_expect, rel=1e-11)
+    if k is None:
+        assert np.array_equal(u, u0)
+    else:
+        np.testing.assert_allclose(u, u_expect, rtol=0, atol=1e-13 * np.abs(u_expect).max())
+    F.invalidate(pr.mesh)
+
+
 def test_femeko_named_dropins(oracle):
     pr = W.linear(10)
     m = pr.mesh
@@ -81,3 +106,18 @@ def test_femeko_named_dropins(oracle):
     sc = oracle.tangent_scale(m, Hv, dmu)
     assert np.all(np.abs(At.data - Atr.nzval) <= 1e-12 * sc)
     F.invalidate(m)
+
+
+def test_line_search_refuses_replaced_values(oracle):
+    """A matrix object whose device values a later assembly replaced is not
+    searched along: lineSearch raises instead of using the other matrix."""
+    pr = W.linear(6)
+    A1 = F.stiffnessMatrix(pr.mesh, pr.mu)
+    A2 = F.stiffnessMatrix(pr.mesh, 2.0 * pr.mu)
+    u = np.zeros(pr.mesh.nv)
+    du = np.ones(pr.mesh.nv)
+    RHS = np.zeros(pr.mesh.nv)
+    with pytest.raises(F.FemekoHIPError):
+        F.lineSearch(u, du, A1, RHS)
+    F.lineSearch(u, du, A2, RHS)  # the current values
+    F.invalidate(pr.mesh)

@@ -12,6 +12,8 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as sla
 
 from femeko_hip import workloads as W
+
+import linesearch_cases as LS
 from femeko_hip.mesh import MESH, MU0, boundary_faces, boundary_node_mask, element_volumes, kuhn_box_mesh, kuhn_counts
 
 
@@ -300,6 +302,30 @@ def test_line_search(oracle):
     # a 50x overshoot is cut back until the residual drops below the old one
     assert res < r0
     assert np.linalg.norm(RHS - S @ u) == pytest.approx(res, rel=1e-12)
+
+
+
+@pytest.mark.parametrize("case", range(len(LS.CASES)))
+def test_line_search_closed_form(oracle, case):
+    """The accepted step of lineSearch is fixed by alpha* = 2 r0.w/||w||^2
+    (tests/linesearch_cases.py): a different step factor, a >= test or a
+    flipped min_step branch changes k, u or the residue."""
+    astar, min_step, k_expect = LS.CASES[case]
+    pr = W.linear(6)
+    A = oracle.stiffnessMatrix(pr.mesh, pr.mu)
+    S = A.to_scipy().tocsr()
+    rng = np.random.default_rng(100 + case)
+    u, du, RHS = LS.make_case(S, rng, astar, pr.mesh.nv)
+    k, alpha, res_expect, u_expect = LS.expected(S, u, du, RHS, min_step)
+    assert k == k_expect
+    u0 = u.copy()
+    res = oracle.lineSearch(u, du, A, RHS, min_step)
+    assert res == pytest.approx(res_expect, rel=1e-12)
+    if k is None:
+        assert np.array_equal(u, u0)
+    else:
+        np.testing.assert_allclose(u, u_expect, rtol=0, atol=1e-13 * np.abs(u_expect).max())
+        assert not np.array_equal(u, u0) or k_expect is None
 
 
 # ---------------------------------------------------------------------------
