@@ -385,8 +385,32 @@ def load_mesh_dump(folder) -> MESH:
             mesh.normal = _read_matrix(path("normal.txt"), 3, np.float64)
             if mesh.normal.shape[1] != ns:
                 raise ValueError("normal.txt: one normal per surface triangle expected")
-        mesh.shell_id = tuple(int(v) for v in np.unique(sT[3])) if ns else ()
+        mesh.shell_id = outer_shell_tags(t, sT)
     return mesh
+
+
+def outer_shell_tags(t: np.ndarray, surfaceT: np.ndarray) -> tuple:
+    """Surface tags (row 4 of surfaceT) whose triangles all lie on the outer
+    boundary, i.e. are faces of exactly one element. Femeko's surfaceT holds
+    every surface triangle Gmsh meshed, internal interfaces between bodies
+    included (src/mesh.jl:147-166), while its shell_id keeps only the outer
+    surfaces of the bounding shell (unifyModel's setdiff with the inner cells'
+    boundary, src/geometry.jl:225-230): a dump does not store shell_id, so
+    it is recovered from the connectivity."""
+    sT = np.asarray(surfaceT)
+    if sT.shape[1] == 0:
+        return ()
+    t0 = np.asarray(t, dtype=np.int64) - 1
+    local = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])
+    faces = np.sort(t0.T[:, local].reshape(-1, 3), axis=1)
+    keys, cnt = np.unique(faces, axis=0, return_counts=True)
+    outer = keys[cnt == 1]
+    nv = int(t0.max()) + 1 if t0.size else 0
+    code = lambda f: (f[:, 0] * nv + f[:, 1]) * nv + f[:, 2]  # noqa: E731
+    tri = np.sort(np.asarray(sT[:3], dtype=np.int64).T - 1, axis=1)
+    on_outer = np.isin(code(tri), code(outer))
+    tags = np.asarray(sT[3], dtype=np.int64)
+    return tuple(int(g) for g in np.unique(tags) if on_outer[tags == g].all())
 
 
 def _dump_rows(path) -> int:

@@ -94,3 +94,54 @@ def test_dumped_mesh_assembles_on_the_gpu_as_the_oracle(oracle, tmp_path):
         assert np.all(np.abs(dm.values() - A.nzval) <= 1e-12 * bound)
     finally:
         dm.close()
+
+
+def _with_interface(n=4):
+    """A Kuhn box whose surfaceT also holds the interface between an inner
+    body (elements with centroid in [-0.5,0.5]^3, tag 2 triangles) and the
+    air around it, as Gmsh meshes every surface of a multi-body model."""
+    m = M.boundary_faces(M.kuhn_box_mesh(n, jitter=0.0, seed=1))
+    body = M.elements_in_box(m, (-0.5,) * 3, (0.5,) * 3)
+    t0 = (m.t - 1).T.astype(np.int64)
+    local = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])
+    faces = np.sort(t0[:, local].reshape(-1, 3), axis=1)
+    owner = np.repeat(np.arange(m.nt), 4)
+    _, inv, cnt = np.unique(faces, axis=0, return_inverse=True, return_counts=True)
+    inv = inv.ravel()
+    iface = []
+    for key in np.nonzero(cnt == 2)[0]:
+        e = owner[inv == key]
+        if body[e[0]] != body[e[1]]:
+            iface.append(faces[np.nonzero(inv == key)[0][0]])
+    iface = np.array(iface, dtype=np.int64)
+    P = m.p
+    nrm = np.cross((P[:, iface[:, 1]] - P[:, iface[:, 0]]).T, (P[:, iface[:, 2]] - P[:, iface[:, 0]]).T).T
+    area = np.linalg.norm(nrm, axis=0) / 2
+    nrm /= 2 * area
+    sT = np.vstack([(iface.T + 1), np.full((1, iface.shape[0]), 2)]).astype(np.int32)
+    outer = (np.asarray(m.surfaceT).copy(), np.asarray(m.normal).copy(), np.asarray(m.AE).copy())
+    m.surfaceT = np.asfortranarray(np.hstack([outer[0], sT]))
+    m.normal = np.asfortranarray(np.hstack([outer[1], nrm]))
+    m.AE = np.concatenate([outer[2], area])
+    return m, outer, iface.shape[0]
+
+
+def test_dump_shell_id_keeps_only_outer_surfaces(tmp_path, oracle):
+    """surfaceT with an internal interface tag: the inferred shell_id is the
+    outer boundary's tag only (src/geometry.jl:225-230), so BoundaryIntegral
+    (src/FEM3D.jl:284-297) over it equals the load of the outer faces alone."""
+    m, outer, n_iface = _with_interface()
+    assert n_iface > 0
+    M.save_mesh_dump(m, tmp_path)
+    r = M.load_mesh_dump(tmp_path)
+    assert r.shell_id == (1,)
+    F = np.array([1.0, -2.0, 0.5])
+    got = oracle.BoundaryIntegral(r, F, r.shell_id)
+    ref_mesh = M.MESH(p=m.p, t=m.t, VE=m.VE)
+    ref_mesh.surfaceT, ref_mesh.normal, ref_mesh.AE = (np.asfortranarray(outer[0]), np.asfortranarray(outer[1]),
+                                                       outer[2])
+    want = oracle.BoundaryIntegral(ref_mesh, F, (1,))
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-15)
+    # all tags would add the interface's flux on the body's surface nodes
+    both = oracle.BoundaryIntegral(r, F, (1, 2))
+    assert np.abs(both - want).max() > 1e-3
