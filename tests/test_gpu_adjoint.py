@@ -14,8 +14,11 @@ from femeko_hip import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def test_adjoint_matches_oracle_and_fd(oracle):
-    pr = W.adjoint(n=8, seed=31)
+@pytest.mark.parametrize("n,seed", [(8, 31), (20, 20)])
+def test_adjoint_matches_oracle_and_fd(oracle, n, seed):
+    """n = 20, seed 20 is the config-0 mesh (Kuhn n=20, jitter 0.2h, seed 20)
+    with the adjoint workload's design rho and target."""
+    pr = W.adjoint(n=n, seed=seed)
     m = pr.mesh
     rho = pr.meta["rho"]
     Hs = pr.meta["Hstar"]
@@ -28,7 +31,7 @@ def test_adjoint_matches_oracle_and_fd(oracle):
     assert np.abs(grad - grad_ref).max() <= 1e-7 * scale
     _n, g2 = pullback(2.5)
     np.testing.assert_allclose(g2, 2.5 * grad)
-    rng = np.random.default_rng(31)
+    rng = np.random.default_rng(seed)
     # largest-gradient elements plus random ones
     idx = np.concatenate([np.argsort(-np.abs(grad))[:4], rng.choice(m.nt, 4, replace=False)])
     h = 1e-6

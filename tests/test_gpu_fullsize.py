@@ -10,7 +10,12 @@ properties that hold at any size (the oracle is too slow at these sizes):
   at the returned potential;
 - 64 meshes x 3 fields: every system below relres 1e-10, and sampled systems
   equal to single-mesh FP64 solves;
-- n = 80 adjoint: the gradient against central differences on 4 elements.
+- n = 80 adjoint: the gradient against central differences on 32 seeded
+  elements (the 16 largest gradients and 16 uniform draws).
+
+test_boxes160_against_oracle checks the headline workload against the
+oracle's FP64 Jacobi-PCG on the host (OpenMP, rtol 1e-13): phi, |H| and the
+energy of the FP32-refined solve within 1e-9.
 """
 import numpy as np
 import pytest
@@ -116,12 +121,48 @@ def test_adjoint80_central_differences():
         _J, grad, _its = ADJ.fieldObjectiveGradient(m, rho, tg, Hs, pr.mask, pr.g)
         gmax = float(np.abs(grad).max())
         h = 1e-6
-        for e in np.argsort(-np.abs(grad))[:4]:
+        rng = np.random.default_rng(80)
+        idx = np.concatenate([np.argsort(-np.abs(grad))[:16], rng.choice(m.nt, 16, replace=False)])
+        for e in idx:
             rp, rm = rho.copy(), rho.copy()
             rp[e] += h
             rm[e] -= h
             fd = (ADJ.fieldObjective(m, rp, tg, Hs, pr.mask, pr.g, rtol=1e-13)
                   - ADJ.fieldObjective(m, rm, tg, Hs, pr.mask, pr.g, rtol=1e-13)) / (2 * h)
             assert abs(fd - grad[e]) < 1e-4 * gmax, (e, fd, grad[e])
+            if abs(grad[e]) >= 1e-2 * gmax:  # below, the differences' rounding noise dominates
+                assert abs(fd - grad[e]) < 2e-4 * abs(grad[e]), (e, fd, grad[e])
     finally:
         fem3d.invalidate(m)
+
+
+def test_boxes160_against_oracle(oracle):
+    """The FP32-refined solve (rtol 5e-11) of the n = 160 workload against the
+    oracle: its assembly (src/FEM3D.jl:40-76 restated, OpenMP) and its FP64
+    Jacobi-PCG to rtol 1e-13 on the host cores; phi (relative L2), |H|
+    (VE-weighted relative L2) and the energy within 1e-9."""
+    pr = W.boxes(160)
+    m = pr.mesh
+    A = oracle.stiffnessMatrix(m, pr.mu, nthreads=oracle.threads())
+    ref, it, rel, s = oracle.pcg_dirichlet(A, pr.mask, pr.g, 1e-13, exact=False)
+    assert s == 0 and rel < 1e-13
+    del A
+    _H, Hn_ref = oracle.field(m, ref)
+    E_ref = oracle.energy(m, Hn_ref, pr.mu)
+    dm = DeviceMesh(m, context(0))
+    try:
+        dm.set_coefficient(pr.mu)
+        dm.assemble()
+        dm.set_dirichlet(pr.mask, pr.g)
+        rep = dm.solve("f32", rtol=5e-11)
+        assert rep.relres < 5e-11
+        phi = dm.potential()
+        _H, Hn, E = dm.field(want_H=False, want_Hn=True, want_energy=True)
+    finally:
+        dm.close()
+    VE = m.VE
+    err_phi = np.linalg.norm(phi - ref) / np.linalg.norm(ref)
+    err_H = np.sqrt(np.sum(VE * (Hn - Hn_ref) ** 2) / np.sum(VE * Hn_ref ** 2))
+    err_E = abs(E - E_ref) / abs(E_ref)
+    print(f"n=160 vs oracle (PCG {it} its to {rel:.1e}): phi {err_phi:.2e} |H| {err_H:.2e} energy {err_E:.2e}")
+    assert err_phi < 1e-9 and err_H < 1e-9 and err_E < 1e-9
