@@ -265,6 +265,9 @@ namespace fk {
 
 // ---- helpers implemented in the .hip files --------------------------------
 void set_device(fk_context *ctx);
+// per-device address of the allocation-free entry points' static scratch
+// (call with the device current)
+void noalloc_setup(int device);
 void build_mesh(fk_mesh *m, const double *p, const int32_t *t, int32_t base, const double *VE);
 void element_volumes(fk_mesh *m, double *VE_host);
 void assemble(fk_mesh *m, int method);

@@ -330,6 +330,7 @@ int fk_context_create(int device, fk_context **out) {
         const bool profiled = tool && tool[0] != '\0';
         ctx->graphs = g ? g[0] != '0' : !profiled;
         FK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        fk::noalloc_setup(device);
     } catch (const fk::Error &e) {
         delete ctx;
         return e.code;

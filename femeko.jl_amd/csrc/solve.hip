@@ -780,6 +780,38 @@ __global__ __launch_bounds__(kBlock) void k_field(const int4 *__restrict__ tet, 
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
+// ---- allocation-free field recovery ------------------------------------------
+// Block partials of the energy sum live in this file-scope array, one copy per
+// GPU for the life of the process. Host code never passes `scratch` itself to
+// a kernel (in host code the name is a host-side stand-in): it uses the
+// address hipGetSymbolAddress returned for the device, kept per device in
+// g_scratch. Every call on a GPU shares the array: the field kernel
+// overwrites the partials of all its blocks and the summing kernel reads only
+// those, so no clearing is needed; calls that overlap on one GPU (two
+// streams, two contexts) race on it.
+constexpr int kScratchPartials = 4096;
+static __device__ __align__(256) double scratch[kScratchPartials];
+constexpr int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices] = {};
+
+// Called with `device` current (fk_context_create), once per device.
+void noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) fail(FK_ERR_UNSUPPORTED, "device index beyond the library's table");
+    if (g_scratch[device]) return;
+    void *ptr = nullptr;
+    FK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device] = static_cast<double *>(ptr);
+}
+
+// energy = part[0] + part[1] + ... in index order (the order fk_field sums
+// the same partials on the host), one thread
+__global__ void k_sum_in_order(const double *__restrict__ part, int n, double *__restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    *out = s;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -2049,6 +2081,27 @@ int fk_field(fk_mesh *m, double *H, double *Hn, double *energy) {
     return guard(m->ctx, "fk_field", [&] {
         fk::set_device(m->ctx);
         fk::field(m, H, Hn, energy);
+    });
+}
+
+int fk_field_noalloc(fk_mesh *m, const double *phi, double *H, double *Hn, double *energy) {
+    if (!m) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_field_noalloc", [&] {
+        if (!phi) fk::fail(FK_ERR_INVALID_ARG, "phi (device, nv values) is required");
+        if (energy && !m->have_coef) fk::fail(FK_ERR_STATE, "energy needs a coefficient");
+        const int nb = std::max(1, m->nblocks);
+        if (nb > fk::kScratchPartials)
+            fk::fail(FK_ERR_UNSUPPORTED, "the field kernel's " + std::to_string(nb) + " block partials exceed the " +
+                                             std::to_string(fk::kScratchPartials) + " the static scratch holds");
+        const int dev = m->ctx->device;
+        double *part = dev >= 0 && dev < fk::kMaxDevices ? fk::g_scratch[dev] : nullptr;
+        if (!part) fk::fail(FK_ERR_STATE, "no scratch address for this device (context set-up failed)");
+        fk::set_device(m->ctx);
+        hipStream_t st = m->ctx->stream;
+        fk::k_field<<<nb, fk::kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->perm.get(), phi,
+                                               energy ? m->coef.get() : nullptr, m->nt, H, Hn, part);
+        if (energy) fk::k_sum_in_order<<<1, 64, 0, st>>>(part, nb, energy);
+        FK_HIP(hipGetLastError());
     });
 }
 

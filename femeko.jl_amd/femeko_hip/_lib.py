@@ -77,6 +77,7 @@ SIGNATURES = {
     "fk_potential": (_c.c_int, [_vp, _dp]),
     "fk_set_potential": (_c.c_int, [_vp, _dp]),
     "fk_field": (_c.c_int, [_vp, _dp, _dp, _dp]),
+    "fk_field_noalloc": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "fk_boundary_integral": (_c.c_int, [_vp, _c.c_int64, _ip, _dp, _dp, _dp, _ip, _c.c_int32, _c.c_int32, _dp]),
     "fk_scalar_potential": (_c.c_int, [_dp, _dp, _ip, _ip, _dp, _c.c_int32, _c.c_int32, _c.c_int32, _dp, _dp, _dp,
                                        _c.c_int32]),

@@ -176,6 +176,21 @@ int fk_set_potential(fk_mesh *mesh, const double *phi);
  * NULL; src/magneticProperties.jl:268-273 with the element permeability). */
 int fk_field(fk_mesh *mesh, double *H, double *Hn, double *energy);
 
+/* Allocation-free variant of fk_field, with no counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). All
+ * pointers are DEVICE pointers: phi (nv, caller node order) in, H (3 x nt),
+ * Hn (nt) and energy (one double) out, each output may be NULL. The same
+ * kernel as fk_field writes H and |H|; the energy's block partials go to a
+ * static per-GPU scratch of 4096 doubles and are summed on the device in the
+ * order fk_field sums them, so every output equals fk_field's bit for bit.
+ * Enqueued on the context's stream and not waited for (fk_context_synchronize
+ * waits); no allocation, copy or synchronization. A mesh whose field kernel
+ * needs more block partials than the scratch holds returns
+ * FK_ERR_UNSUPPORTED. Calls that overlap on one GPU (two streams or
+ * contexts) race on the shared scratch. */
+int fk_field_noalloc(fk_mesh *mesh, const double *phi, double *H, double *Hn, double *energy);
+
 /* ---- Neumann flux load and Lagrange saddle solve ----------------------------
  * BoundaryIntegral(mesh, F, shell_id) (src/FEM3D.jl:284-297): for surface
  * triangles whose tag (row 4 of surfaceT, 4 x ns) is one of shell_ids,
