@@ -532,6 +532,7 @@ void tangential(fk_mesh *m, const double *H, const double *dmu, double *nzval_ho
     FK_HIP(hipGetLastError());
     FK_HIP(hipMemcpyAsync(nzval_host, val.get(), m->nnz * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
+    values_to_caller(m, nzval_host);
 }
 
 void lagrange(fk_mesh *m, double *C_host) {
@@ -545,8 +546,10 @@ void lagrange(fk_mesh *m, double *C_host) {
                                                            C.get(), e0, e1);
     }
     FK_HIP(hipGetLastError());
-    FK_HIP(hipMemcpyAsync(C_host, C.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    NodesOut<double> out(m, C_host);
+    FK_HIP(hipMemcpyAsync(out.ptr, C.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
+    out.commit();
 }
 
 }  // namespace fk
@@ -582,6 +585,7 @@ int fk_values(fk_mesh *m, double *values) {
         FK_HIP(hipMemcpyAsync(values, m->val64.get(), m->nnz * sizeof(double), hipMemcpyDeviceToHost,
                               m->ctx->stream));
         FK_HIP(hipStreamSynchronize(m->ctx->stream));
+        fk::values_to_caller(m, values);
     });
 }
 
@@ -594,6 +598,7 @@ int fk_values_f32(fk_mesh *m, float *values) {
         FK_HIP(hipMemcpyAsync(values, m->val32.get(), m->nnz * sizeof(float), hipMemcpyDeviceToHost,
                               m->ctx->stream));
         FK_HIP(hipStreamSynchronize(m->ctx->stream));
+        fk::values_to_caller(m, values);
     });
 }
 

@@ -98,9 +98,10 @@ void demag_field(fk_mesh *m, const uint8_t *inside_host, const double *M_host, c
     m->demag_Hd.ensure(3 * nv);
     DevBuf<uint8_t> &in_caller = m->demag_in_caller, &inside = m->demag_inside;
     DevBuf<double> &M = m->demag_M, &vol = m->demag_vol, &Hd = m->demag_Hd;
+    const NodesIn<double> M_in(m, M_host, 3), vol_in(m, vol_host);
     FK_HIP(hipMemcpyAsync(in_caller.get(), inside_host, nt, hipMemcpyHostToDevice, st));
-    FK_HIP(hipMemcpyAsync(M.get(), M_host, 3 * nv * sizeof(double), hipMemcpyHostToDevice, st));
-    FK_HIP(hipMemcpyAsync(vol.get(), vol_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(M.get(), M_in.ptr, 3 * nv * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(vol.get(), vol_in.ptr, nv * sizeof(double), hipMemcpyHostToDevice, st));
     {
         // checked before anything the call keeps (the previous potential) changes
         DevBuf<int32_t> bad(2);
@@ -137,8 +138,10 @@ void demag_field(fk_mesh *m, const uint8_t *inside_host, const double *M_host, c
     }
     k_div_volumes<<<grid_for(3 * nv, 256), 256, 0, st>>>(Hd.get(), vol.get(), nv);
     FK_HIP(hipGetLastError());
-    FK_HIP(hipMemcpyAsync(Hd_host, Hd.get(), 3 * nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    NodesOut<double> Hd_out(m, Hd_host, 3);
+    FK_HIP(hipMemcpyAsync(Hd_out.ptr, Hd.get(), 3 * nv * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
+    Hd_out.commit();
     if (iters_out) *iters_out = res.iters;
     if (relres_out) *relres_out = res.relres;
 }

@@ -251,6 +251,18 @@ struct fk_mesh {
     fk::DevBuf<double> dmu_nat;       // dmu in caller order (gather Newton assembly)
     bool coef_nat_valid = false;
 
+    // internal node numbering (reorder.hip): internal node k is the caller's
+    // node node_perm[k], node_inv is the inverse; empty unless reordered
+    bool reordered = false;
+    std::vector<int32_t> node_perm, node_inv;
+    fk::DevBuf<int32_t> d_node_perm;   // node_perm on the device
+    fk::DevBuf<double> phi_internal;  // fk_field_noalloc's potential in the internal order
+    double mean_dist_caller = 0.0, mean_dist_internal = 0.0;
+    // the CSR pattern in the caller's numbering, built on the first export:
+    // caller_map[q] = internal CSR position of caller position q
+    bool caller_csr_built = false;
+    std::vector<int32_t> caller_rowptr, caller_colidx, caller_map;
+
     int lanes = 16;                   // lanes per CSR row in SpMV
     int sell_block = 1024;            // threads per block of the SELL SpMV
     int sell_unroll = 1;              // slices in flight per wave in the SELL SpMV
@@ -311,6 +323,60 @@ struct NeumannResult {
 NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter, const double *u0);
 void phi_axpy_f64(fk_mesh *m, const double *x, double s);
 void k_init_phi_dev(fk_mesh *m, int keep_free);
+
+// ---- node numbering (reorder.hip) ------------------------------------------
+void validate_input(const double *p, const int32_t *t, int64_t nv, int64_t nt, int32_t base);
+// empty: keep the caller's numbering; else perm[k] = caller node at internal k
+std::vector<int32_t> choose_numbering(const int32_t *t, int64_t nt, int64_t nv, int32_t base, int mode,
+                                      double *mean_caller, double *mean_internal);
+void build_caller_csr(fk_mesh *m);
+
+// Per-node host arrays (comps values per node, node-major) between the
+// caller's node order and the internal one. NodesIn.ptr is the array in the
+// internal order (the caller's own array when the mesh is not renumbered).
+template <typename T>
+struct NodesIn {
+    std::vector<T> buf;
+    const T *ptr;
+    NodesIn(const fk_mesh *m, const T *caller, int comps = 1) : ptr(caller) {
+        if (!m->reordered || !caller) return;
+        buf.resize(static_cast<size_t>(m->nv) * comps);
+        const int32_t *perm = m->node_perm.data();
+        for (int64_t k = 0; k < m->nv; ++k)
+            for (int c = 0; c < comps; ++c) buf[k * comps + c] = caller[static_cast<int64_t>(perm[k]) * comps + c];
+        ptr = buf.data();
+    }
+};
+// NodesOut.ptr receives the internal order; commit() writes the caller's
+// array (call after the device copy into ptr has completed).
+template <typename T>
+struct NodesOut {
+    const fk_mesh *m;
+    T *caller;
+    int comps;
+    std::vector<T> buf;
+    T *ptr;
+    NodesOut(const fk_mesh *mesh, T *out, int c = 1) : m(mesh), caller(out), comps(c), ptr(out) {
+        if (!m->reordered || !out) return;
+        buf.resize(static_cast<size_t>(m->nv) * comps);
+        ptr = buf.data();
+    }
+    void commit() {
+        if (!m->reordered || !caller) return;
+        const int32_t *perm = m->node_perm.data();
+        for (int64_t k = 0; k < m->nv; ++k)
+            for (int c = 0; c < comps; ++c) caller[static_cast<int64_t>(perm[k]) * comps + c] = buf[k * comps + c];
+    }
+};
+// values of the internal CSR order -> the caller's CSR order (in place)
+template <typename T>
+void values_to_caller(fk_mesh *m, T *vals) {
+    if (!m->reordered) return;
+    build_caller_csr(m);
+    std::vector<T> tmp(vals, vals + m->nnz);
+    const int32_t *map = m->caller_map.data();
+    for (int64_t q = 0; q < m->nnz; ++q) vals[q] = tmp[map[q]];
+}
 
 // Run body; map library errors to a status and remember the message.
 template <typename F>

@@ -133,10 +133,7 @@ void setup_partition(fk_mesh *m) {
     m->chunk = ((m->nv + nb - 1) / nb + 63) / 64 * 64;
 }
 
-void build_mesh(fk_mesh *m, const double *p, const int32_t *t, int32_t base, const double *VE) {
-    const int64_t nv = m->nv, nt = m->nt;
-    hipStream_t st = m->ctx->stream;
-    // ---- validate on the host
+void validate_input(const double *p, const int32_t *t, int64_t nv, int64_t nt, int32_t base) {
     for (int64_t i = 0; i < 3 * nv; ++i)
         if (!std::isfinite(p[i])) fail(FK_ERR_NONFINITE, "non-finite node coordinate");
     for (int64_t e = 0; e < nt; ++e) {
@@ -148,6 +145,12 @@ void build_mesh(fk_mesh *m, const double *p, const int32_t *t, int32_t base, con
                 if (nd[i] == nd[j]) fail(FK_ERR_DEGENERATE, "element " + std::to_string(e + base) + " repeats a node");
         }
     }
+}
+
+// p, t already validated (and in the internal numbering)
+void build_mesh(fk_mesh *m, const double *p, const int32_t *t, int32_t base, const double *VE) {
+    const int64_t nv = m->nv, nt = m->nt;
+    hipStream_t st = m->ctx->stream;
     // ---- colour and order elements by colour
     std::vector<uint8_t> colour(static_cast<size_t>(nt));
     int32_t nc = 0;
@@ -364,7 +367,12 @@ int fk_context_synchronize(fk_context *ctx) {
 
 int fk_mesh_create(fk_context *ctx, const double *p, int64_t nv, const int32_t *t, int64_t nt, int32_t index_base,
                    const double *VE, fk_mesh **out) {
-    if (!ctx || !out) return FK_ERR_INVALID_ARG;
+    return fk_mesh_create_ex(ctx, p, nv, t, nt, index_base, VE, FK_REORDER_AUTO, out);
+}
+
+int fk_mesh_create_ex(fk_context *ctx, const double *p, int64_t nv, const int32_t *t, int64_t nt,
+                      int32_t index_base, const double *VE, int32_t reorder, fk_mesh **out) {
+    if (!ctx || !out || reorder < FK_REORDER_AUTO || reorder > FK_REORDER_RCM) return FK_ERR_INVALID_ARG;
     *out = nullptr;
     if (!p || !t || nv < 4 || nt < 1 || (index_base != 0 && index_base != 1)) {
         ctx->err = "fk_mesh_create: bad arguments";
@@ -381,7 +389,30 @@ int fk_mesh_create(fk_context *ctx, const double *p, int64_t nv, const int32_t *
     m->nt = nt;
     int s = guard(ctx, "fk_mesh_create", [&] {
         fk::set_device(ctx);
-        fk::build_mesh(m, p, t, index_base, VE);
+        fk::validate_input(p, t, nv, nt, index_base);
+        std::vector<int32_t> perm =
+            fk::choose_numbering(t, nt, nv, index_base, reorder, &m->mean_dist_caller, &m->mean_dist_internal);
+        if (perm.empty()) {
+            fk::build_mesh(m, p, t, index_base, VE);
+            return;
+        }
+        // the mesh in the internal numbering: p's columns permuted, t mapped
+        std::vector<int32_t> inv(static_cast<size_t>(nv));
+        for (int64_t k = 0; k < nv; ++k) inv[perm[k]] = static_cast<int32_t>(k);
+        std::vector<double> pi(static_cast<size_t>(3 * nv));
+        for (int64_t k = 0; k < nv; ++k)
+            for (int c = 0; c < 3; ++c) pi[3 * k + c] = p[3 * static_cast<int64_t>(perm[k]) + c];
+        std::vector<int32_t> ti(static_cast<size_t>(4 * nt));
+        for (int64_t q = 0; q < 4 * nt; ++q) ti[q] = inv[t[q] - index_base];
+        m->reordered = true;
+        m->node_perm = std::move(perm);
+        m->node_inv = std::move(inv);
+        fk::build_mesh(m, pi.data(), ti.data(), 0, VE);
+        // device copies for the entry points that take device arrays in the
+        // caller's node order (allocated here: those calls allocate nothing)
+        m->d_node_perm.alloc(nv);
+        m->phi_internal.alloc(nv);
+        FK_HIP(hipMemcpy(m->d_node_perm.get(), m->node_perm.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice));
     });
     if (s != FK_OK) {
         delete m;
@@ -426,9 +457,15 @@ int fk_pattern(fk_mesh *m, int32_t base, int32_t *rowptr, int32_t *colidx) {
     return guard(m->ctx, "fk_pattern", [&] {
         fk::set_device(m->ctx);
         hipStream_t st = m->ctx->stream;
-        FK_HIP(hipMemcpyAsync(rowptr, m->rowptr.get(), (m->nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        FK_HIP(hipMemcpyAsync(colidx, m->colidx.get(), m->nnz * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        FK_HIP(hipStreamSynchronize(st));
+        if (m->reordered) {
+            fk::build_caller_csr(m);
+            std::copy(m->caller_rowptr.begin(), m->caller_rowptr.end(), rowptr);
+            std::copy(m->caller_colidx.begin(), m->caller_colidx.end(), colidx);
+        } else {
+            FK_HIP(hipMemcpyAsync(rowptr, m->rowptr.get(), (m->nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            FK_HIP(hipMemcpyAsync(colidx, m->colidx.get(), m->nnz * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            FK_HIP(hipStreamSynchronize(st));
+        }
         if (base) {
             for (int64_t i = 0; i <= m->nv; ++i) rowptr[i] += 1;
             for (int64_t i = 0; i < m->nnz; ++i) colidx[i] += 1;

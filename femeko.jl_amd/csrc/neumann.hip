@@ -136,14 +136,24 @@ __global__ __launch_bounds__(256) void k_lagrange_n(const int4 *__restrict__ tet
     C[nd.w] += v;
 }
 
-void boundary_integral(fk_mesh *m, int64_t ns, const int32_t *sT, const double *nrm, const double *AE,
+void boundary_integral(fk_mesh *m, int64_t ns, const int32_t *sT_caller, const double *nrm, const double *AE,
                        const double *F, const int32_t *ids, int nids, int32_t base, double *rhs_host) {
     hipStream_t st = m->ctx->stream;
     for (int64_t s = 0; s < ns; ++s)
         for (int i = 0; i < 3; ++i) {
-            const int64_t v = static_cast<int64_t>(sT[4 * s + i]) - base;
+            const int64_t v = static_cast<int64_t>(sT_caller[4 * s + i]) - base;
             if (v < 0 || v >= m->nv) fail(FK_ERR_INDEX, "surface triangle node number out of range");
         }
+    // triangle nodes in the internal numbering
+    std::vector<int32_t> sT_int;
+    const int32_t *sT = sT_caller;
+    if (m->reordered && ns > 0) {
+        sT_int.assign(sT_caller, sT_caller + 4 * ns);
+        for (int64_t s = 0; s < ns; ++s)
+            for (int i = 0; i < 3; ++i) sT_int[4 * s + i] = m->node_inv[sT_caller[4 * s + i] - base] + base;
+        sT = sT_int.data();
+    }
+    NodesOut<double> rhs_out(m, rhs_host);
     DevBuf<int32_t> dT(4 * std::max<int64_t>(ns, 1)), dids(std::max(nids, 1));
     DevBuf<double> dn(3 * std::max<int64_t>(ns, 1)), dA(std::max<int64_t>(ns, 1)), rhs(m->nv);
     if (ns > 0) {
@@ -157,8 +167,9 @@ void boundary_integral(fk_mesh *m, int64_t ns, const int32_t *sT, const double *
         k_boundary_integral<<<grid_for(ns, 256), 256, 0, st>>>(ns, dT.get(), dn.get(), dA.get(), F[0], F[1], F[2],
                                                                dids.get(), nids, base, rhs.get());
     FK_HIP(hipGetLastError());
-    FK_HIP(hipMemcpyAsync(rhs_host, rhs.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(rhs_out.ptr, rhs.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
+    rhs_out.commit();
 }
 
 static void sums3(fk_mesh *m, const double *a, const double *c, double out[3]) {
@@ -245,10 +256,13 @@ void solve_neumann(fk_mesh *m, const double *b_host, double rtol, int64_t max_it
     if (!m->have_values) fail(FK_ERR_STATE, "matrix not assembled");
     hipStream_t st = m->ctx->stream;
     ensure_work_f64(m);
-    FK_HIP(hipMemcpyAsync(m->w64.r.get(), b_host, m->nv * sizeof(double), hipMemcpyHostToDevice, st));
+    const NodesIn<double> b_in(m, b_host);
+    NodesOut<double> u_out(m, u_host);
+    FK_HIP(hipMemcpyAsync(m->w64.r.get(), b_in.ptr, m->nv * sizeof(double), hipMemcpyHostToDevice, st));
     const NeumannResult res = solve_neumann_dev(m, rtol, max_iter, nullptr);
-    FK_HIP(hipMemcpyAsync(u_host, m->w64.x.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(u_out.ptr, m->w64.x.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
+    u_out.commit();
     if (mult_out) *mult_out = res.multiplier;
     if (iters_out) *iters_out = res.iters;
     if (relres_out) *relres_out = res.relres;
@@ -295,11 +309,12 @@ static void scalar_potential(double *u, const double *p, const int32_t *t, const
     std::vector<double> rhs(static_cast<size_t>(nv));
     boundary_integral(m, ns, surfaceT, normal, AE.data(), Hext, &shell_id, 1, 0, rhs.data());
     for (auto &v : rhs) v = -v;
+    const NodesIn<double> rhs_in(m, rhs.data());
     upload_coefficient(m, mu);
     assemble(m, FK_ASM_GATHER);
     ensure_work_f64(m);
     hipStream_t st = m->ctx->stream;
-    FK_HIP(hipMemcpyAsync(m->w64.r.get(), rhs.data(), nv * sizeof(double), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(m->w64.r.get(), rhs_in.ptr, nv * sizeof(double), hipMemcpyHostToDevice, st));
     solve_neumann_dev(m, rtol, std::max<int64_t>(1000, 20 * nv), nullptr);
     // fix the constant: sum_i a_ii u_i = 0
     const int nb = std::max(1, m->nblocks);
@@ -316,8 +331,10 @@ static void scalar_potential(double *u, const double *p, const int32_t *t, const
     }
     k_shift<<<grid_for(nv, 256), 256, 0, st>>>(m->w64.x.get(), dd > 0.0 ? du / dd : 0.0, nv);
     FK_HIP(hipGetLastError());
-    FK_HIP(hipMemcpyAsync(u, m->w64.x.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    NodesOut<double> u_out(m, u);
+    FK_HIP(hipMemcpyAsync(u_out.ptr, m->w64.x.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
+    u_out.commit();
 }
 
 thread_local std::string g_bridge_error;

@@ -424,9 +424,13 @@ double line_search(fk_mesh *m, double *u_host, const double *du_host, const doub
     hipStream_t st = m->ctx->stream;
     const int64_t nv = m->nv;
     DevBuf<double> u(nv), du(nv), rhs(nv), y(nv);
-    FK_HIP(hipMemcpyAsync(u.get(), u_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
-    FK_HIP(hipMemcpyAsync(du.get(), du_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
-    FK_HIP(hipMemcpyAsync(rhs.get(), rhs_host, nv * sizeof(double), hipMemcpyHostToDevice, st));
+    {
+        const NodesIn<double> ui(m, u_host), dui(m, du_host), ri(m, rhs_host);
+        FK_HIP(hipMemcpyAsync(u.get(), ui.ptr, nv * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(du.get(), dui.ptr, nv * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(rhs.get(), ri.ptr, nv * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipStreamSynchronize(st));
+    }
     const int nb = std::max(1, m->nblocks);
     m->part_misc.ensure(nb);
     auto norm_of = [&](const double *x) {
@@ -448,8 +452,10 @@ double line_search(fk_mesh *m, double *u_host, const double *du_host, const doub
         residue = norm_of(y.get());
     }
     if (!(alf < min_step)) {
-        FK_HIP(hipMemcpyAsync(u_host, y.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
+        NodesOut<double> uo(m, u_host);
+        FK_HIP(hipMemcpyAsync(uo.ptr, y.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
         FK_HIP(hipStreamSynchronize(st));
+        uo.commit();
     }
     return residue;
 }

@@ -803,6 +803,14 @@ void noalloc_setup(int device) {
     g_scratch[device] = static_cast<double *>(ptr);
 }
 
+// out[k] = in[perm[k]]: a per-node device array from the caller's node order
+// to the internal one
+__global__ void k_gather_nodes(const double *__restrict__ in, const int32_t *__restrict__ perm, int64_t n,
+                               double *__restrict__ out) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = in[perm[k]];
+}
+
 // energy = part[0] + part[1] + ... in index order (the order fk_field sums
 // the same partials on the host), one thread
 __global__ void k_sum_in_order(const double *__restrict__ part, int n, double *__restrict__ out) {
@@ -1818,8 +1826,12 @@ void k_init_phi_dev(fk_mesh *m, int keep_free) {
     FK_HIP(hipGetLastError());
 }
 
-void set_dirichlet(fk_mesh *m, const uint8_t *mask, const double *g) {
+void set_dirichlet(fk_mesh *m, const uint8_t *mask_caller, const double *g_caller) {
     hipStream_t st = m->ctx->stream;
+    const NodesIn<uint8_t> mask_in(m, mask_caller);
+    const NodesIn<double> g_in(m, g_caller);
+    const uint8_t *mask = mask_in.ptr;
+    const double *g = g_in.ptr;
     int64_t nfixed = 0;
     for (int64_t i = 0; i < m->nv; ++i) {
         if (mask[i]) {
@@ -1942,10 +1954,13 @@ void solve_rhs(fk_mesh *m, const double *b, double rtol, int64_t max_iter, doubl
         m->dinv64_valid = true;
     }
     PcgWork<double> &w = m->w64;
-    FK_HIP(hipMemcpyAsync(w.r.get(), b, m->nv * sizeof(double), hipMemcpyHostToDevice, st));
+    const NodesIn<double> b_in(m, b);
+    NodesOut<double> x_out(m, x);
+    FK_HIP(hipMemcpyAsync(w.r.get(), b_in.ptr, m->nv * sizeof(double), hipMemcpyHostToDevice, st));
     const CgState s = run_pcg<double>(m, rtol, max_iter, -1.0);
-    FK_HIP(hipMemcpyAsync(x, w.x.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(x_out.ptr, w.x.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
+    x_out.commit();
     if (iters) *iters = s.iters;
     if (relres) *relres = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
     if (s.status != FK_OK) fail(s.status, std::string("solve stopped: ") + fk_status_string(s.status));
@@ -2060,8 +2075,10 @@ int fk_potential(fk_mesh *m, double *phi) {
     return guard(m->ctx, "fk_potential", [&] {
         if (!m->have_phi) fk::fail(FK_ERR_STATE, "no potential");
         fk::set_device(m->ctx);
-        FK_HIP(hipMemcpyAsync(phi, m->phi.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+        fk::NodesOut<double> out(m, phi);
+        FK_HIP(hipMemcpyAsync(out.ptr, m->phi.get(), m->nv * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
         FK_HIP(hipStreamSynchronize(m->ctx->stream));
+        out.commit();
     });
 }
 
@@ -2070,7 +2087,8 @@ int fk_set_potential(fk_mesh *m, const double *phi) {
     return guard(m->ctx, "fk_set_potential", [&] {
         fk::set_device(m->ctx);
         m->phi.ensure(m->nv);
-        FK_HIP(hipMemcpyAsync(m->phi.get(), phi, m->nv * sizeof(double), hipMemcpyHostToDevice, m->ctx->stream));
+        const fk::NodesIn<double> in(m, phi);
+        FK_HIP(hipMemcpyAsync(m->phi.get(), in.ptr, m->nv * sizeof(double), hipMemcpyHostToDevice, m->ctx->stream));
         FK_HIP(hipStreamSynchronize(m->ctx->stream));
         m->have_phi = true;
     });
@@ -2098,6 +2116,12 @@ int fk_field_noalloc(fk_mesh *m, const double *phi, double *H, double *Hn, doubl
         if (!part) fk::fail(FK_ERR_STATE, "no scratch address for this device (context set-up failed)");
         fk::set_device(m->ctx);
         hipStream_t st = m->ctx->stream;
+        if (m->reordered) {
+            // the potential into the internal node order (buffers allocated with the mesh)
+            fk::k_gather_nodes<<<fk::grid_for(m->nv, 256), 256, 0, st>>>(phi, m->d_node_perm.get(), m->nv,
+                                                                          m->phi_internal.get());
+            phi = m->phi_internal.get();
+        }
         fk::k_field<<<nb, fk::kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->perm.get(), phi,
                                                energy ? m->coef.get() : nullptr, m->nt, H, Hn, part);
         if (energy) fk::k_sum_in_order<<<1, 64, 0, st>>>(part, nb, energy);

@@ -31,6 +31,9 @@ FK_PREC_F32_RESTART = 3
 FK_ASM_COLOURED = 0
 FK_ASM_ATOMIC = 1
 FK_ASM_GATHER = 2
+FK_REORDER_AUTO = 0
+FK_REORDER_NONE = 1
+FK_REORDER_RCM = 2
 
 
 class FemekoHIPError(RuntimeError):
@@ -57,6 +60,9 @@ SIGNATURES = {
     "fk_context_last_error": (_c.c_char_p, [_vp]),
     "fk_context_synchronize": (_c.c_int, [_vp]),
     "fk_mesh_create": (_c.c_int, [_vp, _dp, _c.c_int64, _ip, _c.c_int64, _c.c_int32, _dp, _vpp]),
+    "fk_mesh_create_ex": (_c.c_int, [_vp, _dp, _c.c_int64, _ip, _c.c_int64, _c.c_int32, _dp, _c.c_int32, _vpp]),
+    "fk_mesh_numbering": (_c.c_int, [_vp, _ip, _dp, _dp]),
+    "fk_rcm_order": (_c.c_int, [_ip, _c.c_int64, _c.c_int64, _c.c_int32, _ip]),
     "fk_mesh_destroy": (_c.c_int, [_vp]),
     "fk_mesh_info": (_c.c_int, [_vp, _i64p, _ip, _i64p]),
     "fk_mesh_volumes": (_c.c_int, [_vp, _dp]),

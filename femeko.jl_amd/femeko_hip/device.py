@@ -117,7 +117,12 @@ class DeviceMesh:
     CSR pattern and element -> slot map, coefficients, values, Dirichlet data,
     potential (fk_mesh)."""
 
-    def __init__(self, mesh, ctx: Context | None = None, check_volumes: bool = True, index_base: int = 1):
+    def __init__(self, mesh, ctx: Context | None = None, check_volumes: bool = True, index_base: int = 1,
+                 reorder: str = "auto"):
+        """reorder: 'auto' (default: the library renumbers the nodes internally
+        by reverse Cuthill-McKee unless the numbering is a stencil numbering or
+        already compact), 'none' or 'rcm'. Results are always in the caller's
+        numbering."""
         self.ctx = ctx or context(0)
         self._L = self.ctx._L
         self.nv = int(mesh.p.shape[1])
@@ -126,8 +131,9 @@ class DeviceMesh:
         t = np.ascontiguousarray(np.asarray(mesh.t, dtype=np.int32).T)     # (nt,4) == 4 x nt column-major
         VE = _f64(mesh.VE, self.nt, "VE") if (check_volumes and getattr(mesh, "VE", None) is not None) else None
         h = ctypes.c_void_p()
-        s = self._L.fk_mesh_create(self.ctx.handle, dptr(p), self.nv, iptr(t), self.nt, int(index_base),
-                                   dptr(VE), ctypes.byref(h))
+        code = {"auto": _lib.FK_REORDER_AUTO, "none": _lib.FK_REORDER_NONE, "rcm": _lib.FK_REORDER_RCM}[reorder]
+        s = self._L.fk_mesh_create_ex(self.ctx.handle, dptr(p), self.nv, iptr(t), self.nt, int(index_base),
+                                      dptr(VE), code, ctypes.byref(h))
         self.ctx.check(s, "fk_mesh_create")
         self.handle = h
         _live_meshes.add(self)
@@ -139,6 +145,15 @@ class DeviceMesh:
         self.nnz = int(nnz.value)
         self.ncolours = int(nc.value)
         self.unused_nodes = int(unused.value)
+
+    def numbering(self):
+        """(renumbered, mean neighbour distance in the caller's numbering, in
+        the internal one) of fk_mesh_numbering."""
+        r = ctypes.c_int32(0)
+        a, b = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        self.ctx.check(self._L.fk_mesh_numbering(self.handle, ctypes.byref(r), ctypes.byref(a), ctypes.byref(b)),
+                       "fk_mesh_numbering")
+        return bool(r.value), float(a.value), float(b.value)
 
     # -- pattern / assembly --------------------------------------------------
     def pattern(self, base: int = 1):

@@ -442,3 +442,17 @@ def save_mesh_dump(mesh: MESH, folder) -> None:
             put("AE.txt", np.asarray(mesh.AE, dtype=np.float64).reshape(-1, 1))
         if mesh.normal is not None:
             put("normal.txt", np.asarray(mesh.normal, dtype=np.float64))
+
+
+def rcm_order(mesh: MESH) -> np.ndarray:
+    """The library's reverse Cuthill-McKee order of the mesh's nodes (the
+    internal numbering fk_mesh_create picks for a non-stencil mesh):
+    perm[k] = the old number of new node k (0-based). Runs on the host."""
+    from . import _lib
+    L = _lib.load()
+    t = np.ascontiguousarray(np.asarray(mesh.t, dtype=np.int32).T)
+    perm = np.empty(mesh.nv, np.int32)
+    s = L.fk_rcm_order(_lib.iptr(t), int(mesh.nv), int(t.shape[0]), 1, _lib.iptr(perm))
+    if s:
+        raise _lib.FemekoHIPError(s, "fk_rcm_order")
+    return perm.astype(np.int64)

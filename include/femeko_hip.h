@@ -86,6 +86,32 @@ int fk_context_synchronize(fk_context *ctx);
  * returns FK_ERR_INVALID_ARG. */
 int fk_mesh_create(fk_context *ctx, const double *p, int64_t nv, const int32_t *t, int64_t nt,
                    int32_t index_base, const double *VE, fk_mesh **out);
+/* Node numbering inside the library. The solver's matrix copies and the
+ * gather kernels read node data at the columns of each row, so their speed
+ * depends on how far apart neighbouring node numbers are; a mesh generator's
+ * numbering need not keep them close. fk_mesh_create (FK_REORDER_AUTO)
+ * renumbers the nodes internally by reverse Cuthill-McKee unless the
+ * caller's numbering is a stencil numbering (a structured grid) or its mean
+ * neighbour distance is within 2x of RCM's. Every per-node array of the
+ * ABI stays in the caller's node order and every exported matrix in the
+ * caller's numbering: the renumbering is not visible in any result.
+ * FK_REORDER_NONE keeps the caller's numbering, FK_REORDER_RCM always
+ * renumbers. No counterpart in the reference (Femeko solves in Gmsh's
+ * numbering, src/mesh.jl:62-245). */
+#define FK_REORDER_AUTO 0
+#define FK_REORDER_NONE 1
+#define FK_REORDER_RCM 2
+int fk_mesh_create_ex(fk_context *ctx, const double *p, int64_t nv, const int32_t *t, int64_t nt,
+                      int32_t index_base, const double *VE, int32_t reorder, fk_mesh **out);
+/* Whether the mesh was renumbered, and the mean |i - j| over neighbouring
+ * nodes (i, j) in the caller's numbering and in the internal one (both 0
+ * when FK_REORDER_AUTO kept a stencil numbering without measuring). */
+int fk_mesh_numbering(const fk_mesh *mesh, int32_t *reordered, double *mean_distance_caller,
+                      double *mean_distance_internal);
+/* The reverse Cuthill-McKee order fk_mesh_create uses, on the host (no
+ * GPU): perm[k] = the 0-based node numbered k (nv entries). t as for
+ * fk_mesh_create. */
+int fk_rcm_order(const int32_t *t, int64_t nv, int64_t nt, int32_t index_base, int32_t *perm);
 int fk_mesh_destroy(fk_mesh *mesh);
 /* nnz of the pattern, number of element colours, nodes used by no element */
 int fk_mesh_info(const fk_mesh *mesh, int64_t *nnz, int32_t *ncolours, int64_t *unused_nodes);
@@ -188,7 +214,9 @@ int fk_field(fk_mesh *mesh, double *H, double *Hn, double *energy);
  * waits); no allocation, copy or synchronization. A mesh whose field kernel
  * needs more block partials than the scratch holds returns
  * FK_ERR_UNSUPPORTED. Calls that overlap on one GPU (two streams or
- * contexts) race on the shared scratch. */
+ * contexts) race on the shared scratch. On a mesh the library renumbered
+ * (fk_mesh_numbering) phi is first gathered into the internal node order
+ * in a buffer allocated with the mesh. */
 int fk_field_noalloc(fk_mesh *mesh, const double *phi, double *H, double *Hn, double *energy);
 
 /* ---- Neumann flux load and Lagrange saddle solve ----------------------------

@@ -47,13 +47,18 @@ class DevArray:
         self.hip.hipFree(self.ptr)
 
 
-def _one_call(hip, n, seed):
+def _one_call(hip, n, seed, scattered=False):
     pr = W.linear(n, seed=seed)
+    if scattered:
+        # a node numbering the library renumbers inside: phi arrives in the
+        # caller's order and is gathered into the internal one
+        pr = W.renumbered(pr, np.random.default_rng(seed).permutation(pr.mesh.nv), "random")
     ctx = context(0)
     L = _lib.load()
     dm = DeviceMesh(pr.mesh, ctx)
     bufs = []
     try:
+        assert dm.numbering()[0] == scattered
         dm.set_coefficient(pr.mu)
         dm.assemble()
         dm.set_dirichlet(pr.mask, pr.g)
@@ -101,3 +106,7 @@ def test_noalloc_field_rejects_missing_potential():
     finally:
         dm.close()
 
+
+def test_noalloc_field_on_a_renumbered_mesh():
+    hip = _hip()
+    _one_call(hip, 10, 8, scattered=True)
