@@ -15,13 +15,15 @@ remaining error sits in smooth modes that weigh little in the residual).
 
 Reported beside the headline: the plain FP64 GPU solve of the same system
 (speed-up), the classic restarted refinement (FP32 PCG to 1e-5 inside an FP64
-residual-correction loop), the error of the refined potential, |H| and energy
-against an FP64 solve at rtol 1e-13, the SpMV kernel's achieved bandwidth (HIP
-events around every SpMV launch of an instrumented solve), and the CPU oracle
-timed on a bounded sample of the same workload.
+residual-correction loop), the SpMV kernel's achieved bandwidth (back-to-back
+launches between two HIP events on the library stream), and, after the GPU
+part, the oracle on the host cores: its full assembly and Jacobi-PCG solve of
+the same system to the FP64 path's tolerance (the CPU baseline, plus a
+1-thread sample), continued to rtol 1e-13 as the FP64 reference that the
+errors of the GPU potentials, |H| and energy are reported against.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160] [--no-cpu]
-                [--shape 1024,1] [--delta 0.95] [--rtol 5e-11] [--rtol64 1e-10]
+                [--shape 256,1] [--delta 0.95] [--rtol 5e-11] [--rtol64 1e-10]
                 [--format auto|stencil|sell64|sell64-rowidx|csr]
                 [--workload boxes|batch|froehlich|tabulated|demag|adjoint]
 """
@@ -58,46 +60,90 @@ def stencil_spmv_bytes(K, nv, s):
     return K * s * nv + 2 * s * nv
 
 
-def cpu_baseline(pr, iters_f64, nthreads, budget_s=20.0):
-    """Oracle (C restatement of the CPU path) on the host cores: full assembly
-    of the same mesh, then PCG iterations timed for a bounded count and scaled
-    to the FP64 iteration count of the GPU solve."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_legs(pr, nthreads, rtol64):
+    """The oracle (the C restatement of the CPU path) on the host, untimed
+    for the GPU line:
+      1. the CPU baseline on `nthreads` cores: full assembly (OpenMP element
+         matrices + the serial sparse() build, FEMc.cpp:102-133) and a full
+         Jacobi-PCG solve to rtol64, the FP64 GPU path's tolerance -- both
+         measured, nothing extrapolated;
+      2. the same PCG continued to rtol 1e-13 from that potential: the FP64
+         reference the GPU potentials, |H| and energy are checked against;
+      3. a 1-thread sample: element matrices of 1/16 of the elements and 10
+         PCG iterations on one thread, scaled to the full mesh and to the
+         iteration count of 1."""
     from oracle import oracle as O
+    L = O.lib()
     m = pr.mesh
     t0 = time.perf_counter()
     Ak = O.localStiffnessMatrix(m, pr.mu, nthreads=nthreads)
-    A = O.sparse_from_local(m, Ak)
-    del Ak
-    t_asm = time.perf_counter() - t0
-    # time a bounded number of PCG iterations (threaded gather SpMV)
-    k = 5
+    t_local = time.perf_counter() - t0
     t0 = time.perf_counter()
+    A = O.sparse_from_local(m, Ak)
+    t_sparse = time.perf_counter() - t0
+    del Ak
+    mask = np.ascontiguousarray(pr.mask, dtype=np.uint8)
+    g = np.ascontiguousarray(pr.g, dtype=np.float64)
     phi = np.zeros(m.nv)
-    O.lib().or_pcg_iterations(A.n, O._i(A.colptr), O._i(A.rowval), O._d(A.nzval),
-                              O._u(np.ascontiguousarray(pr.mask)), O._d(np.ascontiguousarray(pr.g)), k,
-                              O._d(phi), nthreads)
-    t_k = time.perf_counter() - t0
-    if t_asm + t_k < budget_s:
-        k2 = int(max(k, min(200, (budget_s - t_asm) / max(t_k / k, 1e-6))))
-        t0 = time.perf_counter()
-        O.lib().or_pcg_iterations(A.n, O._i(A.colptr), O._i(A.rowval), O._d(A.nzval),
-                                  O._u(np.ascontiguousarray(pr.mask)), O._d(np.ascontiguousarray(pr.g)), k2,
-                                  O._d(phi), nthreads)
-        t_k = time.perf_counter() - t0
-        k = k2
-    t_it = t_k / k
-    t_total = t_asm + t_it * iters_f64
-    return {
-        "value": m.nv / t_total,
+    it = O._i64(0)
+    rel = O.ctypes.c_double(0.0)
+    t0 = time.perf_counter()
+    s1 = L.or_pcg_dirichlet(A.n, O._i(A.colptr), O._i(A.rowval), O._d(A.nzval), O._u(mask), O._d(g), rtol64,
+                            1000000, O._d(phi), O.ctypes.byref(it), O.ctypes.byref(rel), nthreads, 0)
+    t_solve = time.perf_counter() - t0
+    it1, rel1 = int(it.value), float(rel.value)
+    log(f"oracle on {nthreads} threads: assembly {t_local + t_sparse:.2f} s, PCG to {rtol64:g} {t_solve:.2f} s "
+        f"({it1} iterations, status {s1})")
+    s2 = L.or_pcg_dirichlet(A.n, O._i(A.colptr), O._i(A.rowval), O._d(A.nzval), O._u(mask), O._d(g), 1e-13,
+                            1000000, O._d(phi), O.ctypes.byref(it), O.ctypes.byref(rel), nthreads, 0)
+    it2, rel2 = int(it.value), float(rel.value)
+    _H, Hn_ref = O.field(m, phi, nthreads=nthreads)
+    E_ref = O.energy(m, Hn_ref, pr.mu)
+    log(f"oracle reference: PCG continued to 1e-13 ({it2} more iterations, relres {rel2:.1e}, status {s2})")
+    # one thread: element matrices of 1/16 of the elements, 10 PCG iterations
+    k = max(1, m.nt // 16)
+    t_sub = np.ascontiguousarray(np.asarray(m.t, dtype=np.int32)[:, :k].T)
+    Ak1 = np.empty((k, 16))
+    t0 = time.perf_counter()
+    L.or_local_stiffness(O._d(O._p(m)), O._i(t_sub), k, O._d(np.ascontiguousarray(m.VE[:k])),
+                         O._d(np.ascontiguousarray(pr.mu[:k])), O._d(Ak1), 1)
+    t_local1 = (time.perf_counter() - t0) * (m.nt / k)
+    phi1 = np.zeros(m.nv)
+    t0 = time.perf_counter()
+    L.or_pcg_iterations(A.n, O._i(A.colptr), O._i(A.rowval), O._d(A.nzval), O._u(mask), O._d(g), 10, O._d(phi1), 1)
+    t_it1 = (time.perf_counter() - t0) / 10
+    t_1thread = t_local1 + t_sparse + t_it1 * it1
+    cpu = {
+        "value": m.nv / (t_local + t_sparse + t_solve),
         "unit": "DOF/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": (f"oracle C restatement (OpenMP) on the same n={pr.meta['n']} mesh: full assembly "
-                   f"{t_asm:.2f} s + {k} Jacobi-PCG iterations ({t_it*1e3:.1f} ms each) scaled to the "
-                   f"{iters_f64} FP64 iterations of the GPU solve"),
-        "t_assembly_s": t_asm,
-        "t_iteration_ms": t_it * 1e3,
+        "sample": (f"oracle C restatement (OpenMP, {nthreads} threads) on the same n={pr.meta['n']} mesh: full "
+                   f"assembly {t_local + t_sparse:.2f} s (element matrices {t_local:.2f} s + serial sparse() build "
+                   f"{t_sparse:.2f} s) + full Jacobi-PCG to rtol {rtol64:g} {t_solve:.2f} s ({it1} iterations), "
+                   f"measured end to end"),
+        "t_assembly_s": t_local + t_sparse,
+        "t_solve_s": t_solve,
+        "iterations": it1,
+        "host": {"cpu_count": os.cpu_count(), "model": _cpu_model()},
+        "single_thread": {"value": m.nv / t_1thread, "unit": "DOF/s", "cores": 1,
+                          "sample": (f"element matrices of {k} elements ({t_local1 * k / m.nt:.2f} s, scaled "
+                                     f"x{m.nt / k:.0f}) + the serial sparse() build + 10 PCG iterations "
+                                     f"({t_it1 * 1e3:.0f} ms each) scaled to {it1}")},
     }
+    ref = {"phi": phi, "Hn": Hn_ref, "energy": E_ref, "iterations": it1 + it2, "relres": rel2}
+    return cpu, ref
 
 
 def _pmc_traffic(args, solver_format):
@@ -363,8 +409,8 @@ def main():
                     help="relative residual the plain FP64 solve stops at (the configured 1e-10)")
     ap.add_argument("--delta", type=float, default=0.95,
                     help="FP32 solve: residual drop that triggers an FP64 residual replacement")
-    ap.add_argument("--shape", default="1024,1",
-                    help="SELL SpMV launch shape: threads per block, slices in flight per wave")
+    ap.add_argument("--shape", default="256,1",
+                    help="FP32 SELL SpMV launch shape: threads per block, slices in flight per wave")
     ap.add_argument("--renumber", default="none", choices=["none", "rcm", "random"],
                     help="renumber the nodes (reverse Cuthill-McKee or random) to measure a general, "
                          "non-stencil numbering of the same mesh")
@@ -411,13 +457,12 @@ def main():
     log(f"device mesh: nnz={dm.nnz} colours={dm.ncolours} ({time.perf_counter()-t0:.1f} s)")
     fmt_name, stencil_offsets = dm.solver_format()
     sell = dm.sell_info() if fmt_name == "sell64" else None
-    sell_index = None
+    sell_index = sell_groups = None
     if sell is not None:
-        # the SELL copy stores int16 offsets from the row when the node
-        # bandwidth max |j - i| (+ 64) fits, int32 columns otherwise
-        t = np.asarray(m.t, dtype=np.int64)
-        bw = max(int(np.abs(t[a_] - t[b_]).max()) for a_ in range(4) for b_ in range(a_ + 1, 4))
-        sell_index = "int16" if bw + 64 <= 32767 else "int32"
+        # int16 offsets from the row or int32 columns, and the batch width, as
+        # the library chose them
+        ib, sell_groups = dm.sell_layout()
+        sell_index = {2: "short", 4: "int"}[ib]
     log(f"solver format: {fmt_name} {stencil_offsets if stencil_offsets else ''}")
     if sell:
         log(f"SELL copy: {sell[0]} slices, {sell[1]} with shared offsets, {sell[2]} stored entries")
@@ -463,18 +508,12 @@ def main():
     t_step64 = (time.perf_counter() - t0) / nrep64
     log(f"f64 step {t_step64*1e3:.1f} ms, iterations {rep64.iterations}, relres {rep64.relres:.2e}")
     phi64 = dm.potential()
-
-    # ---- FP64 reference at rtol 1e-13 for the error report
-    dm.solve("f64", rtol=1e-13)
-    phi_ref = dm.potential()
-    _H, Hn_ref, E_ref = dm.field(want_H=False, want_Hn=True, want_energy=True)
-    VE = m.VE
-    err_phi = float(np.linalg.norm(phi32 - phi_ref) / np.linalg.norm(phi_ref))
-    err_H = float(np.sqrt(np.sum(VE * (Hn32 - Hn_ref) ** 2) / np.sum(VE * Hn_ref ** 2)))
-    err_E = float(abs(E32 - E_ref) / abs(E_ref))
-    err_phi64 = float(np.linalg.norm(phi64 - phi_ref) / np.linalg.norm(phi_ref))
-    log(f"errors vs FP64 rtol 1e-13: phi {err_phi:.2e} |H| {err_H:.2e} energy {err_E:.2e}; "
-        f"the FP64 path's phi {err_phi64:.2e}")
+    _H, Hn64, E64 = dm.field(want_H=False, want_Hn=True, want_energy=True)
+    if args.no_cpu:
+        # no oracle run: the error reference is the GPU's own FP64 solve at 1e-13
+        dm.solve("f64", rtol=1e-13)
+        _H, Hn_g, E_g = dm.field(want_H=False, want_Hn=True, want_energy=True)
+        gpu_ref = {"phi": dm.potential(), "Hn": Hn_g, "energy": E_g}
 
     # ---- practical streaming ceiling: plain 1 GiB device copy
     copy_ms = ctx.copy_probe(1 << 30, 5)
@@ -539,14 +578,12 @@ def main():
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": (16 * m.nt + 8 * m.nt + 24 * m.nv + 8 * dm.nnz) / t_asm / 1e9 / HBM_PEAK_GBS},
         "fp64": {"ms_per_step": t_step64 * 1e3, "iterations": rep64.iterations, "relres": rep64.relres,
-                 "phi_rel_l2_vs_rtol1e-13": err_phi64,
                  "dof_per_s": m.nv / t_step64},
         "speedup_vs_fp64": t_step64 / t_step,
-        "error_vs_fp64_rtol1e-13": {"phi_rel_l2": err_phi, "Hn_rel_l2_VE": err_H, "energy_rel": err_E},
         "roofline": {
             "bound": "hbm",
             "kernel": {"csr": "k_cg_spmv<float,%d>" % args.lanes,
-                       "sell64": "k_cg_spmv_sell<float,%s,%d,%d>" % ((sell_index,) + shape),
+                       "sell64": "k_cg_spmv_sell<float,%s,%d,%d,%s>" % ((sell_index,) + shape + (sell_groups,)),
                        "stencil": "k_cg_spmv_stencil<float,256>"}[fmt_name],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -583,14 +620,29 @@ def main():
         out["roofline"]["traffic"] = traffic
         out["roofline"]["traffic_GBs"] = traffic / (spmv_us * 1e-6) / 1e9
         out["roofline"]["traffic_frac"] = out["roofline"]["traffic_GBs"] / HBM_PEAK_GBS
+    dm.close()
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
-        log(f"CPU baseline with {nthreads} threads ...")
-        out["cpu_baseline"] = cpu_baseline(pr, rep64.iterations, nthreads)
+        log(f"oracle legs with {nthreads} threads ...")
+        out["cpu_baseline"], ref = oracle_legs(pr, nthreads, args.rtol64)
+        basis = f"oracle FP64 Jacobi-PCG on the host to relres {ref['relres']:.1e} ({ref['iterations']} iterations)"
         log(f"CPU baseline {out['cpu_baseline']['value']:.3e} DOF/s")
     else:
         out["cpu_baseline"] = None
-    dm.close()
+        ref = gpu_ref
+        basis = "the GPU's own FP64 solve to rtol 1e-13 (no oracle run)"
+    VE = m.VE
+
+    def errors(phi, Hn, E):
+        return {"phi_rel_l2": float(np.linalg.norm(phi - ref["phi"]) / np.linalg.norm(ref["phi"])),
+                "Hn_rel_l2_VE": float(np.sqrt(np.sum(VE * (Hn - ref["Hn"]) ** 2) / np.sum(VE * ref["Hn"] ** 2))),
+                "energy_rel": float(abs(E - ref["energy"]) / abs(ref["energy"]))}
+
+    out["error_vs_oracle"] = {"basis": basis, "f32_refined": errors(phi32, Hn32, E32),
+                              "fp64": errors(phi64, Hn64, E64)}
+    e = out["error_vs_oracle"]["f32_refined"]
+    log(f"errors of the FP32-refined solve vs {basis}: phi {e['phi_rel_l2']:.2e} |H| {e['Hn_rel_l2_VE']:.2e} "
+        f"energy {e['energy_rel']:.2e}; FP64 path phi {out['error_vs_oracle']['fp64']['phi_rel_l2']:.2e}")
     print(json.dumps(out))
 
 

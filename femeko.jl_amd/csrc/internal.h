@@ -218,6 +218,7 @@ struct fk_mesh {
     bool have_sell = false, sell64_valid = false, sell32_valid = false;
     int64_t nslices = 0, sell_len = 0, sell_chunk = 0;
     int64_t nshared = 0;              // slices in the shared-offset form
+    int32_t sell_max_groups = 0;      // widest slice, in groups of 4 entries
     fk::DevBuf<int64_t> sell_ptr, sell_cptr;
     fk::DevBuf<int32_t> sell_width, sell_col, sell_map, sell_soff;
     fk::DevBuf<int16_t> sell_dcol;    // int16 column offsets from the row (when sell_short)
@@ -264,8 +265,8 @@ struct fk_mesh {
     std::vector<int32_t> caller_rowptr, caller_colidx, caller_map;
 
     int lanes = 16;                   // lanes per CSR row in SpMV
-    int sell_block = 1024;            // threads per block of the SELL SpMV
-    int sell_unroll = 1;              // slices in flight per wave in the SELL SpMV
+    int sell_block = 256;             // threads per block of the FP32 SELL SpMV
+    int sell_unroll = 1;              // slices in flight per wave in the FP32 SELL SpMV
     int nblocks = 0;                  // persistent grid of the solver kernels
     int64_t chunk = 0;                // rows per logical block
 

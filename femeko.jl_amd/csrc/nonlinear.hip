@@ -109,8 +109,8 @@ __global__ __launch_bounds__(kBlock) void k_field_mu(const int4 *__restrict__ te
                                                      const double *__restrict__ phi,
                                                      const uint8_t *__restrict__ region, double mu0, double Ms,
                                                      double a, int64_t nt, double *__restrict__ mu,
-                                                     double *__restrict__ dmu, double *__restrict__ part,
-                                                     int32_t *__restrict__ bad) {
+                                                     double *__restrict__ dmu, double *__restrict__ hn_prev,
+                                                     double *__restrict__ part, int32_t *__restrict__ bad) {
     __shared__ double sh[kBlock / 64 + 1];
     double dev = 0.0;
     for (int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; e < nt;
@@ -125,11 +125,18 @@ __global__ __launch_bounds__(kBlock) void k_field_mu(const int4 *__restrict__ te
             const double h = -(u0 * G.g[0][c] + u1 * G.g[1][c] + u2 * G.g[2][c] + u3 * G.g[3][c]);
             hh += h * h;
         }
-        const double d = a + sqrt(hh);
+        const double hn = sqrt(hh);
+        const double d = a + hn;
         const double mnew = mu0 * (1.0 + Ms / d);
         const double dnew = -mu0 * Ms / (d * d);
         if (!isfinite(mnew) || !isfinite(dnew)) atomicAdd(bad, 1);
-        dev = fmax(dev, fabs(mnew - mu[e]) / mnew);
+        if (hn_prev) {
+            // the reference's measure mu0 | |H| - |H|_prev | (NewtonRaphson.jl:200)
+            dev = fmax(dev, mu0 * fabs(hn - hn_prev[e]));
+            hn_prev[e] = hn;
+        } else {
+            dev = fmax(dev, fabs(mnew - mu[e]) / mnew);
+        }
         mu[e] = mnew;
         dmu[e] = dnew;
     }
@@ -221,6 +228,12 @@ __global__ __launch_bounds__(kBlock) void k_field_spline(const int4 *__restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
+// y += x
+__global__ void k_add_into(double *__restrict__ y, const double *__restrict__ x, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
 // y = u + alpha du (trial of the line search)
 __global__ void k_axpy_to(const double *__restrict__ u, const double *__restrict__ du, double alpha,
                           double *__restrict__ y, int64_t n) {
@@ -295,6 +308,37 @@ static void nonlinear_loop(fk_mesh *m, const uint8_t *region_host, const LawArgs
     m->values_gen += 1;
 }
 
+// The law's region (caller element order) into m->region (colour order).
+static void upload_region(fk_mesh *m, const uint8_t *region_host) {
+    hipStream_t st = m->ctx->stream;
+    DevBuf<uint8_t> tmp(m->nt);
+    m->region.ensure(m->nt);
+    FK_HIP(hipMemcpyAsync(tmp.get(), region_host, m->nt, hipMemcpyHostToDevice, st));
+    k_gather_u8<<<grid_for(m->nt, 256), 256, 0, st>>>(tmp.get(), m->perm.get(), m->nt, m->region.get());
+    FK_HIP(hipGetLastError());
+    FK_HIP(hipStreamSynchronize(st));
+}
+
+// val64 = K(mu) (+ the tangent on a Newton step) at the resident potential and
+// r = -K(mu) phi, all rows; the row-gather form, or one coloured launch per
+// colour when the rows exceed its LDS segment.
+static void newton_values(fk_mesh *m, int newton, double *r) {
+    hipStream_t st = m->ctx->stream;
+    if (!assemble_newton_gather(m, m->coef.get(), m->dmu.get(), m->phi.get(), newton, r)) {
+        k_fill_f64b<<<grid_for(m->nnz, 256), 256, 0, st>>>(m->val64.get(), m->nnz, 0.0);
+        k_fill_f64b<<<grid_for(m->nv, 256), 256, 0, st>>>(r, m->nv, 0.0);
+        for (int c = 0; c < m->ncolours; ++c) {
+            const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
+            if (e1 <= e0) continue;
+            k_assemble_newton<<<grid_for(e1 - e0, 256), 256, 0, st>>>(
+                m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->coef.get(), m->dmu.get(), m->phi.get(),
+                m->slot.get(), m->val64.get(), r, newton, e0, e1);
+        }
+    }
+    FK_HIP(hipGetLastError());
+    values_changed(m);
+}
+
 static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const LawArgs &law, double rtol,
                                 double tol, int max_outer, double switch_tol, int32_t *outer_out, int64_t *cg_out,
                                 double *dev_out) {
@@ -303,15 +347,7 @@ static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const La
     hipStream_t st = m->ctx->stream;
     const int64_t nv = m->nv, nt = m->nt;
     const double mu0 = 3.141592653589793 * 4e-7;
-    // region in colour order
-    {
-        DevBuf<uint8_t> tmp(nt);
-        m->region.ensure(nt);
-        FK_HIP(hipMemcpyAsync(tmp.get(), region_host, nt, hipMemcpyHostToDevice, st));
-        k_gather_u8<<<grid_for(nt, 256), 256, 0, st>>>(tmp.get(), m->perm.get(), nt, m->region.get());
-        FK_HIP(hipGetLastError());
-        FK_HIP(hipStreamSynchronize(st));
-    }
+    upload_region(m, region_host);
     m->coef.ensure(nt);
     m->dmu.ensure(nt);
     m->val64.ensure(m->nnz);
@@ -337,19 +373,7 @@ static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const La
     PcgWork<double> &w = m->w64;
     for (it = 1; it <= max_outer; ++it) {
         const int newton = dev < switch_tol ? 1 : 0;
-        if (!assemble_newton_gather(m, m->coef.get(), m->dmu.get(), m->phi.get(), newton, w.r.get())) {
-            k_fill_f64b<<<grid_for(m->nnz, 256), 256, 0, st>>>(m->val64.get(), m->nnz, 0.0);
-            k_fill_f64b<<<grid_for(nv, 256), 256, 0, st>>>(w.r.get(), nv, 0.0);
-            for (int c = 0; c < m->ncolours; ++c) {
-                const int64_t e0 = m->colour_off[c], e1 = m->colour_off[c + 1];
-                if (e1 <= e0) continue;
-                k_assemble_newton<<<grid_for(e1 - e0, 256), 256, 0, st>>>(
-                    m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->coef.get(), m->dmu.get(), m->phi.get(),
-                    m->slot.get(), m->val64.get(), w.r.get(), newton, e0, e1);
-            }
-        }
-        FK_HIP(hipGetLastError());
-        values_changed(m);
+        newton_values(m, newton, w.r.get());
         compute_dinv_f64(m);
         m->dinv64_valid = true;
         const CgState s = run_pcg_f64(m, rtol, 1000000, -1.0);
@@ -366,7 +390,7 @@ static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const La
         else
             k_field_mu<<<nb, kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->phi.get(),
                                               m->region.get(), mu0, law.Ms, law.a, nt, m->coef.get(),
-                                              m->dmu.get(), m->part_misc.get(), bad.get());
+                                              m->dmu.get(), nullptr, m->part_misc.get(), bad.get());
         FK_HIP(hipGetLastError());
         int32_t hbad[2] = {0, 0};
         FK_HIP(hipMemcpyAsync(m->hpart.ptr, m->part_misc.get(), nb * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -419,6 +443,40 @@ void nonlinear_spline(fk_mesh *m, const uint8_t *region_host, const double *t_mu
     nonlinear_loop(m, region_host, law, rtol, tol_H, max_outer, switch_tol, outer_out, cg_out, dev_out);
 }
 
+// lineSearch (src/FEM3D.jl:123-152) on device vectors with A the resident
+// values (all rows): alpha <- 0.9 alpha while ||rhs - A(u + alpha du)|| >
+// ||rhs - A u|| and alpha > min_step; u becomes u + alpha du unless alpha fell
+// below min_step. y: nv scratch. Returns the last trial's residue; *alpha_out
+// the last trial step and *moved whether u changed.
+static double line_search_dev(fk_mesh *m, double *u, const double *du, const double *rhs, double min_step,
+                              double *y, double *alpha_out, bool *moved) {
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = m->nv;
+    const int nb = std::max(1, m->nblocks);
+    m->part_misc.ensure(nb);
+    auto norm_of = [&](const double *x) {
+        k_resid_norm<<<nb, kBlock, 0, st>>>(m->rowptr.get(), m->colidx.get(), m->val64.get(), x, rhs,
+                                            m->part_misc.get(), nv);
+        FK_HIP(hipGetLastError());
+        FK_HIP(hipMemcpyAsync(m->hpart.ptr, m->part_misc.get(), nb * sizeof(double), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        return std::sqrt(host_sum(m->hpart.ptr, nb));
+    };
+    double alf = 1.0;
+    const double residual_old = norm_of(u);
+    k_axpy_to<<<grid_for(nv, 256), 256, 0, st>>>(u, du, 1.0, y, nv);
+    double residue = norm_of(y);
+    while (residue > residual_old && alf > min_step) {
+        alf *= 0.9;
+        k_axpy_to<<<grid_for(nv, 256), 256, 0, st>>>(u, du, alf, y, nv);
+        residue = norm_of(y);
+    }
+    *moved = !(alf < min_step);
+    if (*moved) FK_HIP(hipMemcpyAsync(u, y, nv * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (alpha_out) *alpha_out = alf;
+    return residue;
+}
+
 double line_search(fk_mesh *m, double *u_host, const double *du_host, const double *rhs_host, double min_step) {
     if (!m->have_values) fail(FK_ERR_STATE, "matrix not assembled");
     hipStream_t st = m->ctx->stream;
@@ -431,33 +489,134 @@ double line_search(fk_mesh *m, double *u_host, const double *du_host, const doub
         FK_HIP(hipMemcpyAsync(rhs.get(), ri.ptr, nv * sizeof(double), hipMemcpyHostToDevice, st));
         FK_HIP(hipStreamSynchronize(st));
     }
-    const int nb = std::max(1, m->nblocks);
-    m->part_misc.ensure(nb);
-    auto norm_of = [&](const double *x) {
-        k_resid_norm<<<nb, kBlock, 0, st>>>(m->rowptr.get(), m->colidx.get(), m->val64.get(), x, rhs.get(),
-                                            m->part_misc.get(), nv);
-        FK_HIP(hipGetLastError());
-        FK_HIP(hipMemcpyAsync(m->hpart.ptr, m->part_misc.get(), nb * sizeof(double), hipMemcpyDeviceToHost, st));
-        FK_HIP(hipStreamSynchronize(st));
-        return std::sqrt(host_sum(m->hpart.ptr, nb));
-    };
-    // src/FEM3D.jl:123-152
-    double alf = 1.0;
-    const double residual_old = norm_of(u.get());
-    k_axpy_to<<<grid_for(nv, 256), 256, 0, st>>>(u.get(), du.get(), 1.0, y.get(), nv);
-    double residue = norm_of(y.get());
-    while (residue > residual_old && alf > min_step) {
-        alf *= 0.9;
-        k_axpy_to<<<grid_for(nv, 256), 256, 0, st>>>(u.get(), du.get(), alf, y.get(), nv);
-        residue = norm_of(y.get());
-    }
-    if (!(alf < min_step)) {
+    bool moved = false;
+    const double residue = line_search_dev(m, u.get(), du.get(), rhs.get(), min_step, y.get(), nullptr, &moved);
+    if (moved) {
         NodesOut<double> uo(m, u_host);
-        FK_HIP(hipMemcpyAsync(uo.ptr, y.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipMemcpyAsync(uo.ptr, u.get(), nv * sizeof(double), hipMemcpyDeviceToHost, st));
         FK_HIP(hipStreamSynchronize(st));
         uo.commit();
     }
     return residue;
+}
+
+// ---- the reference's nonlinear driver ---------------------------------------
+// Magnetostatics/NewtonRaphson.jl:148-212: Neumann boundary load and the
+// Lagrange multiplier of the constant, every step damped by lineSearch.
+//   att = 1: A(mu0), no tangent; att > 1: A(mu) + At(H, dmu)
+//   du from [A + At, Lag; Lag', 0] [du; l] = [RHS - A u; 0]   (:171)
+//   u updated by lineSearch(u, du, A, RHS) with A = A(mu)      (:175)
+//   H of u, div = mu0 max |H - Hold| over the region           (:200)
+//   mu, dmu from the law at |H| on the region                  (:204-207)
+// while div > tol_H and att < max_outer. The bordered solve is the projected
+// Jacobi-PCG of solve_neumann_dev (A + At is singular with null space the
+// constants, like A: the tangent's rows sum to zero too); everything stays on
+// the device, the host reads the scalar decisions.
+static void newton_saddle_body(fk_mesh *m, const uint8_t *region_host, const double *rhs_host, const LawArgs &law,
+                               double rtol, double tol_H, int max_outer, double min_step, int32_t *outer_out,
+                               int64_t *cg_out, double *div_out, double *residue_out) {
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = m->nv, nt = m->nt;
+    const double mu0 = 3.141592653589793 * 4e-7;
+    upload_region(m, region_host);
+    m->coef.ensure(nt);
+    m->dmu.ensure(nt);
+    m->hn_prev.ensure(nt);
+    m->val64.ensure(m->nnz);
+    m->phi.ensure(nv);
+    ensure_work_f64(m);
+    k_fill_f64b<<<grid_for(nt, 256), 256, 0, st>>>(m->coef.get(), nt, mu0);
+    k_fill_f64b<<<grid_for(nt, 256), 256, 0, st>>>(m->dmu.get(), nt, 0.0);
+    k_fill_f64b<<<grid_for(nt, 256), 256, 0, st>>>(m->hn_prev.get(), nt, 0.0);
+    k_fill_f64b<<<grid_for(nv, 256), 256, 0, st>>>(m->phi.get(), nv, 0.0);
+    FK_HIP(hipGetLastError());
+    m->have_coef = true;
+    m->coef_nat_valid = false;
+    m->have_phi = true;
+    DevBuf<double> rhs(nv), y(nv), junk(nv);
+    {
+        const NodesIn<double> ri(m, rhs_host);
+        FK_HIP(hipMemcpyAsync(rhs.get(), ri.ptr, nv * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipStreamSynchronize(st));
+    }
+    const int nb = std::max(1, m->nblocks);
+    m->part_misc.ensure(nb);
+    DevBuf<int32_t> bad(2);  // non-finite, outside the table
+    PcgWork<double> &w = m->w64;
+    double div = INFINITY, residue = 0.0;
+    int32_t att = 0;
+    int64_t cg = 0;
+    while (div > tol_H && att < max_outer) {
+        ++att;
+        const int newton = att > 1 ? 1 : 0;
+        newton_values(m, newton, w.r.get());  // A (+ At), r = -A u
+        k_add_into<<<grid_for(nv, 256), 256, 0, st>>>(w.r.get(), rhs.get(), nv);
+        FK_HIP(hipGetLastError());
+        const NeumannResult nr = solve_neumann_dev(m, rtol, 1000000, nullptr);  // du in w.x
+        cg += nr.iters;
+        if (newton) newton_values(m, 0, junk.get());  // lineSearch takes A(mu) alone
+        bool moved = false;
+        residue = line_search_dev(m, m->phi.get(), w.x.get(), rhs.get(), min_step, y.get(), nullptr, &moved);
+        FK_HIP(hipMemsetAsync(bad.get(), 0, 2 * sizeof(int32_t), st));
+        if (law.spline)
+            k_field_spline<<<nb, kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->phi.get(),
+                                                  m->region.get(), mu0, *law.spline, nt, m->coef.get(),
+                                                  m->dmu.get(), m->hn_prev.get(), m->part_misc.get(), bad.get(),
+                                                  bad.get() + 1);
+        else
+            k_field_mu<<<nb, kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->phi.get(),
+                                              m->region.get(), mu0, law.Ms, law.a, nt, m->coef.get(),
+                                              m->dmu.get(), m->hn_prev.get(), m->part_misc.get(), bad.get());
+        FK_HIP(hipGetLastError());
+        int32_t hbad[2] = {0, 0};
+        FK_HIP(hipMemcpyAsync(m->hpart.ptr, m->part_misc.get(), nb * sizeof(double), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipMemcpyAsync(hbad, bad.get(), 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FK_HIP(hipStreamSynchronize(st));
+        if (hbad[1]) fail(FK_ERR_DOMAIN, std::to_string(hbad[1]) + " element(s) with |H| outside the table");
+        if (hbad[0]) fail(FK_ERR_NONFINITE, "Nans/Infs in mu");  // NewtonRaphson.jl:210-212
+        div = host_max(m->hpart.ptr, nb);
+    }
+    if (outer_out) *outer_out = att;
+    if (cg_out) *cg_out = cg;
+    if (div_out) *div_out = div;
+    if (residue_out) *residue_out = residue;
+    if (div > tol_H) fail(FK_ERR_NOT_CONVERGED, "mu0 max |H - Hold| did not reach its tolerance");
+}
+
+void newton_saddle(fk_mesh *m, const uint8_t *region_host, const double *rhs_host, double Ms, double a,
+                   const double *t_mu, int n_mu, const double *c_mu, const double *t_dmu, int n_dmu,
+                   const double *c_dmu, double rtol, double tol_H, int max_outer, double min_step,
+                   int32_t *outer_out, int64_t *cg_out, double *div_out, double *residue_out) {
+    hipStream_t st = m->ctx->stream;
+    LawArgs law;
+    law.Ms = Ms;
+    law.a = a;
+    DevBuf<double> tm, cm, td, cd;
+    SplineLaw sl{};
+    if (t_mu) {
+        check_spline(t_mu, n_mu, c_mu, "mu spline");
+        check_spline(t_dmu, n_dmu, c_dmu, "dmu spline");
+        tm.alloc(n_mu); cm.alloc(n_mu - 4); td.alloc(n_dmu); cd.alloc(n_dmu - 4);
+        FK_HIP(hipMemcpyAsync(tm.get(), t_mu, n_mu * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(cm.get(), c_mu, (n_mu - 4) * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(td.get(), t_dmu, n_dmu * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(cd.get(), c_dmu, (n_dmu - 4) * sizeof(double), hipMemcpyHostToDevice, st));
+        sl = SplineLaw{tm.get(), cm.get(), td.get(), cd.get(), n_mu, n_dmu};
+        law.spline = &sl;
+    } else if (!(a > 0.0) || !std::isfinite(Ms)) {
+        fail(FK_ERR_INVALID_ARG, "Froehlich law needs a > 0 and a finite Ms");
+    }
+    try {
+        newton_saddle_body(m, region_host, rhs_host, law, rtol, tol_H, max_outer, min_step, outer_out, cg_out,
+                           div_out, residue_out);
+    } catch (...) {
+        m->have_values = false;
+        m->values_gen += 1;
+        throw;
+    }
+    // the values hold A(mu) of the last step, not of the final mu
+    m->have_values = false;
+    m->values_gen += 1;
 }
 
 void get_coefficient(fk_mesh *m, double *f_host) {
@@ -516,6 +675,20 @@ int fk_get_coefficient(fk_mesh *m, double *f) {
     return guard(m->ctx, "fk_get_coefficient", [&] {
         fk::set_device(m->ctx);
         fk::get_coefficient(m, f);
+    });
+}
+
+int fk_newton_saddle(fk_mesh *m, const uint8_t *region, const double *rhs, double Ms, double a, const double *t_mu,
+                     int32_t n_mu, const double *c_mu, const double *t_dmu, int32_t n_dmu, const double *c_dmu,
+                     double rtol, double tol_H, int32_t max_outer, double min_step, int32_t *outer,
+                     int64_t *cg_iterations, double *final_change, double *residue) {
+    if (!m || !region || !rhs || !(rtol > 0.0) || !(tol_H >= 0.0) || max_outer < 1 || !(min_step >= 0.0) ||
+        (t_mu && (!c_mu || !t_dmu || !c_dmu)))
+        return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_newton_saddle", [&] {
+        fk::set_device(m->ctx);
+        fk::newton_saddle(m, region, rhs, Ms, a, t_mu, n_mu, c_mu, t_dmu, n_dmu, c_dmu, rtol, tol_H, max_outer,
+                          min_step, outer, cg_iterations, final_change, residue);
     });
 }
 

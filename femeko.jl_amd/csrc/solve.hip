@@ -396,20 +396,102 @@ __device__ __forceinline__ T sell_row(const Sell<T, IDX> &A, const T *__restrict
     return (a0 + a1) + (a2 + a3);
 }
 
-// (A x)_row: FP32 in the difference form; FP64 in the standard form with the
-// diagonal term added last
-template <typename T, typename IDX>
+// off-diagonal part of row `row` (lane `lane` of slice s), all loads of the
+// row issued before any is used: the per-row indices of the NG groups and
+// the tail, then the values and the gathers of x, then the sums. The loads
+// are unconditional: a group or tail entry past the slice's width reads a
+// valid stand-in (the first values of the array, column 0 of the row's
+// range) and its term is dropped, so no branch splits the batch and a slice
+// costs two dependent memory round trips (indices, then values and x)
+// instead of two per group of 4 entries. The sums run in the order of the
+// per-group loop (sell_row).
+template <int NG, bool DIFF, typename T, typename IDX>
+__device__ __forceinline__ T sell_row_batched(const Sell<T, IDX> &A, const T *__restrict__ x, int64_t s, int lane,
+                                              int64_t row, T xi) {
+    const int64_t base = A.ptr[s];
+    const int Wr = A.width[s];
+    const int64_t cb = A.cptr[s];
+    const int W = Wr < 0 ? -Wr : Wr;
+    const int ng = W >> 2;
+    const int tail = W - 4 * ng;
+    const int64_t n = A.n;
+    const Quad<T> *v4 = reinterpret_cast<const Quad<T> *>(A.val + base) + lane;
+    const Quad<T> *v4_any = reinterpret_cast<const Quad<T> *>(A.val);
+    const int64_t tb = base + 256 * static_cast<int64_t>(ng) + lane;
+    Quad<T> V[NG];
+    T xv[NG][4];
+    T vt[3], xt[3];
+    if (Wr < 0) {
+        const int32_t *o = A.soff + cb;  // uniform over the wave
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const bool ok = g < ng;
+            V[g] = ok ? v4[64 * g] : v4_any[0];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xv[g][j] = x[clamp_col(row + (ok ? o[4 * g + j] : 0), n)];
+        }
+        const int32_t *ot = o + 4 * ng;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const bool ok = t < tail;
+            vt[t] = A.val[ok ? tb + 64 * t : 0];
+            xt[t] = x[clamp_col(row + (ok ? ot[t] : 0), n)];
+        }
+    } else {
+        const Quad<IDX> *c4 = reinterpret_cast<const Quad<IDX> *>(A.col + cb) + lane;
+        const Quad<IDX> *c4_any = reinterpret_cast<const Quad<IDX> *>(A.col);
+        const int64_t ct = cb + 256 * static_cast<int64_t>(ng) + lane;
+        Quad<IDX> C[NG];
+        IDX ci[3];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) C[g] = g < ng ? c4[64 * g] : c4_any[0];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) ci[t] = A.col[t < tail ? ct + 64 * t : 0];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            V[g] = g < ng ? v4[64 * g] : v4_any[0];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xv[g][j] = x[clamp_col(col_of(C[g].v[j], row), n)];
+        }
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            vt[t] = A.val[t < tail ? tb + 64 * t : 0];
+            xt[t] = x[clamp_col(col_of(ci[t], row), n)];
+        }
+    }
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+        if (g < ng) {
+            a0 += term<DIFF>(V[g].v[0], xv[g][0], xi);
+            a1 += term<DIFF>(V[g].v[1], xv[g][1], xi);
+            a2 += term<DIFF>(V[g].v[2], xv[g][2], xi);
+            a3 += term<DIFF>(V[g].v[3], xv[g][3], xi);
+        }
+    if (tail > 0) a0 += term<DIFF>(vt[0], xt[0], xi);
+    if (tail > 1) a1 += term<DIFF>(vt[1], xt[1], xi);
+    if (tail > 2) a2 += term<DIFF>(vt[2], xt[2], xi);
+    return (a0 + a1) + (a2 + a3);
+}
+
+// (A x)_row: the PCG products in the difference form (both precisions: no
+// diagonal is read); the FP64 true residual (RESID) in the standard form
+// with the assembled diagonal added last. NG > 0: rows of at most 4 NG + 3
+// entries with all loads issued first (sell_row_batched); NG = 0: any width,
+// group by group (sell_row).
+template <int NG, bool RESID = false, typename T, typename IDX>
 __device__ __forceinline__ T sell_apply(const Sell<T, IDX> &A, const T *__restrict__ x, int64_t s, int lane,
                                         int64_t row, T xi) {
-    constexpr bool DIFF = sizeof(T) == 4;
-    const T off = sell_row<DIFF>(A, x, s, lane, row, xi);
+    constexpr bool DIFF = !RESID;
+    const T off = NG > 0 ? sell_row_batched<(NG > 0 ? NG : 1), DIFF>(A, x, s, lane, row, xi)
+                         : sell_row<DIFF>(A, x, s, lane, row, xi);
     if (DIFF) return off;
     return row < A.n ? off + A.diag[row] * xi : off;
 }
 
 // K1 in SELL form: q = A p, partial p.q; each wave owns whole slices, U
 // slices in flight per wave (independent loads interleaved)
-template <typename T, typename IDX, int BLOCK, int U>
+template <typename T, typename IDX, int BLOCK, int U, int NG>
 __global__ __launch_bounds__(BLOCK) void k_cg_spmv_sell(const Sell<T, IDX> A, const T *__restrict__ p,
                                                         T *__restrict__ q, double *__restrict__ part,
                                                         int64_t nslices, int64_t chunk_slices, int nblocks,
@@ -432,7 +514,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_spmv_sell(const Sell<T, IDX> A, co
         for (int u = 0; u < U; ++u) {
             const int64_t row = (s + u * NW) * 64 + lane;
             xi[u] = row < n ? p[row] : T(0);
-            y[u] = sell_apply(A, p, s + u * NW, lane, row, xi[u]);
+            y[u] = sell_apply<NG>(A, p, s + u * NW, lane, row, xi[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -446,7 +528,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_spmv_sell(const Sell<T, IDX> A, co
     for (; s < s1; s += NW) {
         const int64_t row = s * 64 + lane;
         const T xi = row < n ? p[row] : T(0);
-        const T y = sell_apply(A, p, s, lane, row, xi);
+        const T y = sell_apply<NG>(A, p, s, lane, row, xi);
         if (row < n) {
             q[row] = y;
             acc += static_cast<double>(xi) * static_cast<double>(y);
@@ -458,7 +540,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_spmv_sell(const Sell<T, IDX> A, co
 
 // FP64 r = -mask(A phi), SELL form
 // (gate: when given, the kernel runs only in a replacement step, act == 1)
-template <typename IDX>
+template <typename IDX, int NG>
 __global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const Sell<double, IDX> A,
                                                               const double *__restrict__ phi,
                                                               const uint8_t *__restrict__ mask,
@@ -475,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const Sell<double,
     for (int64_t s = s0 + wv; s < s1; s += kBlock / 64) {
         const int64_t row = s * 64 + lane;
         const double xi = row < A.n ? phi[row] : 0.0;
-        const double y = sell_apply(A, phi, s, lane, row, xi);
+        const double y = sell_apply<NG, true>(A, phi, s, lane, row, xi);
         if (row < A.n) {
             const double v = mask[row] ? 0.0 : -y;
             r[row] = v;
@@ -507,8 +589,11 @@ struct Stencil {
 // value 0, a clamped backward term is dropped), then the terms are summed.
 // FP64: loads guarded by the row bounds over the runtime count, few
 // registers, so 512-thread blocks run 8 waves per SIMD and the occupancy
-// hides the latency (measured at n=160: the solver SpMV 93 us against 105 us
-// with the FP32 form's unconditional loads, which cost it occupancy).
+// hides the latency. Measured at n=160 (PCG product, difference form, 72 B
+// per row): 81 us, against 86 us with the FP32 form's unconditional loads
+// (occupancy), 114 us with a compile-time offset loop, 82 us on the FP32
+// kernel's short-lived 256-thread grid, and 88 us for the standard form,
+// which also streams the diagonal.
 template <bool DIFF, int K, typename T>
 __device__ __forceinline__ T stencil_row(const Stencil<T> &A, const T *__restrict__ x, int64_t i, T xi) {
     if (sizeof(T) == 8) {
@@ -544,9 +629,11 @@ __device__ __forceinline__ T stencil_row(const Stencil<T> &A, const T *__restric
     return a0 + a1;
 }
 
-template <int K, typename T>
+// the PCG product in the difference form (both precisions); the FP64 true
+// residual (RESID) in the standard form with the assembled diagonal
+template <int K, bool RESID = false, typename T>
 __device__ __forceinline__ T stencil_apply(const Stencil<T> &A, const T *__restrict__ x, int64_t i, T xi) {
-    constexpr bool DIFF = sizeof(T) == 4;
+    constexpr bool DIFF = !RESID;
     const T off = stencil_row<DIFF, K>(A, x, i, xi);
     return DIFF ? off : off + A.diag[i] * xi;
 }
@@ -615,7 +702,7 @@ __global__ __launch_bounds__(BLOCK) void k_neg_residual_stencil(const Stencil<do
     for (int64_t row0 = R.start; row0 < R.end; row0 += R.step) {
         const int64_t i = row0 + threadIdx.x;
         if (i >= R.end) break;
-        const double y = stencil_apply<K>(A, phi, i, phi[i]);
+        const double y = stencil_apply<K, true>(A, phi, i, phi[i]);
         const double v = mask[i] ? 0.0 : -y;
         r[i] = v;
         acc += v * v;
@@ -963,8 +1050,12 @@ static void build_sell(fk_mesh *m) {
         }
         width[s] = w;
     });
-    int32_t bw = 0;
-    for (int64_t s = 0; s < ns; ++s) bw = std::max(bw, band[s]);
+    int32_t bw = 0, maxg = 0;
+    for (int64_t s = 0; s < ns; ++s) {
+        bw = std::max(bw, band[s]);
+        maxg = std::max(maxg, (width[s] < 0 ? -width[s] : width[s]) >> 2);
+    }
+    m->sell_max_groups = maxg;
     m->bandwidth = bw;
     m->sell_short = m->allow_short && bw + 64 <= 32767;
     // pass 2: offsets of values and indices
@@ -1266,17 +1357,44 @@ static Sell<T, IDX> sell_view(fk_mesh *m, const IDX *col) {
                         sell_values<T>(m), sell_diag<T>(m), m->nv};
 }
 
+// Groups of 4 entries the batched SELL row loads at once (FP32 product):
+// 4 when the widest slice has at most 4 groups (19 entries), else 0 (group
+// by group). Measured at n=160 in RCM numbering (58.5M stored entries, at
+// most 4 groups), FP32 / FP64 us per SpMV: 256 threads 80.1 / 145.7 batched,
+// 84.2 / 148.1 group by group; 1024 threads 100.0 / 145.2 batched, 87.2 /
+// 142.7 group by group; 8-group instances 103-126 / 206-598 (their padding
+// loads). The FP32 product therefore runs batched on 256-thread blocks, the
+// FP64 product and residual group by group on 1024 (the residual on the
+// solver grid).
+static int sell_batch_groups(const fk_mesh *m) { return m->sell_max_groups <= 4 ? 4 : 0; }
+
 template <typename T, typename IDX, int BLOCK, int U>
 static void launch_sell(fk_mesh *m, PcgWork<T> &w, const IDX *col, int parity) {
     const int nb = spmv_blocks<T>(m);
     const int64_t chunk = nb == m->nblocks ? m->sell_chunk : (m->nslices + nb - 1) / nb;
-    k_cg_spmv_sell<T, IDX, BLOCK, U><<<nb, BLOCK, 0, m->ctx->stream>>>(
-        sell_view<T, IDX>(m, col), w.p.get(), w.q.get(), w.part_pq.get(), m->nslices, chunk, nb, w.state.get(),
-        parity);
+    const Sell<T, IDX> A = sell_view<T, IDX>(m, col);
+    hipStream_t st = m->ctx->stream;
+    if (sizeof(T) == 4 && sell_batch_groups(m) == 4)
+        k_cg_spmv_sell<T, IDX, BLOCK, U, 4><<<nb, BLOCK, 0, st>>>(A, w.p.get(), w.q.get(), w.part_pq.get(),
+                                                                 m->nslices, chunk, nb, w.state.get(), parity);
+    else
+        k_cg_spmv_sell<T, IDX, BLOCK, U, 0><<<nb, BLOCK, 0, st>>>(A, w.p.get(), w.q.get(), w.part_pq.get(),
+                                                                 m->nslices, chunk, nb, w.state.get(), parity);
+}
+
+template <typename IDX>
+static void launch_neg_residual_sell(fk_mesh *m, const IDX *col, const double *phi, double *r, const RuState *gate) {
+    const Sell<double, IDX> A = sell_view<double, IDX>(m, col);
+    k_neg_residual_sell<IDX, 0><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(
+        A, phi, m->mask.get(), r, m->part_misc.get(), m->nslices, m->sell_chunk, m->nblocks, gate);
 }
 
 template <typename T, typename IDX>
 static void launch_sell_shape(fk_mesh *m, PcgWork<T> &w, const IDX *col, int parity) {
+    if (sizeof(T) == 8) {  // FP64: 1024-thread blocks on the solver grid
+        launch_sell<T, IDX, 1024, 1>(m, w, col, parity);
+        return;
+    }
     const int shape = m->sell_block * 4 + m->sell_unroll;
     switch (shape) {
         case 256 * 4 + 2: launch_sell<T, IDX, 256, 2>(m, w, col, parity); break;
@@ -1779,13 +1897,9 @@ static void enqueue_neg_residual(fk_mesh *m, const double *phi, double *r, const
         });
     } else if (m->format != FK_SPMV_CSR) {
         if (m->sell_short)
-            k_neg_residual_sell<int16_t><<<m->nblocks, kBlock, 0, st>>>(
-                sell_view<double, int16_t>(m, m->sell_dcol.get()), phi, m->mask.get(), r, m->part_misc.get(),
-                m->nslices, m->sell_chunk, m->nblocks, gate);
+            launch_neg_residual_sell<int16_t>(m, m->sell_dcol.get(), phi, r, gate);
         else
-            k_neg_residual_sell<int32_t><<<m->nblocks, kBlock, 0, st>>>(
-                sell_view<double, int32_t>(m, m->sell_col.get()), phi, m->mask.get(), r, m->part_misc.get(),
-                m->nslices, m->sell_chunk, m->nblocks, gate);
+            launch_neg_residual_sell<int32_t>(m, m->sell_col.get(), phi, r, gate);
     } else switch (m->lanes) {
         case 4: launch_neg_residual<4>(m, phi, r, m->part_misc.get(), gate); break;
         case 8: launch_neg_residual<8>(m, phi, r, m->part_misc.get(), gate); break;
@@ -2214,6 +2328,16 @@ int fk_sell_info(fk_mesh *m, int64_t *nslices, int64_t *shared_slices, int64_t *
         if (nslices) *nslices = m->nslices;
         if (shared_slices) *shared_slices = m->nshared;
         if (entries) *entries = m->sell_len;
+    });
+}
+
+int fk_sell_layout(fk_mesh *m, int32_t *index_bytes, int32_t *batch_groups) {
+    if (!m) return FK_ERR_INVALID_ARG;
+    return guard(m->ctx, "fk_sell_layout", [&] {
+        if (fk::solver_kind(m) != FK_SPMV_SELL64) fk::fail(FK_ERR_STATE, "the solver format is not SELL-64");
+        fk::build_sell(m);
+        if (index_bytes) *index_bytes = m->sell_short ? 2 : 4;
+        if (batch_groups) *batch_groups = fk::sell_batch_groups(m);
     });
 }
 

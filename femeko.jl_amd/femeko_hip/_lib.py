@@ -94,6 +94,8 @@ SIGNATURES = {
     "fk_nonlinear_spline": (_c.c_int, [_vp, _up, _dp, _c.c_int32, _dp, _dp, _c.c_int32, _dp, _c.c_double,
                                        _c.c_double, _c.c_int32, _c.c_double, _ip, _i64p, _dp]),
     "fk_demag_field": (_c.c_int, [_vp, _up, _dp, _dp, _c.c_double, _c.c_int64, _c.c_int32, _dp, _i64p, _dp]),
+    "fk_newton_saddle": (_c.c_int, [_vp, _up, _dp, _c.c_double, _c.c_double, _dp, _c.c_int32, _dp, _dp, _c.c_int32,
+                                    _dp, _c.c_double, _c.c_double, _c.c_int32, _c.c_double, _ip, _i64p, _dp, _dp]),
     "fk_line_search": (_c.c_int, [_vp, _dp, _dp, _dp, _c.c_double, _dp]),
     "fk_get_coefficient": (_c.c_int, [_vp, _dp]),
     "fk_adjoint_field_target": (_c.c_int, [_vp, _up, _dp, _c.c_double, _dp, _dp, _i64p]),
@@ -117,6 +119,7 @@ SIGNATURES = {
     "fk_spmv_burst": (_c.c_int, [_vp, _c.c_int32, _c.c_int32, _c.POINTER(_c.c_double)]),
     "fk_set_lanes_per_row": (_c.c_int, [_vp, _c.c_int32]),
     "fk_set_spmv_format": (_c.c_int, [_vp, _c.c_int32]),
+    "fk_sell_layout": (_c.c_int, [_vp, _ip, _ip]),
     "fk_set_sell_shape": (_c.c_int, [_vp, _c.c_int32, _c.c_int32]),
     "fk_sell_info": (_c.c_int, [_vp, _c.c_void_p, _c.c_void_p, _c.c_void_p]),
     "fk_solver_format": (_c.c_int, [_vp, _c.c_void_p, _c.c_void_p, _c.c_void_p]),

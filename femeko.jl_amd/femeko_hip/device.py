@@ -329,6 +329,39 @@ class DeviceMesh:
             self.ctx.check(s, "fk_nonlinear_spline")
         return {"outer": int(outer.value), "cg_iterations": int(cg.value), "change": float(ch.value), "status": s}
 
+    def newton_saddle(self, region, rhs, law, rtol: float = 1e-10, tol_H: float = 1e-4, max_outer: int = 100,
+                      min_step: float = 1e-4, raise_on_error: bool = True):
+        """The reference's Neumann + Lagrange Newton loop with lineSearch
+        (Magnetostatics/NewtonRaphson.jl:109-212) on the GPU. rhs: the
+        driver's RHS = -BoundaryIntegral(mesh, Bext, shell_id); law:
+        ("froehlich", Ms, a) or ("spline", spl_mu, spl_dmu) with FITPACK-form
+        splines. Returns dict(outer, cg_iterations, change, residue, status);
+        the potential and mu stay resident."""
+        reg = np.ascontiguousarray(region, dtype=np.uint8)
+        if reg.size != self.nt:
+            raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, "region size")
+        b = _f64(rhs, self.nv, "rhs")
+        outer = ctypes.c_int32(0)
+        cg = ctypes.c_int64(0)
+        ch = ctypes.c_double(0.0)
+        res = ctypes.c_double(0.0)
+        if law[0] == "froehlich":
+            Ms, a = float(law[1]), float(law[2])
+            tm = cm = td = cd = None
+            nm = nd = 0
+        else:
+            Ms, a = 0.0, 1.0
+            tm, cm = _spline_arrays(law[1], "spl_mu")
+            td, cd = _spline_arrays(law[2], "spl_dmu")
+            nm, nd = tm.size, td.size
+        s = self._L.fk_newton_saddle(self.handle, uptr(reg), dptr(b), Ms, a, dptr(tm), nm, dptr(cm), dptr(td), nd,
+                                     dptr(cd), float(rtol), float(tol_H), int(max_outer), float(min_step),
+                                     ctypes.byref(outer), ctypes.byref(cg), ctypes.byref(ch), ctypes.byref(res))
+        if raise_on_error:
+            self.ctx.check(s, "fk_newton_saddle")
+        return {"outer": int(outer.value), "cg_iterations": int(cg.value), "change": float(ch.value),
+                "residue": float(res.value), "status": s}
+
     def demag_field(self, inside, M, volumes, rtol: float = 1.4901161193847656e-08, max_iter: int = 100000,
                     warm: bool = False):
         """magnetostaticField (Micromagnetics/LL.jl:162-209) with the resident
@@ -408,6 +441,14 @@ class DeviceMesh:
         self.ctx.check(self._L.fk_sell_info(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
                        "fk_sell_info")
         return int(a.value), int(b.value), int(c.value)
+
+    def sell_layout(self):
+        """(index bytes, batch groups) of the SELL copy: 2 (int16 offsets from
+        the row) or 4 (int32 columns); 4 when the FP32 product loads 4 groups
+        of a row at once, 0 when it walks them one by one."""
+        b, g = ctypes.c_int32(0), ctypes.c_int32(0)
+        self.ctx.check(self._L.fk_sell_layout(self.handle, ctypes.byref(b), ctypes.byref(g)), "fk_sell_layout")
+        return int(b.value), int(g.value)
 
     def set_sell_shape(self, block: int, unroll: int = 1) -> None:
         """SELL SpMV launch shape: threads per block and slices in flight per wave."""

@@ -300,6 +300,28 @@ int fk_nonlinear_spline(fk_mesh *mesh, const uint8_t *region, const double *t_mu
                         const double *c_mu, const double *t_dmu, int32_t n_dmu, const double *c_dmu,
                         double rtol, double tol_H, int32_t max_outer, double switch_tol, int32_t *outer,
                         int64_t *cg_iterations, double *final_change);
+/* The reference's own nonlinear driver, Magnetostatics/NewtonRaphson.jl:
+ * 109-212, on the GPU: Neumann load and the Lagrange multiplier of the
+ * constant, every Newton step damped by lineSearch. rhs (nv, caller node
+ * order) is the driver's RHS = -BoundaryIntegral(mesh, Bext, shell_id)
+ * (fk_boundary_integral, negated). From u = 0 and mu = mu0: per attempt,
+ * A = A(mu) on the element region's law (mu0 elsewhere), on attempts after
+ * the first plus the tangent of tangentialStiffnessMatrix at the current
+ * field and dmu (src/FEM3D.jl:79-119); du from [A+At Lag; Lag' 0][du; l] =
+ * [rhs - A u; 0] (projected Jacobi-PCG to rtol relative to the right-hand
+ * side, Lag' du = 0); u updated by lineSearch(u, du, A, rhs, min_step)
+ * (src/FEM3D.jl:123-152); then H, change = mu0 max | |H| - |H|_prev | over
+ * the region, mu and dmu from the law. Loops while change > tol_H and
+ * attempts < max_outer (FK_ERR_NOT_CONVERGED if the change is still above).
+ * The law: with t_mu == NULL the Froehlich law mu0 (1 + Ms/(a+|H|)), else
+ * the tabulated splines as for fk_nonlinear_spline (|H| outside the table:
+ * FK_ERR_DOMAIN). outer: attempts; cg_iterations: PCG iterations in all;
+ * final_change: the last change; residue: the last lineSearch residue. The
+ * potential and mu stay resident (fk_potential, fk_get_coefficient). */
+int fk_newton_saddle(fk_mesh *mesh, const uint8_t *region, const double *rhs, double Ms, double a, const double *t_mu,
+                     int32_t n_mu, const double *c_mu, const double *t_dmu, int32_t n_dmu, const double *c_dmu,
+                     double rtol, double tol_H, int32_t max_outer, double min_step, int32_t *outer,
+                     int64_t *cg_iterations, double *final_change, double *residue);
 /* lineSearch(u, du, A, RHS, minStep) (src/FEM3D.jl:123-152) with A the last
  * assembled matrix (all rows): alpha <- 0.9 alpha until
  * ||RHS - A(u + alpha du)|| <= ||RHS - A u||; u (nv) is updated in place
@@ -406,9 +428,10 @@ int fk_set_lanes_per_row(fk_mesh *mesh, int32_t lanes);
  *     off-diagonal pair is stored once. FK_ERR_UNSUPPORTED on other patterns.
  *   FK_SPMV_AUTO (default): FK_SPMV_STENCIL when the pattern allows it (at
  *     most 25% padding), else FK_SPMV_SELL64.
- * The SELL and stencil kernels compute the FP32 product in the difference
- * form sum_j a_ij (x_j - x_i) (rows of a P1 stiffness matrix sum to zero),
- * the FP64 product in the standard form. Results agree to rounding. */
+ * The SELL and stencil kernels compute the PCG product in the difference
+ * form sum_j a_ij (x_j - x_i) (rows of a P1 stiffness matrix sum to zero)
+ * in both precisions, the FP64 true residual of the mixed solves in the
+ * standard form with the assembled diagonal. Results agree to rounding. */
 #define FK_SPMV_CSR 0
 #define FK_SPMV_SELL64 1
 #define FK_SPMV_SELL64_ROWIDX 2
@@ -422,11 +445,18 @@ int fk_solver_format(fk_mesh *mesh, int32_t *format, int32_t *stencil_offsets, i
 /* SELL copy statistics (built on demand): slices, slices in the shared-offset
  * form, stored entries (values including padding). Any pointer may be NULL. */
 int fk_sell_info(fk_mesh *mesh, int64_t *nslices, int64_t *shared_slices, int64_t *entries);
-/* Launch shape of the SELL SpMV on the persistent grid of the solver
- * kernels: threads per block (256, 512, 1024; default 1024) and slices in
- * flight per wavefront (1 or 2; 1024 threads only with 1). Rows are computed
- * alike in every shape; the p.q partial sums group differently, so solves
- * agree to rounding. */
+/* SELL copy layout (built on demand): bytes per stored per-row index, 2
+ * (int16 offsets from the row, when every column lies within 32767 of its
+ * row) or 4 (int32 columns); and the groups of 4 entries the FP32 product
+ * loads at once (4), or 0 when it walks the groups one by one (slices wider
+ * than 19 entries). Either pointer may be NULL. */
+int fk_sell_layout(fk_mesh *mesh, int32_t *index_bytes, int32_t *batch_groups);
+/* Launch shape of the FP32 SELL SpMV: threads per block (256, 512, 1024;
+ * default 256, one 64-row slice per wavefront, blocks swept in dispatch
+ * order per XCD) and slices in flight per wavefront (1 or 2; 1024 threads
+ * only with 1). The FP64 product runs on 1024-thread blocks of the solver
+ * grid. Rows are computed alike in every shape; the p.q partial sums group
+ * differently, so solves agree to rounding. */
 int fk_set_sell_shape(fk_mesh *mesh, int32_t block, int32_t unroll);
 
 #ifdef __cplusplus

@@ -438,6 +438,56 @@ def newton_spline(mesh, mask, g, magnet: np.ndarray, spl_mu, spl_dmu, rtol: floa
 
 
 # --------------------------------------------------------------------------
+# The reference's nonlinear driver: Neumann + Lagrange saddle Newton with
+# lineSearch damping (Magnetostatics/NewtonRaphson.jl:109-212).
+def newton_saddle(mesh, RHS: np.ndarray, magnet: np.ndarray, law, tol_H: float = 1e-4, max_att: int = 100,
+                  min_step: float = 1e-4):
+    """NewtonRaphson.jl:148-212 step by step, with the correct H_z of
+    manyBodyIntersecting.jl:241-266 (the driver's Hfield[2,k] -= u[nd]*d is a
+    typo). RHS = -BoundaryIntegral(mesh, Bext, shell_id) (:109); law(Hn) ->
+    (mu, dmu) on the magnet elements (spl, spl_dmu of :57-63). Per attempt:
+    A = A(mu) on the fixed pattern (:154-161), At = tangentialStiffnessMatrix
+    of the previous field and dmu from the second attempt on (:164-168), du
+    from the bordered system [A+At Lag; Lag' 0] \\ [RHS - A u; 0] (:171,
+    direct solve), u updated by lineSearch(u, du, A, RHS) (:175), then H,
+    div = mu0 max |H - Hold| over the magnet (:200), mu and dmu from the law
+    (:204-207) and the non-finite check (:210-212); the loop runs while
+    div > tol_H and att < max_att. Returns dict(phi, H, Hn, mu, dmu, iters,
+    div, residues, divs)."""
+    Ak = localStiffnessMatrix(mesh)
+    Lag = lagrange(mesh)
+    mu = np.full(mesh.nt, MU0)
+    dmu = np.zeros(mesh.nt)
+    u = np.zeros(mesh.nv)
+    H = np.zeros((3, mesh.nt))
+    Hn = np.zeros(mesh.nt)
+    att = 0
+    div = np.inf
+    residues, divs = [], []
+    while div > tol_H and att < max_att:
+        att += 1
+        Hold = Hn.copy()
+        A = sparse_from_local(mesh, Ak, mu)
+        if att > 1:
+            At = tangentialStiffnessMatrix(mesh, H, dmu)
+            J = CSC(A.n, A.colptr, A.rowval, A.nzval + At.nzval)
+        else:
+            J = A
+        du, _lam = saddle_solve(J, Lag, RHS - spmv(A, u))
+        residues.append(lineSearch(u, du, A, RHS, min_step))
+        H, Hn = field(mesh, u)
+        div = float(MU0 * np.max(np.abs(Hn[magnet] - Hold[magnet]))) if magnet.any() else 0.0
+        divs.append(div)
+        mu_m, dmu_m = law(Hn[magnet])
+        mu[magnet] = mu_m
+        dmu[magnet] = dmu_m
+        if not (np.all(np.isfinite(mu)) and np.all(np.isfinite(dmu))):
+            raise FloatingPointError("Nans/Infs in mu")
+    return {"phi": u, "H": H, "Hn": Hn, "mu": mu, "dmu": dmu, "iters": att, "div": div, "residues": residues,
+            "divs": divs}
+
+
+# --------------------------------------------------------------------------
 # Adjoint of the field-target objective (no reference counterpart).
 def adjoint_gradient(mesh, mask, g, rho, target: np.ndarray, Hstar, rtol: float = 1e-13,
                      mur=lambda r: 1.0 + 999.0 * r ** 3, dmur=lambda r: 3.0 * 999.0 * r ** 2):

@@ -121,3 +121,44 @@ def test_line_search_refuses_replaced_values(oracle):
         F.lineSearch(u, du, A1, RHS)
     F.lineSearch(u, du, A2, RHS)  # the current values
     F.invalidate(pr.mesh)
+
+
+def _saddle_case(oracle, tabulated):
+    import os
+    from femeko_hip.material import Spline1D, material_permeability
+    from femeko_hip.mesh import MU0, boundary_faces
+    if tabulated:
+        # the reference driver's own setting: Gd at 270 K, Bext = 0.5 T (NewtonRaphson.jl:30-36)
+        pr = W.tabulated(n=8, seed=12)
+        m = boundary_faces(pr.mesh)
+        RHS = -oracle.BoundaryIntegral(m, np.array([0.5, 0.0, 0.0]), m.shell_id)
+        tab = np.load(os.path.join(os.path.dirname(__file__), "golden", "gd270_bh.npz"))
+        mu_t, dmu_t, _ = material_permeability(tab["H"], tab["B"])
+        spl, spld = Spline1D(tab["H"], mu_t), Spline1D(tab["H"], dmu_t)
+        law_ref = lambda h: (oracle.bspline_eval(spl.t, spl.c, h), oracle.bspline_eval(spld.t, spld.c, h))  # noqa
+        return m, pr.magnet, RHS, law_ref, ("spline", spl, spld), 1e-6
+    pr = W.froehlich(n=8, seed=12)
+    m = boundary_faces(pr.mesh)
+    RHS = -oracle.BoundaryIntegral(m, MU0 * np.array([1e4, 3e3, -2e3]), m.shell_id)
+    return m, pr.magnet, RHS, (lambda h: oracle.froehlich(h, 1e5, 500.0)), ("froehlich", 1e5, 500.0), 1e-9
+
+
+@pytest.mark.parametrize("tabulated", [False, True])
+def test_newton_saddle_matches_oracle(oracle, tabulated):
+    """fk_newton_saddle (NewtonRaphson.jl:109-212 on the GPU) against the
+    oracle's restatement with direct bordered solves: the same number of
+    attempts, potential within 1e-8, mu within 1e-6, the last lineSearch
+    residue within 1e-6 relative."""
+    m, magnet, RHS, law_ref, law, tol = _saddle_case(oracle, tabulated)
+    ref = oracle.newton_saddle(m, RHS, magnet, law_ref, tol_H=tol)
+    dm = DeviceMesh(m, context(0))
+    try:
+        out = dm.newton_saddle(magnet.astype(np.uint8), RHS, law, rtol=1e-12, tol_H=tol)
+        assert out["outer"] == ref["iters"]
+        assert out["change"] <= tol
+        phi = dm.potential()
+        assert np.linalg.norm(phi - ref["phi"]) / np.linalg.norm(ref["phi"]) < 1e-8
+        np.testing.assert_allclose(dm.get_coefficient(), ref["mu"], rtol=1e-6)
+        assert out["residue"] == pytest.approx(ref["residues"][-1], rel=1e-6, abs=1e-7 * ref["residues"][1])
+    finally:
+        dm.close()

@@ -329,6 +329,47 @@ def test_line_search_closed_form(oracle, case):
 
 
 # ---------------------------------------------------------------------------
+# the reference's saddle Newton loop (NewtonRaphson.jl:109-212)
+def _neumann_case(oracle, n=8, seed=12, Hext=(1e4, 3e3, -2e3)):
+    from femeko_hip.mesh import boundary_faces
+    pr = W.froehlich(n=n, seed=seed)
+    m = boundary_faces(pr.mesh)
+    RHS = -oracle.BoundaryIntegral(m, MU0 * np.asarray(Hext, float), m.shell_id)
+    return m, pr.magnet, RHS
+
+
+def test_newton_saddle_linear_law_is_the_saddle_solve(oracle):
+    """A law that does not depend on |H| (mu = 50 mu0, dmu = 0): attempt 1
+    solves with mu0, attempt 2 with the law's mu exactly (the tangent is 0,
+    lineSearch takes the full step), attempt 3 changes nothing, so the loop
+    ends after 3 attempts at the bordered direct solve with that mu."""
+    m, magnet, RHS = _neumann_case(oracle)
+    law = lambda h: (np.full(h.size, 50.0 * MU0), np.zeros(h.size))  # noqa: E731
+    r = oracle.newton_saddle(m, RHS, magnet, law, tol_H=1e-12)
+    assert r["iters"] == 3 and r["div"] < 1e-15
+    mu = np.where(magnet, 50.0 * MU0, MU0)
+    u, _ = oracle.saddle_solve(oracle.stiffnessMatrix(m, mu), oracle.lagrange(m), RHS)
+    assert np.linalg.norm(r["phi"] - u) / np.linalg.norm(u) < 1e-12
+
+
+def test_newton_saddle_converges_to_the_nonlinear_fixed_point(oracle):
+    """Froehlich law: at the end u solves A(mu(|grad u|)) u = RHS with mu from
+    the law evaluated here at the final field (not the loop's arrays), keeps
+    Lag'u = 0, and the last changes fall quadratically (Newton)."""
+    m, magnet, RHS = _neumann_case(oracle)
+    Ms, a = 1e5, 500.0
+    r = oracle.newton_saddle(m, RHS, magnet, lambda h: oracle.froehlich(h, Ms, a), tol_H=1e-9)
+    assert r["div"] <= 1e-9 and r["iters"] < 100
+    _H, Hn = oracle.field(m, r["phi"])
+    mu = np.where(magnet, MU0 * (1.0 + Ms / (a + Hn)), MU0)
+    A = oracle.stiffnessMatrix(m, mu)
+    assert np.linalg.norm(RHS - oracle.spmv(A, r["phi"])) <= 1e-9 * np.linalg.norm(RHS)
+    assert abs(oracle.lagrange(m) @ r["phi"]) <= 1e-12 * np.abs(r["phi"]).max()
+    d = r["divs"]
+    assert d[-1] < 1e-3 * d[-2] and d[-2] < 1e-2 * d[-3]
+
+
+# ---------------------------------------------------------------------------
 # A6/A10: tangent and the mu update
 def test_froehlich_derivative_fd(oracle):
     H = np.array([0.0, 10.0, 500.0, 3e4])
