@@ -192,8 +192,11 @@ def bench_batch(args):
                      for p in probs])
     b.assemble()
     lanes, block, us = b.autotune()
+    if args.batch_update_block:
+        b.set_launch(lanes, args.batch_update_block)
     fmt = b.solver_format()
-    log(f"autotuner: SpMM {fmt} lanes {lanes} block {block} ({us:.1f} us per SpMM)")
+    launch = b.launch()
+    log(f"autotuner: SpMM {fmt} {launch} ({us:.1f} us per SpMM)")
 
     def step():
         b.assemble()
@@ -212,7 +215,7 @@ def bench_batch(args):
                   "DOF/s", t * 1e3, "f32+f64",
                   {"workload": "batch64: 64 Kuhn meshes n~U{10..60}, one mu box each, H0 = x, y, z",
                    "meshes": len(probs), "systems": 3 * len(probs), "nodes": nodes, "nnz": b.total_nnz,
-                   "solver_format": fmt, "autotuned_lanes": lanes, "autotuned_block": block})
+                   "solver_format": fmt, "autotuned_launch": launch})
     out["per_solve_ms"] = t * 1e3 / (3 * len(probs))
     out["iterations_max"] = int(rep.iterations.max())
     out["iterations_mean"] = float(rep.iterations.mean())
@@ -415,6 +418,8 @@ def main():
                     help="renumber the nodes (reverse Cuthill-McKee or random) to measure a general, "
                          "non-stencil numbering of the same mesh")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-update-block", type=int, default=0, choices=[0, 128, 256, 512],
+                    help="batch workload: the update kernels' block size instead of the autotuner's")
     ap.add_argument("--workload", default="boxes",
                     choices=["boxes", "batch", "froehlich", "tabulated", "demag", "adjoint"],
                     help="boxes: the headline n=160 mixed-precision line (default); the others measure the "

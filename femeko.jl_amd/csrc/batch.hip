@@ -43,6 +43,7 @@ struct fk_batch {
     // block table of the stencil SpMM (finer: 1024 rows per block)
     int sp_nblocks = 0;
     int sp_block = 256;         // threads per block of the stencil SpMM
+    int sp_rpt = 2;             // rows per thread and step of the stencil SpMM
     fk::DevBuf<int32_t> sp_blk_mesh, sp_mesh_b0, sp_mesh_b1;
     fk::DevBuf<int64_t> sp_blk_r0, sp_blk_r1;
     // Dirichlet values and potentials, node major [node][rhs]
@@ -59,7 +60,7 @@ struct fk_batch {
     fk::HostBuf<double> hpart;
     fk::HostBuf<int32_t> hcount;
     hipGraphExec_t graph = nullptr;
-    int graph_cfg = -1;
+    int64_t graph_cfg = -1;
     int lanes = 8, block = 512;  // block: the update kernels (and the CSR SpMM)
     // reliable updates, per system
     int precision = FK_PREC_F32_REFINED;
@@ -67,7 +68,7 @@ struct fk_batch {
     fk::DevBuf<double> rr32, rr64, rr_last, thresh64, part_res;
     fk::DevBuf<float> alpha;    // per system: step length of the current iteration, for the p-update kernel
     hipGraphExec_t ru_graph = nullptr;
-    int ru_graph_cfg = -1;
+    int64_t ru_graph_cfg = -1;
     double ru_delta = -1.0;     // baked into the captured chunk
     int64_t ru_max_iter = -1;
     // per-mesh stencil copy (every mesh's pattern a stencil of <= 16 offsets)
@@ -187,8 +188,11 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm(BatchArgs ba, const int32_t *__
 // read at row i - o (the matrix is exactly symmetric). Difference form; NR
 // right-hand sides per row. KC > 0: every mesh has KC offsets (unrolled);
 // KC = 0: the mesh's own count. Loads are unconditional: out-of-mesh
-// neighbours are clamped to a row of the mesh and weighted zero.
-template <int BLOCK, int NR, int KC>
+// neighbours are clamped to a row of the mesh and weighted zero. RPT rows per
+// thread and step (rows BLOCK apart; those past the block's range are
+// clamped to the first and not stored), so their loads are issued together:
+// more loads in flight per wave against more registers (fewer waves).
+template <int BLOCK, int NR, int KC, int RPT>
 __global__ __launch_bounds__(BLOCK) void kb_spmm_stencil(BatchArgs ba, const int64_t *__restrict__ node_off,
                                                          const int32_t *__restrict__ Ks,
                                                          const int32_t *__restrict__ offs,
@@ -210,24 +214,25 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm_stencil(BatchArgs ba, const int
     double acc[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-    // two rows per thread and step (the second clamped to the first past the
-    // block's range, so both rows' loads are issued together)
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += 2 * BLOCK) {
-        const bool two = i + BLOCK < r1;
-        const int64_t ib = two ? i + BLOCK : i;
-        const int32_t li[2] = {static_cast<int32_t>(i - a), static_cast<int32_t>(ib - a)};
-        float xi[2][NR], s[2][NR];
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += RPT * BLOCK) {
+        bool own[RPT];
+        int32_t li[RPT];
+        float xi[RPT][NR], s[RPT][NR];
 #pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            xi[0][j] = p[i * NR + j];
-            xi[1][j] = p[ib * NR + j];
-            s[0][j] = 0.f;
-            s[1][j] = 0.f;
+        for (int h = 0; h < RPT; ++h) {
+            const int64_t ih = i + static_cast<int64_t>(h) * BLOCK;
+            own[h] = h == 0 || ih < r1;
+            li[h] = static_cast<int32_t>((own[h] ? ih : i) - a);
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                xi[h][j] = p[(a + li[h]) * NR + j];
+                s[h][j] = 0.f;
+            }
         }
         auto term = [&](int k, int32_t o) {
             const float *__restrict__ v = v0 + static_cast<int64_t>(k) * n;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < RPT; ++h) {
                 const float wu = v[li[h]];
                 const int32_t lu = min(li[h] + o, static_cast<int32_t>(n - 1));
                 const bool lo_ok = li[h] >= o;
@@ -246,15 +251,13 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm_stencil(BatchArgs ba, const int
             for (int k = 0; k < K; ++k) term(k, offs[16 * mesh + k]);
         }
 #pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            q[i * NR + j] = s[0][j];
-            acc[j] += static_cast<double>(xi[0][j]) * static_cast<double>(s[0][j]);
-        }
-        if (two) {
+        for (int h = 0; h < RPT; ++h) {
+            if (!own[h]) continue;
+            const int64_t row = a + li[h];
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
-                q[ib * NR + j] = s[1][j];
-                acc[j] += static_cast<double>(xi[1][j]) * static_cast<double>(s[1][j]);
+                q[row * NR + j] = s[h][j];
+                acc[j] += static_cast<double>(xi[h][j]) * static_cast<double>(s[h][j]);
             }
         }
     }
@@ -1051,9 +1054,14 @@ static void launch_spmm_lanes(fk_batch *b, int lanes) {
 
 template <int BLOCK, int NR, int KC>
 static void launch_spmm_stencil_k(fk_batch *b) {
-    kb_spmm_stencil<BLOCK, NR, KC><<<b->sp_nblocks, BLOCK, 0, b->ctx->stream>>>(
-        spmm_args_of(b), b->node_off_dev.get(), b->st_K.get(), b->st_off.get(), b->st_vbase.get(), b->st_val32.get(),
-        b->p.get(), b->q.get(), b->part_pq.get(), b->done.get());
+    if (b->sp_rpt == 1)
+        kb_spmm_stencil<BLOCK, NR, KC, 1><<<b->sp_nblocks, BLOCK, 0, b->ctx->stream>>>(
+            spmm_args_of(b), b->node_off_dev.get(), b->st_K.get(), b->st_off.get(), b->st_vbase.get(),
+            b->st_val32.get(), b->p.get(), b->q.get(), b->part_pq.get(), b->done.get());
+    else
+        kb_spmm_stencil<BLOCK, NR, KC, 2><<<b->sp_nblocks, BLOCK, 0, b->ctx->stream>>>(
+            spmm_args_of(b), b->node_off_dev.get(), b->st_K.get(), b->st_off.get(), b->st_vbase.get(),
+            b->st_val32.get(), b->p.get(), b->q.get(), b->part_pq.get(), b->done.get());
 }
 
 // 7 offsets: the Kuhn pattern of structured boxes; other counts run the
@@ -1094,33 +1102,43 @@ static void spmm(fk_batch *b, int block, int lanes) {
     }
 }
 
+// dinv: the Jacobi inverse diagonal the kernels read (the mesh's FP32 one,
+// set up by the solve, unless the caller passes another of nv entries)
 template <int BLOCK, int NR>
-static void iteration_tail(fk_batch *b, int parity, int64_t max_iter) {
+static void iteration_tail(fk_batch *b, int parity, int64_t max_iter, const float *dinv_in = nullptr) {
     hipStream_t st = b->ctx->stream;
     const int nb = b->nblocks, nr = b->nrhs;
     double *rz_cur = b->part_rz.get() + (parity ? nb * nr : 0);
     double *rz_next = b->part_rz.get() + (parity ? 0 : nb * nr);
     fk_mesh *m = b->mesh;
+    const float *dinv = dinv_in ? dinv_in : m->w32.dinv.get();
+    if (!dinv) fail(FK_ERR_STATE, "batch iteration without a Jacobi diagonal");
     kb_update_xr<BLOCK, NR><<<nb, BLOCK, 0, st>>>(args_of(b), b->r.get(), b->q.get(),
-                                              m->w32.dinv.get(), b->part_pq.get(),
+                                              dinv, b->part_pq.get(),
                                               b->stencil == 1 ? b->sp_mesh_b0.get() : b->mesh_b0.get(),
                                               b->stencil == 1 ? b->sp_mesh_b1.get() : b->mesh_b1.get(), rz_cur, rz_next,
                                               b->part_rr.get(), b->done.get(), b->status.get(), b->ndone.get(),
                                               b->alpha.get());
     kb_update_p<BLOCK, NR><<<nb, BLOCK, 0, st>>>(args_of(b), b->x.get(), b->p.get(), b->r.get(), b->alpha.get(),
-                                             m->w32.dinv.get(), rz_next, rz_cur,
+                                             dinv, rz_next, rz_cur,
                                              b->part_rr.get(), b->done.get(), b->status.get(), b->iters.get(),
                                              b->thresh.get(), max_iter, b->ndone.get(), b->rr32.get());
 }
 
 template <int BLOCK>
-static void iteration_tail_nr(fk_batch *b, int parity, int64_t max_iter) {
+static void iteration_tail_nr(fk_batch *b, int parity, int64_t max_iter, const float *dinv = nullptr) {
     switch (b->nrhs) {
-        case 1: iteration_tail<BLOCK, 1>(b, parity, max_iter); break;
-        case 2: iteration_tail<BLOCK, 2>(b, parity, max_iter); break;
-        case 3: iteration_tail<BLOCK, 3>(b, parity, max_iter); break;
-        default: iteration_tail<BLOCK, 4>(b, parity, max_iter); break;
+        case 1: iteration_tail<BLOCK, 1>(b, parity, max_iter, dinv); break;
+        case 2: iteration_tail<BLOCK, 2>(b, parity, max_iter, dinv); break;
+        case 3: iteration_tail<BLOCK, 3>(b, parity, max_iter, dinv); break;
+        default: iteration_tail<BLOCK, 4>(b, parity, max_iter, dinv); break;
     }
+}
+
+// the launch shape a captured iteration chunk bakes in
+static int64_t launch_config(const fk_batch *b) {
+    return b->block + 1000 * static_cast<int64_t>(b->lanes) + 100000 * static_cast<int64_t>(b->sp_block) +
+           100000000 * static_cast<int64_t>(b->sp_rpt);
 }
 
 static void enqueue_batch_iteration(fk_batch *b, int parity, int64_t max_iter) {
@@ -1194,7 +1212,7 @@ static void inner_solve(fk_batch *b, double inner_rtol, int64_t max_iter) {
                                                    b->ndone.get(), b->nsys);
     FK_HIP(hipGetLastError());
     const int G = 16;
-    const int cfg = b->block * 1000 + b->lanes + 1000000 * b->sp_block;
+    const int64_t cfg = launch_config(b);
     if (!b->graph || b->graph_cfg != cfg) {
         if (b->graph) (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -1299,7 +1317,7 @@ static void batch_solve_ru(fk_batch *b, double rtol, double delta, int64_t max_i
     kb_ru_start<<<grid_for(S, 64), 64, 0, st>>>(u, target2.get(), b->thresh.get(), b->iters.get());
     FK_HIP(hipGetLastError());
     const int G = 16;
-    const int cfg = b->block * 1000 + b->lanes + 1000000 * b->sp_block;
+    const int64_t cfg = launch_config(b);
     const double delta_key = delta;
     if (!b->ru_graph || b->ru_graph_cfg != cfg || b->ru_delta != delta_key || b->ru_max_iter != max_iter) {
         if (b->ru_graph) (void)hipGraphExecDestroy(b->ru_graph);
@@ -1534,7 +1552,12 @@ void batch_solve(fk_batch *b, double rtol, double inner_rtol, int max_outer, int
         batch_solve_restart(b, rtol, inner_rtol, max_outer, max_iter, iters_out, outer_out, relres_out);
 }
 
-// time the SpMM kernel for every (block, lanes) candidate on the batch itself
+// Autotuner, on the batch itself. The SpMM: every (block, lanes) candidate
+// of the CSR kernel, or every (block, rows per thread) candidate of the
+// stencil kernel. Then the update kernels' block: both kernels of an
+// iteration tail on a synthetic state in which every system is active and
+// none can finish (threshold 0, positive partials, finite vectors), so each
+// launch does full work. The solve initialises all of that state again.
 void batch_autotune(fk_batch *b, int32_t *lanes_out, int32_t *block_out, double *best_us_out) {
     fk_mesh *m = b->mesh;
     if (!m->have_values) fail(FK_ERR_STATE, "batch not assembled");
@@ -1550,39 +1573,96 @@ void batch_autotune(fk_batch *b, int32_t *lanes_out, int32_t *block_out, double 
     hipEvent_t e0, e1;
     FK_HIP(hipEventCreate(&e0));
     FK_HIP(hipEventCreate(&e1));
-    double best = 1e30;
-    int bl = b->lanes, bb = 256;
+    auto time_us = [&](auto &&launch) {
+        launch();  // warm
+        FK_HIP(hipEventRecord(e0, st));
+        const int reps = 10;
+        for (int k = 0; k < reps; ++k) launch();
+        FK_HIP(hipEventRecord(e1, st));
+        FK_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return ms * 1e3 / reps;
+    };
     const int blocks[3] = {128, 256, 512};
-    const int lanes[3] = {4, 8, 16};
-    // the stencil SpMM has no lanes per row: block sizes only
-    const int nl = b->stencil == 1 ? 1 : 3;
-    for (int bi = 0; bi < 3; ++bi)
-        for (int li = 0; li < nl; ++li) {
-            spmm(b, blocks[bi], lanes[li]);  // warm
-            FK_HIP(hipEventRecord(e0, st));
-            const int reps = 10;
-            for (int k = 0; k < reps; ++k) spmm(b, blocks[bi], lanes[li]);
-            FK_HIP(hipEventRecord(e1, st));
-            FK_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            FK_HIP(hipEventElapsedTime(&ms, e0, e1));
-            const double us = ms * 1e3 / reps;
-            if (us < best) {
-                best = us;
-                bl = b->stencil == 1 ? b->lanes : lanes[li];
-                bb = blocks[bi];
-            }
-        }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    double best = 1e30;
     if (b->stencil == 1) {
-        b->sp_block = bb;  // the update kernels keep their block size
+        int bb = b->sp_block, br = b->sp_rpt;
+        for (int bi = 0; bi < 3; ++bi)
+            for (int rpt = 1; rpt <= 2; ++rpt) {
+                b->sp_rpt = rpt;
+                const double us = time_us([&] { spmm(b, blocks[bi], b->lanes); });
+                if (us < best) {
+                    best = us;
+                    bb = blocks[bi];
+                    br = rpt;
+                }
+            }
+        b->sp_block = bb;
+        b->sp_rpt = br;
     } else {
+        int bl = b->lanes, bb = b->block;
+        const int lanes[3] = {4, 8, 16};
+        for (int bi = 0; bi < 3; ++bi)
+            for (int li = 0; li < 3; ++li) {
+                const double us = time_us([&] { spmm(b, blocks[bi], lanes[li]); });
+                if (us < best) {
+                    best = us;
+                    bl = lanes[li];
+                    bb = blocks[bi];
+                }
+            }
         b->lanes = bl;
         b->block = bb;
     }
-    if (lanes_out) *lanes_out = bl;
-    if (block_out) *block_out = bb;
+    // update kernels (the CSR SpMM keeps the block chosen above)
+    if (b->stencil == 1) {
+        const size_t np = static_cast<size_t>(b->nblocks) * b->nrhs;
+        const size_t npq = std::max<size_t>(np, static_cast<size_t>(b->sp_nblocks) * b->nrhs);
+        std::vector<double> one(std::max(2 * np, npq), 1.0);
+        std::vector<double> thr(b->nsys, 0.0);
+        std::vector<int64_t> it0(b->nsys, 0);
+        FK_HIP(hipMemcpyAsync(b->part_pq.get(), one.data(), npq * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(b->part_rz.get(), one.data(), 2 * np * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(b->part_rr.get(), one.data(), np * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(b->thresh.get(), thr.data(), b->nsys * sizeof(double), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(b->iters.get(), it0.data(), b->nsys * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(b->status.get(), zero.data(), b->nsys * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(b->ndone.get(), zero.data(), sizeof(int32_t), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(b->r.get(), ones.data(), nvr * sizeof(float), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemcpyAsync(b->x.get(), ones.data(), nvr * sizeof(float), hipMemcpyHostToDevice, st));
+        FK_HIP(hipMemsetAsync(b->q.get(), 0, nvr * sizeof(float), st));
+        // the kernels' Jacobi diagonal: a private one (the mesh's is set up
+        // by the first solve, which may not have run yet)
+        DevBuf<float> dinv1(static_cast<size_t>(m->nv));
+        FK_HIP(hipMemcpyAsync(dinv1.get(), ones.data(), m->nv * sizeof(float), hipMemcpyHostToDevice, st));
+        double best_tail = 1e30;
+        int bt = b->block;
+        const int64_t huge = int64_t(1) << 40;
+        for (int bi = 0; bi < 3; ++bi) {
+            int parity = 0;
+            const double us = time_us([&] {
+                switch (blocks[bi]) {
+                    case 128: iteration_tail_nr<128>(b, parity, huge, dinv1.get()); break;
+                    case 256: iteration_tail_nr<256>(b, parity, huge, dinv1.get()); break;
+                    default: iteration_tail_nr<512>(b, parity, huge, dinv1.get()); break;
+                }
+                parity ^= 1;
+            });
+            if (us < best_tail) {
+                best_tail = us;
+                bt = blocks[bi];
+            }
+        }
+        FK_HIP(hipGetLastError());
+        FK_HIP(hipStreamSynchronize(st));  // before dinv1 is freed
+        b->block = bt;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    FK_HIP(hipStreamSynchronize(st));
+    if (lanes_out) *lanes_out = b->lanes;
+    if (block_out) *block_out = b->stencil == 1 ? b->sp_block : b->block;
     if (best_us_out) *best_us_out = best;
 }
 
@@ -1782,6 +1862,15 @@ int fk_batch_set_format(fk_batch *b, int32_t format) {
 int fk_batch_solver_format(fk_batch *b, int32_t *format) {
     if (!b || !format) return FK_ERR_INVALID_ARG;
     *format = b->stencil < 0 ? -1 : (b->stencil == 1 ? FK_SPMV_STENCIL : FK_SPMV_CSR);
+    return FK_OK;
+}
+
+int fk_batch_launch(fk_batch *b, int32_t *lanes, int32_t *block, int32_t *spmm_block, int32_t *spmm_rows) {
+    if (!b) return FK_ERR_INVALID_ARG;
+    if (lanes) *lanes = b->lanes;
+    if (block) *block = b->block;
+    if (spmm_block) *spmm_block = b->sp_block;
+    if (spmm_rows) *spmm_rows = b->sp_rpt;
     return FK_OK;
 }
 

@@ -18,12 +18,22 @@ struct Grad {
     double det;
 };
 
-__device__ __forceinline__ Grad element_gradients(const double *__restrict__ X, const double *__restrict__ Y,
-                                                  const double *__restrict__ Z, int4 nd) {
-    const double x0 = X[nd.x], y0 = Y[nd.x], z0 = Z[nd.x];
-    const double ax = X[nd.y] - x0, ay = Y[nd.y] - y0, az = Z[nd.y] - z0;
-    const double bx = X[nd.z] - x0, by = Y[nd.z] - y0, bz = Z[nd.z] - z0;
-    const double cx = X[nd.w] - x0, cy = Y[nd.w] - y0, cz = Z[nd.w] - z0;
+// The 12 coordinates of an element's nodes (x, y, z per node), loaded apart
+// from the arithmetic so that callers can issue the loads of several
+// elements before using any.
+__device__ __forceinline__ void element_coords(const double *__restrict__ X, const double *__restrict__ Y,
+                                               const double *__restrict__ Z, int4 nd, double c[12]) {
+    c[0] = X[nd.x]; c[1] = Y[nd.x]; c[2] = Z[nd.x];
+    c[3] = X[nd.y]; c[4] = Y[nd.y]; c[5] = Z[nd.y];
+    c[6] = X[nd.z]; c[7] = Y[nd.z]; c[8] = Z[nd.z];
+    c[9] = X[nd.w]; c[10] = Y[nd.w]; c[11] = Z[nd.w];
+}
+
+__device__ __forceinline__ Grad gradients_of(const double c[12]) {
+    const double x0 = c[0], y0 = c[1], z0 = c[2];
+    const double ax = c[3] - x0, ay = c[4] - y0, az = c[5] - z0;
+    const double bx = c[6] - x0, by = c[7] - y0, bz = c[8] - z0;
+    const double cx = c[9] - x0, cy = c[10] - y0, cz = c[11] - z0;
     // cofactor vectors
     const double c1x = by * cz - bz * cy, c1y = bz * cx - bx * cz, c1z = bx * cy - by * cx;  // b x c
     const double c2x = cy * az - cz * ay, c2y = cz * ax - cx * az, c2z = cx * ay - cy * ax;  // c x a
@@ -38,6 +48,13 @@ __device__ __forceinline__ Grad element_gradients(const double *__restrict__ X, 
     G.g[0][1] = -(G.g[1][1] + G.g[2][1] + G.g[3][1]);
     G.g[0][2] = -(G.g[1][2] + G.g[2][2] + G.g[3][2]);
     return G;
+}
+
+__device__ __forceinline__ Grad element_gradients(const double *__restrict__ X, const double *__restrict__ Y,
+                                                  const double *__restrict__ Z, int4 nd) {
+    double c[12];
+    element_coords(X, Y, Z, nd, c);
+    return gradients_of(c);
 }
 
 // 16 bytes of T for one vector load / store

@@ -247,7 +247,7 @@ struct fk_mesh {
     fk::DevBuf<int64_t> inc_sptr;
     fk::DevBuf<int32_t> inc_width;
     fk::DevBuf<uint32_t> inc, inc_off;
-    fk::DevBuf<int4> tet_nat;
+    fk::DevBuf<int4> tet_nat;         // elements in caller order (uploaded with the mesh)
     fk::DevBuf<double> coef_nat;      // coefficient in caller order
     fk::DevBuf<double> dmu_nat;       // dmu in caller order (gather Newton assembly)
     bool coef_nat_valid = false;
@@ -291,6 +291,7 @@ void local_stiffness(fk_mesh *m, const double *f, double *Ak_host);
 void tangential(fk_mesh *m, const double *H, const double *dmu, double *nzval_host);
 void lagrange(fk_mesh *m, double *C_host);
 void upload_coefficient(fk_mesh *m, const double *f);
+const double *coef_caller_order(fk_mesh *m, bool may_allocate);
 void set_dirichlet(fk_mesh *m, const uint8_t *mask, const double *g);
 void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inner_rtol, int max_outer,
            int warm, int64_t *iters, int32_t *outer, double *relres);

@@ -182,8 +182,16 @@ void build_mesh(fk_mesh *m, const double *p, const int32_t *t, int32_t base, con
     }
     m->tet.alloc(nt);
     m->perm.alloc(nt);
+    m->tet_nat.alloc(nt);
     FK_HIP(hipMemcpy(m->tet.get(), tet.data(), nt * sizeof(int4), hipMemcpyHostToDevice));
     FK_HIP(hipMemcpy(m->perm.get(), perm.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
+    // the same elements in caller order: field recovery and the row-gather
+    // assembly walk this copy, whose node reads follow the mesh's own locality
+    for (int64_t e = 0; e < nt; ++e) {
+        const int32_t *nd = t + 4 * e;
+        tet[e] = make_int4(nd[0] - base, nd[1] - base, nd[2] - base, nd[3] - base);
+    }
+    FK_HIP(hipMemcpy(m->tet_nat.get(), tet.data(), nt * sizeof(int4), hipMemcpyHostToDevice));
     tet.clear(); tet.shrink_to_fit();
     perm.clear(); perm.shrink_to_fit();
     // ---- element checks

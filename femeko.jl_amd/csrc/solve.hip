@@ -836,7 +836,9 @@ __global__ void k_phi_axpy(double *__restrict__ phi, const T *__restrict__ x, do
     if (i < n) phi[i] += s * static_cast<double>(x[i]);
 }
 
-// H = -sum phi_i grad_i per element; |H|; energy partials 1/2 mu |H|^2 VE
+// H = -sum phi_i grad_i per element; |H|; energy partials 1/2 mu |H|^2 VE.
+// Elements in caller order (perm null: element e is written at e) or in
+// colour order (written at perm[e]); coef in the same order as tet.
 __global__ __launch_bounds__(kBlock) void k_field(const int4 *__restrict__ tet, const double *__restrict__ X,
                                                   const double *__restrict__ Y, const double *__restrict__ Z,
                                                   const int32_t *__restrict__ perm, const double *__restrict__ phi,
@@ -854,7 +856,7 @@ __global__ __launch_bounds__(kBlock) void k_field(const int4 *__restrict__ tet, 
 #pragma unroll
         for (int c = 0; c < 3; ++c) h[c] = -(u0 * G.g[0][c] + u1 * G.g[1][c] + u2 * G.g[2][c] + u3 * G.g[3][c]);
         const double hh = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
-        const int64_t o = perm[e];
+        const int64_t o = perm ? perm[e] : e;
         if (H) {
             H[3 * o] = h[0];
             H[3 * o + 1] = h[1];
@@ -2089,8 +2091,9 @@ void field(fk_mesh *m, double *H, double *Hn, double *energy) {
     if (Hn) dHn.alloc(m->nt);
     const int nb = std::max(1, m->nblocks);
     m->part_misc.ensure(nb);
-    k_field<<<nb, kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->perm.get(), m->phi.get(),
-                                   energy ? m->coef.get() : nullptr, m->nt, H ? dH.get() : nullptr,
+    // caller element order: coalesced H writes, node reads local to the mesh
+    k_field<<<nb, kBlock, 0, st>>>(m->tet_nat.get(), m->X.get(), m->Y.get(), m->Z.get(), nullptr, m->phi.get(),
+                                   energy ? coef_caller_order(m, true) : nullptr, m->nt, H ? dH.get() : nullptr,
                                    Hn ? dHn.get() : nullptr, m->part_misc.get());
     FK_HIP(hipGetLastError());
     if (H) FK_HIP(hipMemcpyAsync(H, dH.get(), 3 * m->nt * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -2236,8 +2239,16 @@ int fk_field_noalloc(fk_mesh *m, const double *phi, double *H, double *Hn, doubl
                                                                           m->phi_internal.get());
             phi = m->phi_internal.get();
         }
-        fk::k_field<<<nb, fk::kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->perm.get(), phi,
-                                               energy ? m->coef.get() : nullptr, m->nt, H, Hn, part);
+        // caller element order as fk_field, unless the energy needs a
+        // coefficient conversion buffer that was never allocated (this entry
+        // allocates nothing): then the colour-ordered copies
+        const double *coef_nat = energy ? fk::coef_caller_order(m, false) : nullptr;
+        if (!energy || coef_nat)
+            fk::k_field<<<nb, fk::kBlock, 0, st>>>(m->tet_nat.get(), m->X.get(), m->Y.get(), m->Z.get(), nullptr,
+                                                   phi, coef_nat, m->nt, H, Hn, part);
+        else
+            fk::k_field<<<nb, fk::kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
+                                                   m->perm.get(), phi, m->coef.get(), m->nt, H, Hn, part);
         if (energy) fk::k_sum_in_order<<<1, 64, 0, st>>>(part, nb, energy);
         FK_HIP(hipGetLastError());
     });

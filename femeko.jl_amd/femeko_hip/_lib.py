@@ -113,6 +113,7 @@ SIGNATURES = {
     "fk_batch_set_format": (_c.c_int, [_vp, _c.c_int32]),
     "fk_batch_solver_format": (_c.c_int, [_vp, _c.POINTER(_c.c_int32)]),
     "fk_batch_set_launch": (_c.c_int, [_vp, _c.c_int32, _c.c_int32]),
+    "fk_batch_launch": (_c.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "fk_set_timing": (_c.c_int, [_vp, _c.c_int32]),
     "fk_spmv_timing": (_c.c_int, [_vp, _i64p, _dp]),
     "fk_copy_probe": (_c.c_int, [_vp, _c.c_int64, _c.c_int32, _dp]),

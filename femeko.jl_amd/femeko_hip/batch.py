@@ -122,6 +122,13 @@ class Batch:
     def set_launch(self, lanes: int, block: int) -> None:
         self.ctx.check(self._L.fk_batch_set_launch(self.handle, int(lanes), int(block)), "fk_batch_set_launch")
 
+    def launch(self) -> dict:
+        """The launch shape in use: CSR lanes per row, the update kernels'
+        block, the stencil SpMM's block and rows per thread."""
+        v = [ctypes.c_int32(0) for _ in range(4)]
+        self.ctx.check(self._L.fk_batch_launch(self.handle, *[ctypes.byref(x) for x in v]), "fk_batch_launch")
+        return {"lanes": v[0].value, "block": v[1].value, "spmm_block": v[2].value, "spmm_rows_per_thread": v[3].value}
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             self._L.fk_batch_destroy(self.handle)

@@ -385,13 +385,17 @@ int fk_batch_set_format(fk_batch *batch, int32_t format);
 int fk_batch_solver_format(fk_batch *batch, int32_t *format);
 /* phi: nrhs * nv[mesh], right-hand side major */
 int fk_batch_potential(fk_batch *batch, int32_t mesh, double *phi);
-/* Autotuner: times the batched SpMM on this batch and keeps the fastest
- * launch: with the CSR copy lanes per row {4, 8, 16} x block size
- * {128, 256, 512} (the block size also of the update kernels); with the
- * stencil copy the SpMM's own block size {128, 256, 512} (lanes returned
- * unchanged). fk_batch_set_launch sets lanes and the update kernels' block. */
+/* Autotuner: times the batched kernels on this batch and keeps the fastest
+ * launch. With the CSR copy: lanes per row {4, 8, 16} x block size
+ * {128, 256, 512} of the SpMM (the block size also of the update kernels).
+ * With the stencil copy: the SpMM's block size {128, 256, 512} x rows per
+ * thread and step {1, 2}, then the update kernels' block size
+ * {128, 256, 512} (lanes returned unchanged; block = the SpMM's; best_us =
+ * the SpMM's time). fk_batch_set_launch sets lanes and the update kernels'
+ * block; fk_batch_launch reports the launch in use. */
 int fk_batch_autotune(fk_batch *batch, int32_t *lanes, int32_t *block, double *best_us);
 int fk_batch_set_launch(fk_batch *batch, int32_t lanes, int32_t block);
+int fk_batch_launch(fk_batch *batch, int32_t *lanes, int32_t *block, int32_t *spmm_block, int32_t *spmm_rows);
 
 /* ---- instrumentation -------------------------------------------------------
  * Average device time of the sparse matrix-vector kernel over the last solve
