@@ -23,7 +23,7 @@ the same system to the FP64 path's tolerance (the CPU baseline, plus a
 errors of the GPU potentials, |H| and energy are reported against.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160] [--no-cpu]
-                [--shape 256,1] [--delta 0.95] [--rtol 5e-11] [--rtol64 1e-10]
+                [--shape 256,2] [--delta 0.95] [--rtol 5e-11] [--rtol64 1e-10]
                 [--format auto|stencil|sell64|sell64-rowidx|csr]
                 [--workload boxes|batch|froehlich|tabulated|demag|adjoint]
 """
@@ -147,11 +147,12 @@ def oracle_legs(pr, nthreads, rtol64):
 
 
 def _pmc_traffic(args, solver_format):
-    """HBM bytes per SELL SpMV launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by what the same counters read
-    for the streaming kernel k_rr_fold of known traffic), when one exists for
-    this workload and launch shape."""
-    path = os.path.join(ROOT, "profiles", "pmc_spmv_f32_boxes160.txt")
+    """HBM bytes per FP32 SpMV launch from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by what the
+    same counters read for the plain copy k_copy16 of known traffic), when
+    one exists for this workload, format and launch shape."""
+    tag = "" if args.renumber == "none" else "-" + args.renumber
+    path = os.path.join(ROOT, "profiles", f"pmc_spmv_f32_boxes160{tag}.txt")
     if args.n != 160 or not os.path.exists(path):
         return None
     vals = {}
@@ -412,11 +413,13 @@ def main():
                     help="relative residual the plain FP64 solve stops at (the configured 1e-10)")
     ap.add_argument("--delta", type=float, default=0.95,
                     help="FP32 solve: residual drop that triggers an FP64 residual replacement")
-    ap.add_argument("--shape", default="256,1",
+    ap.add_argument("--shape", default="256,2",
                     help="FP32 SELL SpMV launch shape: threads per block, slices in flight per wave")
     ap.add_argument("--renumber", default="none", choices=["none", "rcm", "random"],
                     help="renumber the nodes (reverse Cuthill-McKee or random) to measure a general, "
                          "non-stencil numbering of the same mesh")
+    ap.add_argument("--reorder", default="auto", choices=["auto", "none", "rcm"],
+                    help="the library's internal node renumbering (fk_mesh_create_ex)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-update-block", type=int, default=0, choices=[0, 128, 256, 512],
                     help="batch workload: the update kernels' block size instead of the autotuner's")
@@ -454,19 +457,22 @@ def main():
     m = pr.mesh
     log(f"mesh n={args.n}: nt={m.nt} nv={m.nv} ({time.perf_counter()-t0:.1f} s)")
     t0 = time.perf_counter()
-    dm = DeviceMesh(m, ctx)
+    dm = DeviceMesh(m, ctx, reorder=args.reorder)
     dm.set_lanes(args.lanes)
     dm.set_format(args.format)
     shape = tuple(int(v) for v in args.shape.split(","))
     dm.set_sell_shape(*shape)
     log(f"device mesh: nnz={dm.nnz} colours={dm.ncolours} ({time.perf_counter()-t0:.1f} s)")
+    renumbered, dist_caller, dist_internal = dm.numbering()
+    log(f"node numbering: {'renumbered internally (RCM)' if renumbered else 'caller order kept'}; "
+        f"mean neighbour distance {dist_caller:.0f} (caller), {dist_internal:.0f} (internal)")
     fmt_name, stencil_offsets = dm.solver_format()
     sell = dm.sell_info() if fmt_name == "sell64" else None
-    sell_index = sell_groups = None
+    sell_index = sell_groups = sell_array_bytes = None
     if sell is not None:
         # int16 offsets from the row or int32 columns, and the batch width, as
         # the library chose them
-        ib, sell_groups = dm.sell_layout()
+        ib, sell_groups, sell_array_bytes = dm.sell_layout()
         sell_index = {2: "short", 4: "int"}[ib]
     log(f"solver format: {fmt_name} {stencil_offsets if stencil_offsets else ''}")
     if sell:
@@ -540,7 +546,14 @@ def main():
     spmv_us = dm.spmv_burst("f32", 100)
     B_csr = spmv_bytes(dm.nnz, dm.nv, 4)
     # the compulsory bytes of the copy the solver streams
-    B32 = stencil_spmv_bytes(len(stencil_offsets), dm.nv, 4) if fmt_name == "stencil" else B_csr
+    # the format's own compulsory bytes: the stencil copy's columns, or the
+    # SELL arrays (values with padding, indices, slice tables), plus x and y
+    if fmt_name == "stencil":
+        B32 = stencil_spmv_bytes(len(stencil_offsets), dm.nv, 4)
+    elif fmt_name == "sell64":
+        B32 = sell_array_bytes + 8 * dm.nv
+    else:
+        B32 = B_csr
     achieved = B32 / (spmv_us * 1e-6) / 1e9
     log(f"SpMV f32: {spmv_us:.1f} us per launch back to back, {achieved:.0f} GB/s; "
         f"{spmv_us_in_solve:.1f} us avg over {launches} launches in a solve with events around each")
@@ -567,6 +580,7 @@ def main():
             "nodes": m.nv, "tets": m.nt, "nnz": dm.nnz, "free_dofs": free, "colours": dm.ncolours,
             "lanes_per_row": args.lanes, "spmv_format": fmt_name, "stencil_offsets": list(stencil_offsets),
             "sell_shape": list(shape),
+            "internal_renumbering": renumbered,
             "sell_slices": sell[0] if sell else None, "sell_shared_slices": sell[1] if sell else None,
             "sell_entries": sell[2] if sell else None,
             "update_trigger": args.delta,
@@ -589,15 +603,17 @@ def main():
             "bound": "hbm",
             "kernel": {"csr": "k_cg_spmv<float,%d>" % args.lanes,
                        "sell64": "k_cg_spmv_sell<float,%s,%d,%d,%s>" % ((sell_index,) + shape + (sell_groups,)),
-                       "stencil": "k_cg_spmv_stencil<float,256>"}[fmt_name],
+                       "stencil": "k_cg_spmv_stencil<float,256,%d>" % len(stencil_offsets)}[fmt_name],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "bytes_per_launch": B32,
-            "bytes_basis": ("stencil copy: K*4*nv values + 4*nv x + 4*nv y" if fmt_name == "stencil"
-                            else "CSR: nnz*(4+4) + 4*(nv+1) + 8*nv"),
+            "bytes_basis": {"stencil": "stencil copy: K*4*nv values + 4*nv x + 4*nv y",
+                            "sell64": "SELL copy: 4 B per stored value (padding included) + its indices, "
+                                      "shared offsets and slice tables + 4*nv x + 4*nv y",
+                            "csr": "CSR: nnz*(4+4) + 4*(nv+1) + 8*nv"}[fmt_name],
             "csr_formula_bytes": B_csr,
             "csr_formula_GBs": B_csr / (spmv_us * 1e-6) / 1e9,
             "copy_probe_GBs": copy_gbs,

@@ -214,49 +214,57 @@ __global__ __launch_bounds__(BLOCK) void kb_spmm_stencil(BatchArgs ba, const int
     double acc[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += RPT * BLOCK) {
+    // the mesh's rows through block-uniform base pointers and 32-bit row
+    // indices (a mesh has fewer than 2^32 / NR nodes)
+    const float *__restrict__ pm = p + a * NR;
+    float *__restrict__ qm = q + a * NR;
+    const uint32_t nn = static_cast<uint32_t>(n);
+    const uint32_t l1 = static_cast<uint32_t>(r1 - a);
+    for (uint32_t l0 = static_cast<uint32_t>(r0 - a) + threadIdx.x; l0 < l1; l0 += RPT * BLOCK) {
         bool own[RPT];
-        int32_t li[RPT];
+        uint32_t li[RPT];
         float xi[RPT][NR], s[RPT][NR];
 #pragma unroll
         for (int h = 0; h < RPT; ++h) {
-            const int64_t ih = i + static_cast<int64_t>(h) * BLOCK;
-            own[h] = h == 0 || ih < r1;
-            li[h] = static_cast<int32_t>((own[h] ? ih : i) - a);
+            const uint32_t lh = l0 + static_cast<uint32_t>(h) * BLOCK;
+            own[h] = h == 0 || lh < l1;
+            li[h] = own[h] ? lh : l0;
+            const float *__restrict__ ph = pm + static_cast<size_t>(li[h]) * NR;
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
-                xi[h][j] = p[(a + li[h]) * NR + j];
+                xi[h][j] = ph[j];
                 s[h][j] = 0.f;
             }
         }
-        auto term = [&](int k, int32_t o) {
-            const float *__restrict__ v = v0 + static_cast<int64_t>(k) * n;
+        auto term = [&](int k, uint32_t o) {
+            const float *__restrict__ v = v0 + static_cast<uint64_t>(k) * nn;
 #pragma unroll
             for (int h = 0; h < RPT; ++h) {
                 const float wu = v[li[h]];
-                const int32_t lu = min(li[h] + o, static_cast<int32_t>(n - 1));
+                const uint32_t lu = min(li[h] + o, nn - 1);
                 const bool lo_ok = li[h] >= o;
-                const float wl = lo_ok ? v[lo_ok ? li[h] - o : 0] : 0.f;
-                const int32_t ll = lo_ok ? li[h] - o : 0;
-                const float *pu = p + (a + lu) * NR;
-                const float *pl = p + (a + ll) * NR;
+                const uint32_t ll = lo_ok ? li[h] - o : 0u;
+                const float wl = lo_ok ? v[ll] : 0.f;
+                // a row's NR values at one 64-bit address: one vector load
+                const float *__restrict__ pu = pm + static_cast<size_t>(lu) * NR;
+                const float *__restrict__ pl = pm + static_cast<size_t>(ll) * NR;
 #pragma unroll
                 for (int j = 0; j < NR; ++j) s[h][j] += wu * (pu[j] - xi[h][j]) + wl * (pl[j] - xi[h][j]);
             }
         };
         if constexpr (KC > 0) {
 #pragma unroll
-            for (int k = 0; k < KC; ++k) term(k, off[k]);
+            for (int k = 0; k < KC; ++k) term(k, static_cast<uint32_t>(off[k]));
         } else {
-            for (int k = 0; k < K; ++k) term(k, offs[16 * mesh + k]);
+            for (int k = 0; k < K; ++k) term(k, static_cast<uint32_t>(offs[16 * mesh + k]));
         }
 #pragma unroll
         for (int h = 0; h < RPT; ++h) {
             if (!own[h]) continue;
-            const int64_t row = a + li[h];
+            float *__restrict__ qh = qm + static_cast<size_t>(li[h]) * NR;
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
-                q[row * NR + j] = s[h][j];
+                qh[j] = s[h][j];
                 acc[j] += static_cast<double>(xi[h][j]) * static_cast<double>(s[h][j]);
             }
         }
@@ -1004,7 +1012,8 @@ static void upload_table(const TableHost &t, int nmesh, fk::DevBuf<int32_t> &bm,
 
 // the table of every kernel but the stencil SpMM: ~4 blocks per CU overall.
 // Not padded to a multiple of 8, so logical_block keeps dispatch order: with
-// XCD bands (padded) the batch step measured 80.4 ms against 76.9 ms.
+// XCD bands (padded) the 64-mesh step measured 76.8 ms against 72.3 ms, and
+// with every kernel on the SpMM's banded table of 1024-row blocks 89.9 ms.
 static void build_block_table(fk_batch *b) {
     const int64_t total = b->node_off.back();
     const int64_t target_blocks = std::max<int64_t>(b->nmesh, 4 * static_cast<int64_t>(b->ctx->num_cus));
@@ -1102,16 +1111,14 @@ static void spmm(fk_batch *b, int block, int lanes) {
     }
 }
 
-// dinv: the Jacobi inverse diagonal the kernels read (the mesh's FP32 one,
-// set up by the solve, unless the caller passes another of nv entries)
 template <int BLOCK, int NR>
-static void iteration_tail(fk_batch *b, int parity, int64_t max_iter, const float *dinv_in = nullptr) {
+static void iteration_tail(fk_batch *b, int parity, int64_t max_iter) {
     hipStream_t st = b->ctx->stream;
     const int nb = b->nblocks, nr = b->nrhs;
     double *rz_cur = b->part_rz.get() + (parity ? nb * nr : 0);
     double *rz_next = b->part_rz.get() + (parity ? 0 : nb * nr);
     fk_mesh *m = b->mesh;
-    const float *dinv = dinv_in ? dinv_in : m->w32.dinv.get();
+    const float *dinv = m->w32.dinv.get();  // set up by the solve
     if (!dinv) fail(FK_ERR_STATE, "batch iteration without a Jacobi diagonal");
     kb_update_xr<BLOCK, NR><<<nb, BLOCK, 0, st>>>(args_of(b), b->r.get(), b->q.get(),
                                               dinv, b->part_pq.get(),
@@ -1126,12 +1133,12 @@ static void iteration_tail(fk_batch *b, int parity, int64_t max_iter, const floa
 }
 
 template <int BLOCK>
-static void iteration_tail_nr(fk_batch *b, int parity, int64_t max_iter, const float *dinv = nullptr) {
+static void iteration_tail_nr(fk_batch *b, int parity, int64_t max_iter) {
     switch (b->nrhs) {
-        case 1: iteration_tail<BLOCK, 1>(b, parity, max_iter, dinv); break;
-        case 2: iteration_tail<BLOCK, 2>(b, parity, max_iter, dinv); break;
-        case 3: iteration_tail<BLOCK, 3>(b, parity, max_iter, dinv); break;
-        default: iteration_tail<BLOCK, 4>(b, parity, max_iter, dinv); break;
+        case 1: iteration_tail<BLOCK, 1>(b, parity, max_iter); break;
+        case 2: iteration_tail<BLOCK, 2>(b, parity, max_iter); break;
+        case 3: iteration_tail<BLOCK, 3>(b, parity, max_iter); break;
+        default: iteration_tail<BLOCK, 4>(b, parity, max_iter); break;
     }
 }
 
@@ -1552,12 +1559,13 @@ void batch_solve(fk_batch *b, double rtol, double inner_rtol, int max_outer, int
         batch_solve_restart(b, rtol, inner_rtol, max_outer, max_iter, iters_out, outer_out, relres_out);
 }
 
-// Autotuner, on the batch itself. The SpMM: every (block, lanes) candidate
-// of the CSR kernel, or every (block, rows per thread) candidate of the
-// stencil kernel. Then the update kernels' block: both kernels of an
-// iteration tail on a synthetic state in which every system is active and
-// none can finish (threshold 0, positive partials, finite vectors), so each
-// launch does full work. The solve initialises all of that state again.
+// Autotuner, on the batch itself, with every system active: the SpMM's
+// (block, lanes) candidates of the CSR kernel, or (block, rows per thread)
+// of the stencil kernel. The update kernels keep 512 threads per block with
+// the stencil SpMM: timed the same way (all systems active) 128 won, but in
+// the solve, where the systems of small meshes finish first and the last
+// iterations run a few large meshes' blocks, the 64-mesh step took 78.3 ms
+// with 128 against 72.3 ms with 512 (72.9 with 256).
 void batch_autotune(fk_batch *b, int32_t *lanes_out, int32_t *block_out, double *best_us_out) {
     fk_mesh *m = b->mesh;
     if (!m->have_values) fail(FK_ERR_STATE, "batch not assembled");
@@ -1614,49 +1622,6 @@ void batch_autotune(fk_batch *b, int32_t *lanes_out, int32_t *block_out, double 
             }
         b->lanes = bl;
         b->block = bb;
-    }
-    // update kernels (the CSR SpMM keeps the block chosen above)
-    if (b->stencil == 1) {
-        const size_t np = static_cast<size_t>(b->nblocks) * b->nrhs;
-        const size_t npq = std::max<size_t>(np, static_cast<size_t>(b->sp_nblocks) * b->nrhs);
-        std::vector<double> one(std::max(2 * np, npq), 1.0);
-        std::vector<double> thr(b->nsys, 0.0);
-        std::vector<int64_t> it0(b->nsys, 0);
-        FK_HIP(hipMemcpyAsync(b->part_pq.get(), one.data(), npq * sizeof(double), hipMemcpyHostToDevice, st));
-        FK_HIP(hipMemcpyAsync(b->part_rz.get(), one.data(), 2 * np * sizeof(double), hipMemcpyHostToDevice, st));
-        FK_HIP(hipMemcpyAsync(b->part_rr.get(), one.data(), np * sizeof(double), hipMemcpyHostToDevice, st));
-        FK_HIP(hipMemcpyAsync(b->thresh.get(), thr.data(), b->nsys * sizeof(double), hipMemcpyHostToDevice, st));
-        FK_HIP(hipMemcpyAsync(b->iters.get(), it0.data(), b->nsys * sizeof(int64_t), hipMemcpyHostToDevice, st));
-        FK_HIP(hipMemcpyAsync(b->status.get(), zero.data(), b->nsys * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        FK_HIP(hipMemcpyAsync(b->ndone.get(), zero.data(), sizeof(int32_t), hipMemcpyHostToDevice, st));
-        FK_HIP(hipMemcpyAsync(b->r.get(), ones.data(), nvr * sizeof(float), hipMemcpyHostToDevice, st));
-        FK_HIP(hipMemcpyAsync(b->x.get(), ones.data(), nvr * sizeof(float), hipMemcpyHostToDevice, st));
-        FK_HIP(hipMemsetAsync(b->q.get(), 0, nvr * sizeof(float), st));
-        // the kernels' Jacobi diagonal: a private one (the mesh's is set up
-        // by the first solve, which may not have run yet)
-        DevBuf<float> dinv1(static_cast<size_t>(m->nv));
-        FK_HIP(hipMemcpyAsync(dinv1.get(), ones.data(), m->nv * sizeof(float), hipMemcpyHostToDevice, st));
-        double best_tail = 1e30;
-        int bt = b->block;
-        const int64_t huge = int64_t(1) << 40;
-        for (int bi = 0; bi < 3; ++bi) {
-            int parity = 0;
-            const double us = time_us([&] {
-                switch (blocks[bi]) {
-                    case 128: iteration_tail_nr<128>(b, parity, huge, dinv1.get()); break;
-                    case 256: iteration_tail_nr<256>(b, parity, huge, dinv1.get()); break;
-                    default: iteration_tail_nr<512>(b, parity, huge, dinv1.get()); break;
-                }
-                parity ^= 1;
-            });
-            if (us < best_tail) {
-                best_tail = us;
-                bt = blocks[bi];
-            }
-        }
-        FK_HIP(hipGetLastError());
-        FK_HIP(hipStreamSynchronize(st));  // before dinv1 is freed
-        b->block = bt;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -266,7 +266,7 @@ struct fk_mesh {
 
     int lanes = 16;                   // lanes per CSR row in SpMV
     int sell_block = 256;             // threads per block of the FP32 SELL SpMV
-    int sell_unroll = 1;              // slices in flight per wave in the FP32 SELL SpMV
+    int sell_unroll = 2;              // slices in flight per wave in the FP32 SELL SpMV
     int nblocks = 0;                  // persistent grid of the solver kernels
     int64_t chunk = 0;                // rows per logical block
 

@@ -943,12 +943,13 @@ static int stencil_blocks(const fk_mesh *m) {
     return static_cast<int>(8 * std::max<int64_t>(1, (runs + 1) / 2));
 }
 
-// Blocks of the FP32 SELL SpMV: one 64-row slice per wave, so a block is as
-// short-lived as the stencil kernel's and the blocks of an XCD sweep its
-// band in dispatch order.
+// Blocks of the FP32 SELL SpMV: `unroll` 64-row slices per wave (in flight
+// together), so a block is as short-lived as the stencil kernel's and the
+// blocks of an XCD sweep its band in dispatch order; each block leaves one
+// p.q partial, which the next kernel re-sums in every block.
 static int sell_blocks(const fk_mesh *m) {
     const int64_t ns = (m->nv + 63) / 64;
-    const int64_t per = std::max(1, m->sell_block / 64);
+    const int64_t per = std::max(1, m->sell_block / 64) * std::max(1, m->sell_unroll);
     return static_cast<int>((((ns + per - 1) / per) + 7) / 8 * 8);
 }
 
@@ -2342,13 +2343,21 @@ int fk_sell_info(fk_mesh *m, int64_t *nslices, int64_t *shared_slices, int64_t *
     });
 }
 
-int fk_sell_layout(fk_mesh *m, int32_t *index_bytes, int32_t *batch_groups) {
+int fk_sell_layout(fk_mesh *m, int32_t *index_bytes, int32_t *batch_groups, int64_t *f32_array_bytes) {
     if (!m) return FK_ERR_INVALID_ARG;
     return guard(m->ctx, "fk_sell_layout", [&] {
         if (fk::solver_kind(m) != FK_SPMV_SELL64) fk::fail(FK_ERR_STATE, "the solver format is not SELL-64");
         fk::build_sell(m);
         if (index_bytes) *index_bytes = m->sell_short ? 2 : 4;
         if (batch_groups) *batch_groups = fk::sell_batch_groups(m);
+        if (f32_array_bytes) {
+            // what one FP32 product streams besides x and y: values (padding
+            // included), per-row indices, shared offsets, the slice tables
+            const int64_t ns = m->nslices;
+            *f32_array_bytes = m->sell_len * static_cast<int64_t>(sizeof(float)) +
+                               static_cast<int64_t>(m->sell_col.n) * 4 + static_cast<int64_t>(m->sell_dcol.n) * 2 +
+                               static_cast<int64_t>(m->sell_soff.n) * 4 + (ns + 1) * 8 + ns * 4 + ns * 8;
+        }
     });
 }
 

@@ -120,7 +120,7 @@ SIGNATURES = {
     "fk_spmv_burst": (_c.c_int, [_vp, _c.c_int32, _c.c_int32, _c.POINTER(_c.c_double)]),
     "fk_set_lanes_per_row": (_c.c_int, [_vp, _c.c_int32]),
     "fk_set_spmv_format": (_c.c_int, [_vp, _c.c_int32]),
-    "fk_sell_layout": (_c.c_int, [_vp, _ip, _ip]),
+    "fk_sell_layout": (_c.c_int, [_vp, _ip, _ip, _c.POINTER(_c.c_int64)]),
     "fk_set_sell_shape": (_c.c_int, [_vp, _c.c_int32, _c.c_int32]),
     "fk_sell_info": (_c.c_int, [_vp, _c.c_void_p, _c.c_void_p, _c.c_void_p]),
     "fk_solver_format": (_c.c_int, [_vp, _c.c_void_p, _c.c_void_p, _c.c_void_p]),

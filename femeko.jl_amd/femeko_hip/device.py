@@ -443,12 +443,14 @@ class DeviceMesh:
         return int(a.value), int(b.value), int(c.value)
 
     def sell_layout(self):
-        """(index bytes, batch groups) of the SELL copy: 2 (int16 offsets from
-        the row) or 4 (int32 columns); 4 when the FP32 product loads 4 groups
-        of a row at once, 0 when it walks them one by one."""
-        b, g = ctypes.c_int32(0), ctypes.c_int32(0)
-        self.ctx.check(self._L.fk_sell_layout(self.handle, ctypes.byref(b), ctypes.byref(g)), "fk_sell_layout")
-        return int(b.value), int(g.value)
+        """(index bytes, batch groups, FP32 array bytes) of the SELL copy: 2
+        (int16 offsets from the row) or 4 (int32 columns); 4 when the FP32
+        product loads 4 groups of a row at once, 0 when it walks them one by
+        one; the bytes one FP32 product streams besides x and y."""
+        b, g, a = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int64(0)
+        self.ctx.check(self._L.fk_sell_layout(self.handle, ctypes.byref(b), ctypes.byref(g), ctypes.byref(a)),
+                       "fk_sell_layout")
+        return int(b.value), int(g.value), int(a.value)
 
     def set_sell_shape(self, block: int, unroll: int = 1) -> None:
         """SELL SpMV launch shape: threads per block and slices in flight per wave."""

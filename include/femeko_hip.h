@@ -385,14 +385,14 @@ int fk_batch_set_format(fk_batch *batch, int32_t format);
 int fk_batch_solver_format(fk_batch *batch, int32_t *format);
 /* phi: nrhs * nv[mesh], right-hand side major */
 int fk_batch_potential(fk_batch *batch, int32_t mesh, double *phi);
-/* Autotuner: times the batched kernels on this batch and keeps the fastest
- * launch. With the CSR copy: lanes per row {4, 8, 16} x block size
- * {128, 256, 512} of the SpMM (the block size also of the update kernels).
+/* Autotuner: times the batched SpMM on this batch (all systems active) and
+ * keeps the fastest launch. With the CSR copy: lanes per row {4, 8, 16} x
+ * block size {128, 256, 512} (the block size also of the update kernels).
  * With the stencil copy: the SpMM's block size {128, 256, 512} x rows per
- * thread and step {1, 2}, then the update kernels' block size
- * {128, 256, 512} (lanes returned unchanged; block = the SpMM's; best_us =
- * the SpMM's time). fk_batch_set_launch sets lanes and the update kernels'
- * block; fk_batch_launch reports the launch in use. */
+ * thread and step {1, 2} (lanes returned unchanged; block = the SpMM's); the
+ * update kernels keep their block (512 unless set). fk_batch_set_launch sets
+ * lanes and the update kernels' block; fk_batch_launch reports the launch in
+ * use. */
 int fk_batch_autotune(fk_batch *batch, int32_t *lanes, int32_t *block, double *best_us);
 int fk_batch_set_launch(fk_batch *batch, int32_t lanes, int32_t block);
 int fk_batch_launch(fk_batch *batch, int32_t *lanes, int32_t *block, int32_t *spmm_block, int32_t *spmm_rows);
@@ -451,16 +451,20 @@ int fk_solver_format(fk_mesh *mesh, int32_t *format, int32_t *stencil_offsets, i
 int fk_sell_info(fk_mesh *mesh, int64_t *nslices, int64_t *shared_slices, int64_t *entries);
 /* SELL copy layout (built on demand): bytes per stored per-row index, 2
  * (int16 offsets from the row, when every column lies within 32767 of its
- * row) or 4 (int32 columns); and the groups of 4 entries the FP32 product
- * loads at once (4), or 0 when it walks the groups one by one (slices wider
- * than 19 entries). Either pointer may be NULL. */
-int fk_sell_layout(fk_mesh *mesh, int32_t *index_bytes, int32_t *batch_groups);
+ * row) or 4 (int32 columns); the groups of 4 entries the FP32 product loads
+ * at once (4), or 0 when it walks the groups one by one (slices wider than
+ * 19 entries); and the bytes of the arrays one FP32 product streams besides
+ * x and y (values with their padding, indices, shared offsets, slice
+ * tables), the format's own compulsory traffic with 8 nv for x and y. Any
+ * pointer may be NULL. */
+int fk_sell_layout(fk_mesh *mesh, int32_t *index_bytes, int32_t *batch_groups, int64_t *f32_array_bytes);
 /* Launch shape of the FP32 SELL SpMV: threads per block (256, 512, 1024;
- * default 256, one 64-row slice per wavefront, blocks swept in dispatch
- * order per XCD) and slices in flight per wavefront (1 or 2; 1024 threads
- * only with 1). The FP64 product runs on 1024-thread blocks of the solver
- * grid. Rows are computed alike in every shape; the p.q partial sums group
- * differently, so solves agree to rounding. */
+ * default 256, blocks swept in dispatch order per XCD) and 64-row slices in
+ * flight per wavefront (1 or 2, default 2; 1024 threads only with 1); a
+ * block covers block/64 x unroll slices and leaves one p.q partial. The
+ * FP64 product runs on 1024-thread blocks of the solver grid. Rows are
+ * computed alike in every shape; the p.q partial sums group differently, so
+ * solves agree to rounding. */
 int fk_set_sell_shape(fk_mesh *mesh, int32_t block, int32_t unroll);
 
 #ifdef __cplusplus

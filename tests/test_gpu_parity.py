@@ -246,7 +246,7 @@ def test_sell_launch_shapes_agree(lin20, prec, shape):
         dm.solve(prec, rtol=1e-10)
         assert np.array_equal(dm.potential(), phi)
     finally:
-        dm.set_sell_shape(1024, 1)
+        dm.set_sell_shape(256, 2)  # the default
 
 
 def test_sell_shape_rejects_unknown():
@@ -268,9 +268,16 @@ def test_sell_layout_statistics(lin20):
         dm.set_format("sell64")
         ns, nsh, ent = dm.sell_info()
         assert ns == (dm.nv + 63) // 64 and nsh == ns
+        # streamed bytes: 4 B per stored value, the shared offsets (at least
+        # one per stored column of a slice) and 20 B of slice tables per slice
+        ib, _ng, ab = dm.sell_layout()
+        assert ab >= 4 * ent + 20 * ns
         dm.set_format("sell64-rowidx")
         ns2, nsh2, ent2 = dm.sell_info()
         assert ns2 == ns and nsh2 == 0
+        # per-row indices: one per stored value
+        ib2, _ng2, ab2 = dm.sell_layout()
+        assert ab2 == (4 + ib2) * ent2 + 20 * ns2 + 8
         counts = np.diff(A.colptr) - 1  # off-diagonal entries per row
         padded = sum(64 * counts[s * 64:(s + 1) * 64].max() for s in range(ns))
         assert ent2 == padded
