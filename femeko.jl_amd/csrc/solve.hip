@@ -2278,7 +2278,10 @@ int fk_copy_probe(fk_context *ctx, int64_t bytes, int32_t reps, double *ms_per_c
         const int64_t n = bytes / 16;
         fk::DevBuf<uint4> src(static_cast<size_t>(n)), dst(static_cast<size_t>(n));
         FK_HIP(hipMemsetAsync(src.get(), 0, n * 16, st));
-        const int64_t grid = std::min<int64_t>((n + 255) / 256, 16 * static_cast<int64_t>(ctx->num_cus));
+        // one 16-byte element per thread over the whole buffer: 6.1 TB/s for
+        // 2 GiB measured, against 4.5-4.6 TB/s for a grid-stride loop over
+        // 16-64 blocks per CU
+        const int64_t grid = std::min<int64_t>((n + 255) / 256, 0x7fffffff);
         hipEvent_t e0, e1;
         FK_HIP(hipEventCreate(&e0));
         FK_HIP(hipEventCreate(&e1));

@@ -404,10 +404,11 @@ int fk_batch_launch(fk_batch *batch, int32_t *lanes, int32_t *block, int32_t *sp
 int fk_set_timing(fk_context *ctx, int32_t enable);
 int fk_spmv_timing(fk_context *ctx, int64_t *launches, double *total_ms);
 /* Streaming-bandwidth probe: copy `bytes` (a multiple of 16 is used) between
- * two device buffers `reps` times with a plain 16-byte-per-lane kernel
- * (k_copy16) and return the average milliseconds per copy. Reads and writes
- * `bytes` each per copy: the practical HBM ceiling of the device, and the
- * kernel of known traffic that calibrates hardware byte counters. */
+ * two device buffers `reps` times with a plain kernel, one 16-byte element
+ * per thread (k_copy16), and return the average milliseconds per copy.
+ * Reads and writes `bytes` each per copy: a reference streaming rate for
+ * data larger than the 256 MB MALL, and the kernel of known traffic that
+ * calibrates hardware byte counters. */
 int fk_copy_probe(fk_context *ctx, int64_t bytes, int32_t reps, double *ms_per_copy);
 /* Duration of the solver's SpMV kernel (FK_PREC_F64: the FP64 kernel,
  * FK_PREC_F32_REFINED: the FP32 one, in the format the solver uses for this
