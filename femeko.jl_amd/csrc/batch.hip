@@ -43,7 +43,7 @@ struct fk_batch {
     // block table of the stencil SpMM (finer: 1024 rows per block)
     int sp_nblocks = 0;
     int sp_block = 256;         // threads per block of the stencil SpMM
-    int sp_rpt = 2;             // rows per thread and step of the stencil SpMM
+    int sp_rpt = 1;             // rows per thread and step of the stencil SpMM (autotuned)
     fk::DevBuf<int32_t> sp_blk_mesh, sp_mesh_b0, sp_mesh_b1;
     fk::DevBuf<int64_t> sp_blk_r0, sp_blk_r1;
     // Dirichlet values and potentials, node major [node][rhs]

@@ -54,6 +54,14 @@ def test_batch_matches_oracle(oracle, precision, inner, fmt):
         b.set_dirichlet([p.mask for p in probs], gs)
         lanes, block, us = b.autotune()
         assert lanes in (4, 8, 16) and block in (128, 256, 512) and us > 0
+        launch = b.launch()
+        if fmt == "auto":
+            # the stencil SpMM's own block and rows per thread; the update
+            # kernels keep 512 threads
+            assert launch["spmm_block"] == block and launch["spmm_rows_per_thread"] in (1, 2)
+            assert launch["block"] == 512
+        else:
+            assert launch["lanes"] == lanes and launch["block"] == block
         # Kuhn boxes: every mesh's pattern is a 7-offset stencil
         assert b.solver_format() == ("stencil" if fmt == "auto" else "csr")
         rep = b.solve(rtol=1e-10, inner_rtol=inner)
