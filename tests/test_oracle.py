@@ -447,6 +447,32 @@ def test_adjoint_gradient_central_differences(oracle):
         assert fd == pytest.approx(grad[e], rel=2e-4, abs=1e-9 * np.abs(grad).max())
 
 
+def test_adjoint_gradient_taylor_remainder(oracle):
+    """Taylor test over all elements at once: along a random direction d,
+    R(eps) = J(rho + eps d) - J(rho) - eps grad.d is O(eps^2) when grad is
+    the gradient, so halving eps divides it by 4; with a wrong gradient (a
+    dropped term, a flipped sign) R is O(eps) and halves instead."""
+    pr = W.adjoint(n=5, seed=17)
+    m = pr.mesh
+    rho = pr.meta["rho"]
+    target = pr.magnet
+    Hs = pr.meta["Hstar"]
+    J0, grad, _phi, _lam = oracle.adjoint_gradient(m, pr.mask, pr.g, rho, target, Hs, rtol=1e-14)
+    d = np.random.default_rng(17).uniform(-1.0, 1.0, m.nt) * 0.1
+
+    def remainder(eps, g):
+        return oracle.objective(m, pr.mask, pr.g, rho + eps * d, target, Hs, rtol=1e-14) - J0 - eps * (g @ d)
+
+    eps = [4e-3, 2e-3, 1e-3]
+    r = [abs(remainder(e, grad)) for e in eps]
+    for a, b in zip(r, r[1:]):
+        assert 3.5 < a / b < 4.5, r
+    # the same test rejects a gradient with a term dropped (half of it)
+    rw = [abs(remainder(e, 0.5 * grad)) for e in eps]
+    for a, b in zip(rw, rw[1:]):
+        assert 1.8 < a / b < 2.2, rw
+
+
 def test_kuhn_counts():
     for n in (1, 2, 7, 20):
         m = kuhn_box_mesh(n)
