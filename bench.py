@@ -237,12 +237,15 @@ def bench_batch(args):
 
 
 def bench_froehlich(args):
-    """n=100 Froehlich magnet, Picard/Newton mu loop on the GPU, FP64 PCG to 1e-10."""
+    """n=100 Froehlich magnet, Picard/Newton mu loop on the GPU, update solves
+    to 1e-10 in FP64 PCG or (--loop-precision f32) FP32 PCG with FP64 residual
+    replacement."""
     from femeko_hip import workloads as W
     from femeko_hip.device import DeviceMesh, context
     pr = W.froehlich()
     dm = DeviceMesh(pr.mesh, context(0))
     dm.set_dirichlet(pr.mask, pr.g)
+    dm.set_loop_precision(args.loop_precision)
     reg = pr.magnet.astype(np.uint8)
     for _ in range(args.warmup):
         out = dm.nonlinear_froehlich(reg, pr.meta["Ms"], pr.meta["a"])
@@ -251,20 +254,40 @@ def bench_froehlich(args):
         out = dm.nonlinear_froehlich(reg, pr.meta["Ms"], pr.meta["a"])
     t = (time.perf_counter() - t0) / args.steps
     log(f"froehlich: {t*1e3:.1f} ms, outer {out['outer']}, CG {out['cg_iterations']}, change {out['change']:.2e}")
-    res = _common(args, "nonlinear magnetostatics DOF/s (Picard/Newton mu loop, FP64)", pr.mesh.nv / t, "DOF/s",
-                  t * 1e3, "f64",
+    prec = "FP64" if args.loop_precision == "f64" else "FP32-refined update solves"
+    res = _common(args, f"nonlinear magnetostatics DOF/s (Picard/Newton mu loop, {prec})", pr.mesh.nv / t, "DOF/s",
+                  t * 1e3, "f64" if args.loop_precision == "f64" else "f32+f64",
                   {"workload": "froehlich100: Kuhn n=100, Froehlich mu_r = 1 + 1.2e6/(500+|H|) in [-0.25,0.25]^3, "
                                "|H0| = 1e4", "nodes": pr.mesh.nv, "tets": pr.mesh.nt})
     res.update({"outer_iterations": out["outer"], "cg_iterations": out["cg_iterations"],
                 "final_mu_change": out["change"]})
+    if args.loop_precision == "f32":
+        _loop_vs_f64(dm, lambda: dm.nonlinear_froehlich(reg, pr.meta["Ms"], pr.meta["a"]), res)
     dm.close()
     return res
 
 
+def _loop_vs_f64(dm, run, res):
+    """After the timed region: the same loop run twice with FP64 update
+    solves (the second timed), and the relative 2-norm difference of the
+    potentials."""
+    phi = dm.potential()
+    dm.set_loop_precision("f64")
+    run()
+    t0 = time.perf_counter()
+    run()
+    t = time.perf_counter() - t0
+    ref = dm.potential()
+    d = float(np.linalg.norm(phi - ref) / np.linalg.norm(ref))
+    log(f"FP64 update solves: {t*1e3:.1f} ms; potential difference {d:.2e}")
+    res.update({"f64_loop_ms": t * 1e3, "speedup_vs_f64_loop": t * 1e3 / res["ms_per_step"],
+                "phi_rel_diff_vs_f64_loop": d})
+
+
 def bench_tabulated(args):
     """n=100 gadolinium cube (B(H) table at 270 K, the reference's material)
-    under 0.5 T, mu(|H|) loop with the tabulated law, FP64 PCG to 1e-10,
-    stop at mu0 max| |H| - |H|_prev | < 1e-4 T (NewtonRaphson.jl:36). Picard
+    under 0.5 T, mu(|H|) loop with the tabulated law, update solves to 1e-10
+    (FP32-refined or FP64, --loop-precision), stop at mu0 max| |H| - |H|_prev | < 1e-4 T (NewtonRaphson.jl:36). Picard
     steps until that change is below 1e-2 T, Newton steps after: undamped
     Newton from the second step (the reference damps it with lineSearch)
     drives a few cube-corner elements, where the field is singular, past the
@@ -278,6 +301,7 @@ def bench_tabulated(args):
     pr = W.tabulated()
     dm = DeviceMesh(pr.mesh, context(0))
     dm.set_dirichlet(pr.mask, pr.g)
+    dm.set_loop_precision(args.loop_precision)
     reg = pr.magnet.astype(np.uint8)
     for _ in range(args.warmup):
         out = dm.nonlinear_spline(reg, spl, spl_dmu, rtol=1e-10, tol_H=1e-4, switch_tol=1e-2)
@@ -286,14 +310,18 @@ def bench_tabulated(args):
         out = dm.nonlinear_spline(reg, spl, spl_dmu, rtol=1e-10, tol_H=1e-4, switch_tol=1e-2)
     t = (time.perf_counter() - t0) / args.steps
     log(f"tabulated: {t*1e3:.1f} ms, outer {out['outer']}, CG {out['cg_iterations']}, change {out['change']:.2e}")
-    res = _common(args, "nonlinear magnetostatics DOF/s (tabulated mu(|H|) Newton loop, FP64)", pr.mesh.nv / t,
-                  "DOF/s", t * 1e3, "f64",
+    prec = "FP64" if args.loop_precision == "f64" else "FP32-refined update solves"
+    res = _common(args, f"nonlinear magnetostatics DOF/s (tabulated mu(|H|) Newton loop, {prec})", pr.mesh.nv / t,
+                  "DOF/s", t * 1e3, "f64" if args.loop_precision == "f64" else "f32+f64",
                   {"workload": "tabulated100: Kuhn n=100, Gd B(H) at 270 K (62-point table, cubic splines of mu "
                                "and dmu) in [-0.25,0.25]^3, Bext = 0.5 T along x; Newton once the change "
                                "is below 1e-2 T", "nodes": pr.mesh.nv,
                    "tets": pr.mesh.nt})
     res.update({"outer_iterations": out["outer"], "cg_iterations": out["cg_iterations"],
                 "final_H_change_T": out["change"]})
+    if args.loop_precision == "f32":
+        _loop_vs_f64(dm, lambda: dm.nonlinear_spline(reg, spl, spl_dmu, rtol=1e-10, tol_H=1e-4, switch_tol=1e-2),
+                     res)
     dm.close()
     return res
 
@@ -421,6 +449,8 @@ def main():
     ap.add_argument("--reorder", default="auto", choices=["auto", "none", "rcm"],
                     help="the library's internal node renumbering (fk_mesh_create_ex)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--loop-precision", default="f32", choices=["f64", "f32"],
+                    help="froehlich/tabulated workloads: precision of the mu loop's update solves")
     ap.add_argument("--batch-update-block", type=int, default=0, choices=[0, 128, 256, 512],
                     help="batch workload: the update kernels' block size instead of the autotuner's")
     ap.add_argument("--workload", default="boxes",

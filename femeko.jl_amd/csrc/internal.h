@@ -134,6 +134,7 @@ struct PcgWork {
     int graph_lanes = 0;
     hipGraphExec_t ru_graph = nullptr;
     int ru_graph_cfg = -1;
+    const double *ru_graph_u = nullptr, *ru_graph_b = nullptr;  // the unknown and RHS it baked in
     void reset_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
@@ -141,6 +142,7 @@ struct PcgWork {
         if (ru_graph) (void)hipGraphExecDestroy(ru_graph);
         ru_graph = nullptr;
         ru_graph_cfg = -1;
+        ru_graph_u = ru_graph_b = nullptr;
     }
     ~PcgWork() { reset_graph(); }
 };
@@ -182,6 +184,8 @@ struct fk_mesh {
     bool have_coef = false;
     fk::DevBuf<double> dmu;           // d mu / d|H| per element (colour order), nonlinear loop
     fk::DevBuf<double> hn_prev;       // |H| of the previous step (colour order), tabulated-law loop
+    int loop_precision = FK_PREC_F64; // update solves of the mu(|H|) loops
+    fk::DevBuf<double> inc64;         // FP64 increment of an FP32-refined update solve
     fk::DevBuf<double> demag_u;       // potential of the last demagnetizing-field call
     bool have_demag_u = false;
     // buffers of the demagnetizing-field call, kept across calls
@@ -315,6 +319,7 @@ inline void values_changed(fk_mesh *m) {
     m->sell32_valid = false;
 }
 CgState run_pcg_f64(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref);
+int solve_f32_increment(fk_mesh *m, double *u, const double *b, double rtol, int64_t max_iter, int64_t *iters);
 struct NeumannResult {
     double multiplier;
     int64_t iters;

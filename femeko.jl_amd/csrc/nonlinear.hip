@@ -374,13 +374,25 @@ static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const La
     for (it = 1; it <= max_outer; ++it) {
         const int newton = dev < switch_tol ? 1 : 0;
         newton_values(m, newton, w.r.get());
-        compute_dinv_f64(m);
-        m->dinv64_valid = true;
-        const CgState s = run_pcg_f64(m, rtol, 1000000, -1.0);
-        cg += s.iters;
-        if (s.status != FK_OK && s.status != FK_ERR_NOT_CONVERGED)
-            fail(s.status, std::string("inner solve: ") + fk_status_string(s.status));
-        phi_axpy_f64(m, w.x.get(), 1.0);
+        if (m->loop_precision == FK_PREC_F32_REFINED) {
+            // FP32 recursion on the step's system, FP64 true residual
+            // w.r - (A + At) du replaced as the recursion falls
+            m->inc64.ensure(nv);
+            int64_t its = 0;
+            const int s32 = solve_f32_increment(m, m->inc64.get(), w.r.get(), rtol, 1000000, &its);
+            cg += its;
+            if (s32 != FK_OK && s32 != FK_ERR_NOT_CONVERGED)
+                fail(s32, std::string("inner solve: ") + fk_status_string(s32));
+            phi_axpy_f64(m, m->inc64.get(), 1.0);
+        } else {
+            compute_dinv_f64(m);
+            m->dinv64_valid = true;
+            const CgState s = run_pcg_f64(m, rtol, 1000000, -1.0);
+            cg += s.iters;
+            if (s.status != FK_OK && s.status != FK_ERR_NOT_CONVERGED)
+                fail(s.status, std::string("inner solve: ") + fk_status_string(s.status));
+            phi_axpy_f64(m, w.x.get(), 1.0);
+        }
         FK_HIP(hipMemsetAsync(bad.get(), 0, 2 * sizeof(int32_t), st));
         if (law.spline)
             k_field_spline<<<nb, kBlock, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(), m->phi.get(),
@@ -634,6 +646,12 @@ void get_coefficient(fk_mesh *m, double *f_host) {
 using fk::guard;
 
 extern "C" {
+
+int fk_set_loop_precision(fk_mesh *m, int32_t precision) {
+    if (!m || (precision != FK_PREC_F64 && precision != FK_PREC_F32_REFINED)) return FK_ERR_INVALID_ARG;
+    m->loop_precision = precision;
+    return FK_OK;
+}
 
 int fk_nonlinear_froehlich(fk_mesh *m, const uint8_t *region, double Ms, double a, double rtol, double tol_mu,
                            int32_t max_outer, double switch_tol, int32_t *outer, int64_t *cg_iterations,

@@ -615,6 +615,7 @@ template <typename IDX, int NG>
 __global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const Sell<double, IDX> A,
                                                               const double *__restrict__ phi,
                                                               const uint8_t *__restrict__ mask,
+                                                              const double *__restrict__ b,
                                                               double *__restrict__ r, double *__restrict__ part,
                                                               int64_t nslices, int64_t chunk_slices, int nblocks,
                                                               const RuState *__restrict__ gate) {
@@ -630,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_neg_residual_sell(const Sell<double,
         const double xi = row < A.n ? phi[row] : 0.0;
         const double y = sell_apply<NG, true>(A, phi, s, lane, row, xi);
         if (row < A.n) {
-            const double v = mask[row] ? 0.0 : -y;
+            const double v = mask[row] ? 0.0 : (b ? b[row] - y : -y);
             r[row] = v;
             acc += v * v;
         }
@@ -762,6 +763,7 @@ template <int K, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_neg_residual_stencil(const Stencil<double> A,
                                                                 const double *__restrict__ phi,
                                                                 const uint8_t *__restrict__ mask,
+                                                                const double *__restrict__ b,
                                                                 double *__restrict__ r, double *__restrict__ part,
                                                                 int64_t chunk, int nblocks,
                                                                 const RuState *__restrict__ gate) {
@@ -774,7 +776,7 @@ __global__ __launch_bounds__(BLOCK) void k_neg_residual_stencil(const Stencil<do
         const int64_t i = row0 + threadIdx.x;
         if (i >= R.end) break;
         const double y = stencil_apply<K, true>(A, phi, i, phi[i]);
-        const double v = mask[i] ? 0.0 : -y;
+        const double v = mask[i] ? 0.0 : (b ? b[i] - y : -y);
         r[i] = v;
         acc += v * v;
     }
@@ -849,7 +851,8 @@ __global__ __launch_bounds__(kBlock) void k_neg_residual(const int32_t *__restri
                                                          const int32_t *__restrict__ col,
                                                          const double *__restrict__ val,
                                                          const double *__restrict__ phi,
-                                                         const uint8_t *__restrict__ mask, double *__restrict__ r,
+                                                         const uint8_t *__restrict__ mask,
+                                                         const double *__restrict__ b, double *__restrict__ r,
                                                          double *__restrict__ part, int64_t n, int64_t chunk,
                                                          int nblocks, const RuState *__restrict__ gate) {
     if (gate && gate->act != 1) return;
@@ -865,7 +868,7 @@ __global__ __launch_bounds__(kBlock) void k_neg_residual(const int32_t *__restri
         const bool valid = row < r1;
         const double s = csr_row<double, LPR>(rowptr, col, val, phi, row, valid, lane);
         if (valid && lane == 0) {
-            const double v = mask[row] ? 0.0 : -s;
+            const double v = mask[row] ? 0.0 : (b ? b[row] - s : -s);
             r[row] = v;
             acc += v * v;
         }
@@ -1457,10 +1460,11 @@ static void launch_sell(fk_mesh *m, PcgWork<T> &w, const IDX *col, int parity) {
 }
 
 template <typename IDX>
-static void launch_neg_residual_sell(fk_mesh *m, const IDX *col, const double *phi, double *r, const RuState *gate) {
+static void launch_neg_residual_sell(fk_mesh *m, const IDX *col, const double *phi, const double *b, double *r,
+                                     const RuState *gate) {
     const Sell<double, IDX> A = sell_view<double, IDX>(m, col);
     k_neg_residual_sell<IDX, 0><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(
-        A, phi, m->mask.get(), r, m->part_misc.get(), m->nslices, m->sell_chunk, m->nblocks, gate);
+        A, phi, m->mask.get(), b, r, m->part_misc.get(), m->nslices, m->sell_chunk, m->nblocks, gate);
 }
 
 template <typename T, typename IDX>
@@ -1673,8 +1677,9 @@ static CgState run_pcg(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref)
     return out;
 }
 
-static double neg_residual(fk_mesh *m, const double *phi, double *r);
-static void enqueue_neg_residual(fk_mesh *m, const double *phi, double *r, const RuState *gate);
+static double neg_residual(fk_mesh *m, const double *phi, double *r, const double *b = nullptr);
+static void enqueue_neg_residual(fk_mesh *m, const double *phi, double *r, const RuState *gate,
+                                 const double *b = nullptr);
 
 constexpr int kMaxRestarts = 32;  // direction restarts before the FP32 solve gives up
 
@@ -1804,14 +1809,15 @@ __global__ void k_ru_resume(CgState *__restrict__ cg, RuState *__restrict__ ru) 
 }
 
 // G iterations (G even: the chunk ends on parity 0) and the replacement step
-static void enqueue_ru_chunk(fk_mesh *m, PcgWork<float> &w, int G, bool timed) {
+// of the system A u = b (b null: 0) on the free rows
+static void enqueue_ru_chunk(fk_mesh *m, PcgWork<float> &w, int G, bool timed, double *u, const double *b) {
     hipStream_t st = m->ctx->stream;
     const int nb = m->nblocks;
     for (int k = 0; k < G; ++k) enqueue_iteration<float>(m, w, k & 1, timed);
     RuState *ru = m->ru.get();
     k_ru_check<<<1, 64, 0, st>>>(w.state.get(), ru);
-    k_ru_fold<float><<<nb, kBlock, 0, st>>>(m->phi.get(), w.x.get(), ru, m->nv);
-    enqueue_neg_residual(m, m->phi.get(), m->r64.get(), ru);
+    k_ru_fold<float><<<nb, kBlock, 0, st>>>(u, w.x.get(), ru, m->nv);
+    enqueue_neg_residual(m, u, m->r64.get(), ru, b);
     k_ru_decide<<<1, kBlock, 0, st>>>(m->part_misc.get(), nb, w.state.get(), ru);
     k_ru_replace<float><<<nb, kBlock, 0, st>>>(w.r.get(), w.p.get(), m->r64.get(), w.dinv.get(), w.part_rz.get(), ru,
                                                m->nv, m->chunk, nb);
@@ -1820,21 +1826,22 @@ static void enqueue_ru_chunk(fk_mesh *m, PcgWork<float> &w, int G, bool timed) {
 }
 
 // FP32 Jacobi-PCG whose residual is replaced by the FP64 true residual
-// -mask(A64 phi), phi accumulating the FP32 increments in FP64 ("reliable
-// updates"). A replacement happens at the end of the iteration chunk in which
-// the FP32 recursive residual has fallen by the factor `delta` since the last
-// replacement, or once the recursion stops (target reached, breakdown); it
-// keeps the search direction, so the Krylov space is not restarted.
-// Convergence is decided on the FP64 true residual: ||r|| < rtol ||b||.
-// Returns (iterations, replacements, final ||r||^2).
-static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double bb, double delta, int64_t *iters,
-                             int32_t *nrep, double *rr_out) {
+// mask(b - A64 u) (u = phi and b = 0 for the Dirichlet problem; b the
+// right-hand side of an increment otherwise), u accumulating the FP32
+// increments in FP64 ("reliable updates"). A replacement happens at the end
+// of the iteration chunk in which the FP32 recursive residual has fallen by
+// the factor `delta` since the last replacement, or once the recursion stops
+// (target reached, breakdown); it keeps the search direction, so the Krylov
+// space is not restarted. Convergence is decided on the FP64 true residual:
+// ||r|| < rtol ||b||. Returns (iterations, replacements, final ||r||^2).
+static int solve_f32_replace(fk_mesh *m, double *u, const double *b, double rtol, int64_t max_iter, double bb,
+                             double delta, int64_t *iters, int32_t *nrep, double *rr_out) {
     hipStream_t st = m->ctx->stream;
     const int64_t nv = m->nv;
     PcgWork<float> &w = m->w32;
     const int nb = m->nblocks;
     const double thresh = rtol * rtol * bb;
-    double rr = neg_residual(m, m->phi.get(), m->r64.get());
+    double rr = neg_residual(m, u, m->r64.get(), b);
     *iters = 0;
     *nrep = 0;
     *rr_out = rr;
@@ -1877,7 +1884,7 @@ static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double b
         // synchronous chunks, events around every SpMV
         int64_t before = 0;
         while (true) {
-            enqueue_ru_chunk(m, w, G, true);
+            enqueue_ru_chunk(m, w, G, true, u, b);
             FK_HIP(hipMemcpyAsync(hr, m->ru.get(), sizeof(RuState), hipMemcpyDeviceToHost, st));
             FK_HIP(hipMemcpyAsync(hs, w.state.get(), sizeof(CgState), hipMemcpyDeviceToHost, st));
             FK_HIP(hipStreamSynchronize(st));
@@ -1890,17 +1897,19 @@ static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double b
         hipGraphExec_t *gx = nullptr;
         if (m->ctx->graphs) {
             const int cfg = solver_config(m) * 64 + G;
-            if (!w.ru_graph || w.ru_graph_cfg != cfg) {
+            if (!w.ru_graph || w.ru_graph_cfg != cfg || w.ru_graph_u != u || w.ru_graph_b != b) {
                 if (w.ru_graph) (void)hipGraphExecDestroy(w.ru_graph);
                 w.ru_graph = nullptr;
                 hipGraph_t graph;
                 FK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                enqueue_ru_chunk(m, w, G, false);
+                enqueue_ru_chunk(m, w, G, false, u, b);
                 FK_HIP(hipStreamEndCapture(st, &graph));
                 hipError_t e = hipGraphInstantiate(&w.ru_graph, graph, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(graph);
                 FK_HIP(e);
                 w.ru_graph_cfg = cfg;
+                w.ru_graph_u = u;
+                w.ru_graph_b = b;
             }
             gx = &w.ru_graph;
         }
@@ -1908,7 +1917,7 @@ static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double b
             if (gx)
                 FK_HIP(hipGraphLaunch(*gx, st));
             else
-                enqueue_ru_chunk(m, w, G, false);
+                enqueue_ru_chunk(m, w, G, false, u, b);
         };
         // one chunk in flight ahead of the state copy being examined
         hipEvent_t ev[2];
@@ -1951,45 +1960,46 @@ static int solve_f32_replace(fk_mesh *m, double rtol, int64_t max_iter, double b
 }
 
 template <int LPR>
-static void launch_neg_residual(fk_mesh *m, const double *phi, double *r, double *part, const RuState *gate) {
+static void launch_neg_residual(fk_mesh *m, const double *phi, const double *b, double *r, double *part,
+                                const RuState *gate) {
     k_neg_residual<LPR><<<m->nblocks, kBlock, 0, m->ctx->stream>>>(m->rowptr.get(), m->colidx.get(),
-                                                                   m->val64.get(), phi, m->mask.get(), r, part,
+                                                                   m->val64.get(), phi, m->mask.get(), b, r, part,
                                                                    m->nv, m->chunk, m->nblocks, gate);
 }
 
-// enqueue r = -mask(A64 phi) and its block partials of r.r into part_misc
-// (no host sync); gate as in the kernels
-static void enqueue_neg_residual(fk_mesh *m, const double *phi, double *r, const RuState *gate) {
+// enqueue r = mask(b - A64 phi) (b null: 0) and its block partials of r.r
+// into part_misc (no host sync); gate as in the kernels
+static void enqueue_neg_residual(fk_mesh *m, const double *phi, double *r, const RuState *gate, const double *b) {
     hipStream_t st = m->ctx->stream;
     if (m->kind == FK_SPMV_STENCIL) {
         StencilK<>::run(m->st_K, [&](auto kc) {
             // 512-thread blocks: 8 waves per SIMD, as the FP64 solver SpMV
             // (measured 97 us against 133 us with 256-thread blocks at n=160)
             k_neg_residual_stencil<decltype(kc)::value, 512><<<m->nblocks, 512, 0, st>>>(
-                stencil_view<double>(m, m->st_val64.get(), m->sell_diag64.get()), phi, m->mask.get(), r,
+                stencil_view<double>(m, m->st_val64.get(), m->sell_diag64.get()), phi, m->mask.get(), b, r,
                 m->part_misc.get(), m->chunk, m->nblocks, gate);
         });
     } else if (m->format != FK_SPMV_CSR) {
         if (m->sell_short)
-            launch_neg_residual_sell<int16_t>(m, m->sell_dcol.get(), phi, r, gate);
+            launch_neg_residual_sell<int16_t>(m, m->sell_dcol.get(), phi, b, r, gate);
         else
-            launch_neg_residual_sell<int32_t>(m, m->sell_col.get(), phi, r, gate);
+            launch_neg_residual_sell<int32_t>(m, m->sell_col.get(), phi, b, r, gate);
     } else switch (m->lanes) {
-        case 4: launch_neg_residual<4>(m, phi, r, m->part_misc.get(), gate); break;
-        case 8: launch_neg_residual<8>(m, phi, r, m->part_misc.get(), gate); break;
-        case 32: launch_neg_residual<32>(m, phi, r, m->part_misc.get(), gate); break;
-        case 64: launch_neg_residual<64>(m, phi, r, m->part_misc.get(), gate); break;
-        default: launch_neg_residual<16>(m, phi, r, m->part_misc.get(), gate); break;
+        case 4: launch_neg_residual<4>(m, phi, b, r, m->part_misc.get(), gate); break;
+        case 8: launch_neg_residual<8>(m, phi, b, r, m->part_misc.get(), gate); break;
+        case 32: launch_neg_residual<32>(m, phi, b, r, m->part_misc.get(), gate); break;
+        case 64: launch_neg_residual<64>(m, phi, b, r, m->part_misc.get(), gate); break;
+        default: launch_neg_residual<16>(m, phi, b, r, m->part_misc.get(), gate); break;
     }
     FK_HIP(hipGetLastError());
 }
 
-// r = -mask(A64 phi); returns ||r||^2 (host sync)
-static double neg_residual(fk_mesh *m, const double *phi, double *r) {
+// r = mask(b - A64 phi) (b null: 0); returns ||r||^2 (host sync)
+static double neg_residual(fk_mesh *m, const double *phi, double *r, const double *b) {
     hipStream_t st = m->ctx->stream;
     m->part_misc.ensure(m->nblocks);
     if (m->format != FK_SPMV_CSR) sync_solver_values<double>(m);
-    enqueue_neg_residual(m, phi, r, nullptr);
+    enqueue_neg_residual(m, phi, r, nullptr, b);
     FK_HIP(hipMemcpyAsync(m->hpart.ptr, m->part_misc.get(), m->nblocks * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
     double s = 0.0;
@@ -1998,6 +2008,28 @@ static double neg_residual(fk_mesh *m, const double *phi, double *r) {
 }
 
 // ---- entry points for the other translation units (nonlinear loop, adjoint)
+// FP32-refined solve of A u = b on the free rows with u = 0 on the fixed
+// nodes (u, b FP64 on the device, nv values; u is overwritten), to
+// ||mask(b - A u)|| < rtol ||mask b||: the increment of a Picard or Newton
+// step. Returns the status and the FP32 iterations.
+int solve_f32_increment(fk_mesh *m, double *u, const double *b, double rtol, int64_t max_iter, int64_t *iters) {
+    hipStream_t st = m->ctx->stream;
+    m->r64.ensure(m->nv);
+    ensure_work<float>(m);
+    if (m->format == FK_SPMV_CSR) make_val32(m);
+    if (!m->dinv32_valid) {
+        compute_dinv<float>(m);
+        m->dinv32_valid = true;
+    }
+    FK_HIP(hipMemsetAsync(u, 0, m->nv * sizeof(double), st));
+    const double bb = neg_residual(m, u, m->r64.get(), b);  // ||mask b||^2
+    *iters = 0;
+    if (!(bb > 0.0)) return std::isfinite(bb) ? FK_OK : FK_ERR_NONFINITE;
+    int32_t nrep = 0;
+    double rr = 0.0;
+    return solve_f32_replace(m, u, b, rtol, max_iter, bb, 0.95, iters, &nrep, &rr);
+}
+
 void ensure_work_f64(fk_mesh *m) { ensure_work<double>(m); }
 void compute_dinv_f64(fk_mesh *m) { compute_dinv<double>(m); }
 void compute_dinv_f32(fk_mesh *m) { compute_dinv<float>(m); }
@@ -2097,7 +2129,7 @@ void solve(fk_mesh *m, int precision, double rtol, int64_t max_iter, double inne
         // inner_rtol is the reliable-update trigger: replace once the FP32
         // residual fell by this factor (values outside (0,1) mean 0.95)
         const double delta = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 0.95;
-        status = solve_f32_replace(m, rtol, max_iter, bb, delta, &total, &nout, &rr);
+        status = solve_f32_replace(m, m->phi.get(), nullptr, rtol, max_iter, bb, delta, &total, &nout, &rr);
         rel = std::sqrt(rr / bb);
     } else {
         // restarted refinement: each FP32 PCG runs from scratch to inner_rtol

@@ -17,14 +17,18 @@ import linesearch_cases as LS
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("loop_precision", ["f64", "f32"])
 @pytest.mark.parametrize("n,seed", [(8, 12), (12, 100)])
-def test_froehlich_loop_matches_oracle(oracle, n, seed):
+def test_froehlich_loop_matches_oracle(oracle, n, seed, loop_precision):
+    # f32: the update solves run FP32-refined (FP64 true residual of each
+    # step's own system) to the same rtol; same bars
     pr = W.froehlich(n=n, seed=seed)
     Ms, a = pr.meta["Ms"], pr.meta["a"]
     ref = oracle.newton_froehlich(pr.mesh, pr.mask, pr.g, pr.magnet, Ms, a)
     dm = DeviceMesh(pr.mesh, context(0))
     try:
         dm.set_dirichlet(pr.mask, pr.g)
+        dm.set_loop_precision(loop_precision)
         out = dm.nonlinear_froehlich(pr.magnet.astype(np.uint8), Ms, a, rtol=1e-10, tol_mu=1e-6, max_outer=20)
         assert out["outer"] == ref["iters"]
         assert out["change"] < 1e-6
