@@ -95,30 +95,71 @@ __global__ __launch_bounds__(256) void k_assemble(const int4 *__restrict__ tet, 
 // Row-gather assembly: thread = row i (one wavefront = one slice of 64
 // rows, kGatherRows rows per block). The incidences of slice s are stored
 // transposed from sptr[s] (entry k of lane l at sptr[s] + 64 k + l, width[s]
-// entries per lane, kNoInc padding) so each load is coalesced: w = element <<
-// 2 | local index of the row's node, elements in caller order, ascending per
-// node; off packs the positions (0..255, relative to rowptr[i]) of the
-// element's 4 nodes in row i. Elements are read in caller order, whose
-// spatial locality (neighbouring rows share most elements) keeps them in L2.
-// The block's value segment [rowptr[i0], rowptr[i0 + rows]) is accumulated
-// in LDS (dynamic size) and written once.
+// entries per lane, padding .x = kNoInc) so each load is coalesced; an
+// incidence is one 8-byte word pair: .x = element << 2 | local index of the
+// row's node, elements in caller order, ascending per node; .y packs the
+// positions (0..255, relative to rowptr[i]) of the element's 4 nodes in row
+// i. Elements are read in caller order, whose spatial locality (neighbouring
+// rows share most elements) keeps them in L2. The block's value segment
+// [rowptr[i0], rowptr[i0 + rows]) is accumulated in LDS (dynamic size) and
+// written once.
+//
+// Node coordinates come from 32-byte records (x, y, z, w): two loads per
+// node instead of three, and the record of the row's own node (local index
+// la of every incidence) is loaded once per row, so an incidence loads three
+// records. The cost of this kernel is the number of per-lane addresses its
+// gathers issue more than their bytes: at n=160 (24.6 M elements) the
+// separate X, Y, Z arrays took 1.21 ms, records 1.05 ms, records without the
+// own node 1.01 ms, with bit-identical values. Staging the coordinates of the
+// block's columns in LDS instead (no per-incidence coordinate gathers, 32
+// bytes of LDS per value) took 1.27-1.88 ms: the LDS then holds about 5
+// waves per CU.
 //
 // The incidences are taken kGatherUnroll at a time with every load
 // unconditional (a padding entry or one past the width reads element 0 and
 // adds nothing): the loads of the group (incidence words, then element nodes
 // and coefficients, then coordinates) are in flight together, and no branch
 // sits between a load and its use (a branch there makes the compiler wait
-// for all outstanding loads). Measured at n=160 (24.6 M elements): 1.28 ms
-// with 8 per group, 1.32 ms with 4, 1.52 ms for one incidence per step with
-// a branch on padding. The sums run in the same (element) order either way.
+// for all outstanding loads). With records and the own node, 6 per group
+// (166 VGPRs) measured 1.01 ms, 8 1.02 ms, 4 1.13 ms. The sums run in the
+// same (element) order either way.
 constexpr uint32_t kNoInc = 0xffffffffu;
-constexpr int kGatherUnroll = 8;
+constexpr int kGatherUnroll = 6;
 constexpr int kNewtonUnroll = 4;
+
+// The 12 coordinates (and the 4 w fields when W) of an element's nodes in
+// local order, node la being the row's own node whose record the caller
+// holds: three record loads.
+template <bool W>
+__device__ __forceinline__ void element_records(const double4 *__restrict__ P, int4 nd, int la, const double4 &own,
+                                                double c[12], double w[4]) {
+    const double4 L0 = P[la == 0 ? nd.y : nd.x];
+    const double4 L1 = P[la <= 1 ? nd.z : nd.y];
+    const double4 L2 = P[la <= 2 ? nd.w : nd.z];
+    // node b < la is other node b, node b > la other node b - 1
+    c[0] = la == 0 ? own.x : L0.x;
+    c[1] = la == 0 ? own.y : L0.y;
+    c[2] = la == 0 ? own.z : L0.z;
+    c[3] = la == 1 ? own.x : la == 0 ? L0.x : L1.x;
+    c[4] = la == 1 ? own.y : la == 0 ? L0.y : L1.y;
+    c[5] = la == 1 ? own.z : la == 0 ? L0.z : L1.z;
+    c[6] = la == 2 ? own.x : la <= 1 ? L1.x : L2.x;
+    c[7] = la == 2 ? own.y : la <= 1 ? L1.y : L2.y;
+    c[8] = la == 2 ? own.z : la <= 1 ? L1.z : L2.z;
+    c[9] = la == 3 ? own.x : L2.x;
+    c[10] = la == 3 ? own.y : L2.y;
+    c[11] = la == 3 ? own.z : L2.z;
+    if constexpr (W) {
+        w[0] = la == 0 ? own.w : L0.w;
+        w[1] = la == 1 ? own.w : la == 0 ? L0.w : L1.w;
+        w[2] = la == 2 ? own.w : la <= 1 ? L1.w : L2.w;
+        w[3] = la == 3 ? own.w : L2.w;
+    }
+}
 
 __global__ __launch_bounds__(kGatherRows) void k_assemble_gather(
     const int32_t *__restrict__ rowptr, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
-    const uint32_t *__restrict__ inc, const uint32_t *__restrict__ off, const int4 *__restrict__ tet,
-    const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+    const uint2 *__restrict__ inc, const int4 *__restrict__ tet, const double4 *__restrict__ P,
     const double *__restrict__ coef, int64_t nv, double *__restrict__ val) {
     extern __shared__ double acc[];
     constexpr int UG = kGatherUnroll;
@@ -136,30 +177,27 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_gather(
     for (int32_t k = threadIdx.x; k < seg; k += kGatherRows) acc[k] = 0.0;
     __syncthreads();
     double *a = acc + (rowptr[min(i, nv)] - base);
+    const double4 own = P[min(i, nv - 1)];
     const int64_t sb = sptr[sc] + lane;
     const int W = s < ns ? width[sc] : 0;
     constexpr int U[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
     for (int k = 0; k < W; k += UG) {
-        uint32_t w[UG], o[UG];
+        uint2 w[UG];
 #pragma unroll
-        for (int u = 0; u < UG; ++u) {
-            const int64_t q = sb + 64 * static_cast<int64_t>(min(k + u, W - 1));
-            w[u] = inc[q];
-            o[u] = off[q];
-        }
+        for (int u = 0; u < UG; ++u) w[u] = inc[sb + 64 * static_cast<int64_t>(min(k + u, W - 1))];
         bool use[UG];
         int4 nd[UG];
         double cf[UG];
 #pragma unroll
         for (int u = 0; u < UG; ++u) {
-            use[u] = k + u < W && w[u] != kNoInc;
-            const int64_t e = use[u] ? static_cast<int64_t>(w[u] >> 2) : 0;
+            use[u] = k + u < W && w[u].x != kNoInc;
+            const int64_t e = use[u] ? static_cast<int64_t>(w[u].x >> 2) : 0;
             nd[u] = tet[e];
             cf[u] = coef[e];
         }
         double c[UG][12];
 #pragma unroll
-        for (int u = 0; u < UG; ++u) element_coords(X, Y, Z, nd[u], c[u]);
+        for (int u = 0; u < UG; ++u) element_records<false>(P, nd[u], static_cast<int>(w[u].x & 3u), own, c[u], nullptr);
 #pragma unroll
         for (int u = 0; u < UG; ++u) {
             const Grad G = gradients_of(c[u]);
@@ -168,12 +206,12 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_gather(
             // row i and row j then add the very same value for (i,j), (j,i)
             double kv[10];
             local_values(G, sc6, kv);
-            const int la = static_cast<int>(w[u] & 3u);
+            const int la = static_cast<int>(w[u].x & 3u);
             if (use[u]) {
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     const double v = la == 0 ? kv[U[0][b]] : la == 1 ? kv[U[1][b]] : la == 2 ? kv[U[2][b]] : kv[U[3][b]];
-                    a[(o[u] >> (8 * b)) & 255u] += v;
+                    a[(w[u].y >> (8 * b)) & 255u] += v;
                 }
             }
         }
@@ -183,9 +221,10 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_gather(
 }
 
 // Newton-step assembly of the nonlinear loop in row-gather form (the
-// incidence lists of k_assemble_gather, kNewtonUnroll incidences per step in
-// the same way): row i adds, over its elements e (local index la),
-// K_e(mu)[la][b] = VE mu_e g_la.g_b and, on a Newton step, the tangent
+// incidence lists and node records of k_assemble_gather, kNewtonUnroll
+// incidences per step in the same way; the records' w holds phi): row i
+// adds, over its elements e (local index la), K_e(mu)[la][b] =
+// VE mu_e g_la.g_b and, on a Newton step, the tangent
 // VE mu'_e/|H_e| (H_e.g_la)(H_e.g_b) of src/FEM3D.jl:79-119, with
 // H_e = -grad(phi); and sums the element residual -sum_b K_e(mu)[la][b] phi_b
 // into r[i]. Values and residual are written once (no zero fill); g_la.g_b
@@ -194,10 +233,9 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_gather(
 // are in caller element order.
 __global__ __launch_bounds__(kGatherRows) void k_assemble_newton_gather(
     const int32_t *__restrict__ rowptr, const int64_t *__restrict__ sptr, const int32_t *__restrict__ width,
-    const uint32_t *__restrict__ inc, const uint32_t *__restrict__ off, const int4 *__restrict__ tet,
-    const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
-    const double *__restrict__ mu, const double *__restrict__ dmu, const double *__restrict__ phi, int newton,
-    int64_t nv, double *__restrict__ val, double *__restrict__ r) {
+    const uint2 *__restrict__ inc, const int4 *__restrict__ tet, const double4 *__restrict__ P,
+    const double *__restrict__ mu, const double *__restrict__ dmu, int newton, int64_t nv,
+    double *__restrict__ val, double *__restrict__ r) {
     extern __shared__ double acc[];
     constexpr int UN = kNewtonUnroll;
     const int64_t i0 = static_cast<int64_t>(logical_block(static_cast<int>(gridDim.x))) * kGatherRows;
@@ -213,42 +251,33 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_newton_gather(
     __syncthreads();
     double res = 0.0;
     double *a = acc + (rowptr[min(i, nv)] - base);
+    const double4 own = P[min(i, nv - 1)];
     const int64_t sb = sptr[sc] + lane;
     const int W = s < ns ? width[sc] : 0;
     for (int k = 0; k < W; k += UN) {
-        uint32_t w[UN], o[UN];
+        uint2 w[UN];
 #pragma unroll
-        for (int u = 0; u < UN; ++u) {
-            const int64_t q = sb + 64 * static_cast<int64_t>(min(k + u, W - 1));
-            w[u] = inc[q];
-            o[u] = off[q];
-        }
+        for (int u = 0; u < UN; ++u) w[u] = inc[sb + 64 * static_cast<int64_t>(min(k + u, W - 1))];
         bool use[UN];
         int4 nd[UN];
         double m1[UN], d1[UN];
 #pragma unroll
         for (int u = 0; u < UN; ++u) {
-            use[u] = k + u < W && w[u] != kNoInc;
-            const int64_t e = use[u] ? static_cast<int64_t>(w[u] >> 2) : 0;
+            use[u] = k + u < W && w[u].x != kNoInc;
+            const int64_t e = use[u] ? static_cast<int64_t>(w[u].x >> 2) : 0;
             nd[u] = tet[e];
             m1[u] = mu[e];
             d1[u] = newton ? dmu[e] : 0.0;
         }
         double c[UN][12], ph[UN][4];
 #pragma unroll
-        for (int u = 0; u < UN; ++u) {
-            element_coords(X, Y, Z, nd[u], c[u]);
-            ph[u][0] = phi[nd[u].x];
-            ph[u][1] = phi[nd[u].y];
-            ph[u][2] = phi[nd[u].z];
-            ph[u][3] = phi[nd[u].w];
-        }
+        for (int u = 0; u < UN; ++u) element_records<true>(P, nd[u], static_cast<int>(w[u].x & 3u), own, c[u], ph[u]);
 #pragma unroll
         for (int u = 0; u < UN; ++u) {
             const Grad G = gradients_of(c[u]);
             const double vol = fabs(G.det) * (1.0 / 6.0);
             const double wk = vol * m1[u];
-            const int la = static_cast<int>(w[u] & 3u);
+            const int la = static_cast<int>(w[u].x & 3u);
             double gl[3];
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
@@ -275,7 +304,7 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_newton_gather(
             if (use[u]) {
                 res -= ru;
 #pragma unroll
-                for (int b = 0; b < 4; ++b) a[(o[u] >> (8 * b)) & 255u] += kv[b];
+                for (int b = 0; b < 4; ++b) a[(w[u].y >> (8 * b)) & 255u] += kv[b];
             }
         }
     }
@@ -290,6 +319,19 @@ __global__ __launch_bounds__(kGatherRows) void k_assemble_newton_gather(
 static int64_t gather_blocks(int64_t nv) {
     const int64_t nb = (nv + kGatherRows - 1) / kGatherRows;
     return nb < 8 ? nb : (nb + 7) / 8 * 8;
+}
+
+// P[i] = (X[i], Y[i], Z[i], 0): the row-gather kernels' node records
+__global__ void k_pack_xyzw(const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+                            int64_t nv, double4 *__restrict__ P) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nv) P[i] = make_double4(X[i], Y[i], Z[i], 0.0);
+}
+
+// P[i].w = phi[i]
+__global__ void k_records_w(const double *__restrict__ phi, int64_t nv, double4 *__restrict__ P) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nv) P[i].w = phi[i];
 }
 
 // coef_nat[perm[e]] = coef[e] (colour order -> caller order)
@@ -435,7 +477,7 @@ void build_gather(fk_mesh *m) {
         sptr[s + 1] = sptr[s] + 64 * w;
     }
     const int64_t len = sptr[ns];
-    std::vector<uint32_t> inc(static_cast<size_t>(len), kNoInc), off(static_cast<size_t>(len), 0);
+    std::vector<uint2> inc(static_cast<size_t>(len), make_uint2(kNoInc, 0u));
     const int nth = static_cast<int>(std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())));
     std::vector<std::thread> th;
     for (int q = 0; q < nth; ++q)
@@ -454,8 +496,7 @@ void build_gather(fk_mesh *m) {
                             o |= static_cast<uint32_t>(f - b) << (8 * c);
                         }
                         const int64_t pos = sptr[s] + 64 * (k - ptr[r]) + lane;
-                        inc[pos] = w;
-                        off[pos] = o;
+                        inc[pos] = make_uint2(w, o);
                     }
                 }
         });
@@ -463,11 +504,12 @@ void build_gather(fk_mesh *m) {
     m->inc_sptr.alloc(ns + 1);
     m->inc_width.alloc(ns);
     m->inc.alloc(len);
-    m->inc_off.alloc(len);
+    m->xyzw.alloc(nv);
     FK_HIP(hipMemcpyAsync(m->inc_sptr.get(), sptr.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
     FK_HIP(hipMemcpyAsync(m->inc_width.get(), width.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    FK_HIP(hipMemcpyAsync(m->inc.get(), inc.data(), len * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    FK_HIP(hipMemcpyAsync(m->inc_off.get(), off.data(), len * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    FK_HIP(hipMemcpyAsync(m->inc.get(), inc.data(), len * sizeof(uint2), hipMemcpyHostToDevice, st));
+    k_pack_xyzw<<<grid_for(nv, 256), 256, 0, st>>>(m->X.get(), m->Y.get(), m->Z.get(), nv, m->xyzw.get());
+    FK_HIP(hipGetLastError());
     FK_HIP(hipStreamSynchronize(st));
     m->gather_lds = static_cast<int32_t>(seg_max);
     m->gather_ok = true;
@@ -488,12 +530,12 @@ bool assemble_newton_gather(fk_mesh *m, const double *mu, const double *dmu, con
     if (newton)
         k_coef_to_caller<<<grid_for(m->nt, 256), 256, 0, st>>>(dmu, m->perm.get(), m->nt, m->dmu_nat.get());
     m->coef_nat_valid = false;  // the loop rewrites coef right after this assembly
+    k_records_w<<<grid_for(m->nv, 256), 256, 0, st>>>(phi, m->nv, m->xyzw.get());
     const int64_t nb = gather_blocks(m->nv);
     k_assemble_newton_gather<<<static_cast<unsigned>(nb), kGatherRows,
                                static_cast<size_t>(std::max(m->gather_lds, 1)) * sizeof(double), st>>>(
-        m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->inc_off.get(), m->tet_nat.get(),
-        m->X.get(), m->Y.get(), m->Z.get(), m->coef_nat.get(), m->dmu_nat.get(), phi, newton, m->nv,
-        m->val64.get(), r);
+        m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->tet_nat.get(), m->xyzw.get(),
+        m->coef_nat.get(), m->dmu_nat.get(), newton, m->nv, m->val64.get(), r);
     FK_HIP(hipGetLastError());
     return true;
 }
@@ -527,8 +569,8 @@ void assemble(fk_mesh *m, int method) {
             const int64_t nb = gather_blocks(m->nv);
             k_assemble_gather<<<static_cast<unsigned>(nb), kGatherRows,
                                 static_cast<size_t>(std::max(m->gather_lds, 1)) * sizeof(double), st>>>(
-                m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->inc_off.get(),
-                m->tet_nat.get(), m->X.get(), m->Y.get(), m->Z.get(), coef, m->nv, m->val64.get());
+                m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->tet_nat.get(),
+                m->xyzw.get(), coef, m->nv, m->val64.get());
             FK_HIP(hipGetLastError());
             values_changed(m);
             return;

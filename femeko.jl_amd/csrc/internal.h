@@ -248,9 +248,10 @@ struct fk_mesh {
     // positions in the row; connectivity and coefficient in caller order
     bool gather_built = false, gather_ok = false;
     int32_t gather_lds = 0;           // largest block value segment (doubles)
+    fk::DevBuf<double4> xyzw;         // row-gather node records (x, y, z, w = phi for the Newton step)
     fk::DevBuf<int64_t> inc_sptr;
     fk::DevBuf<int32_t> inc_width;
-    fk::DevBuf<uint32_t> inc, inc_off;
+    fk::DevBuf<uint2> inc;            // row-gather incidences (element << 2 | local, 4 node positions)
     fk::DevBuf<int4> tet_nat;         // elements in caller order (uploaded with the mesh)
     fk::DevBuf<double> coef_nat;      // coefficient in caller order
     fk::DevBuf<double> dmu_nat;       // dmu in caller order (gather Newton assembly)
