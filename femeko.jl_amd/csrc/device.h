@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace fk {
 
@@ -157,6 +158,47 @@ __device__ __forceinline__ double thread_partials_wide(const double *__restrict_
     if ((n & 1) && threadIdx.x == 0) s1 += part[n - 1];
     return (s0 + s1) + (s2 + s3);
 }
+
+// Partials of the previous kernel loaded before anything else a block
+// loads: PER values (P = 1) or 16-byte pairs (P = 2) per thread, every load
+// unconditional (index clamped into the array, the value masked in sum()),
+// so that the caller, after a scheduling barrier, issues its vector loads
+// and the sums wait for these loads only (loads retire in issue order)
+// while the vector loads are still in flight. Partials beyond the PER
+// loads of every thread (grids larger than PER * BLOCK * P blocks) are
+// added by a trailing loop. The order of the sums is fixed, the same in
+// every block.
+template <int BLOCK, int PER, int P>
+struct EarlyPartials {
+    using E = typename std::conditional<P == 2, double2, double>::type;
+    E v[PER];
+    __device__ __forceinline__ void load(const double *__restrict__ part, int n) {
+        const E *__restrict__ pe = reinterpret_cast<const E *>(part);
+        const int last = max(n / P - 1, 0);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) v[u] = pe[min(static_cast<int>(threadIdx.x) + u * BLOCK, last)];
+    }
+    __device__ __forceinline__ double sum(const double *__restrict__ part, int n) const {
+        const int ne = n / P;  // whole elements
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const bool in = static_cast<int>(threadIdx.x) + u * BLOCK < ne;
+            if constexpr (P == 2)
+                s[u & 3] += in ? v[u].x + v[u].y : 0.0;
+            else
+                s[u & 3] += in ? v[u] : 0.0;
+        }
+        for (int i = static_cast<int>(threadIdx.x) + PER * BLOCK; i < ne; i += BLOCK) {
+            if constexpr (P == 2)
+                s[0] += part[2 * i] + part[2 * i + 1];
+            else
+                s[0] += part[i];
+        }
+        if (P == 2 && (n & 1) && threadIdx.x == 0) s[1] += part[n - 1];
+        return (s[0] + s[1]) + (s[2] + s[3]);
+    }
+};
 
 template <int BLOCK>
 __device__ __forceinline__ double sum_partials(const double *__restrict__ part, int n, double *sh) {
