@@ -120,6 +120,43 @@ __device__ __forceinline__ double seg_thread(const double *__restrict__ part, in
     for (; b < b1; b += BLOCK) s0 += part[b * nrhs + k];
     return s0 + s1;
 }
+// seg_thread with the thread's first partial loaded ahead (seg_first) by
+// the caller before its vector loads: the same sums in the same order, so
+// the waits for the partials do not include the vector loads (loads retire
+// in issue order). seg_first's index is clamped into the segment (b1 > b0).
+template <int BLOCK>
+__device__ __forceinline__ double seg_first(const double *__restrict__ part, int b0, int b1, int k, int nrhs) {
+    const int b = b0 + static_cast<int>(threadIdx.x);
+    return part[(b < b1 ? b : b1 - 1) * nrhs + k];
+}
+template <int BLOCK>
+__device__ __forceinline__ double seg_thread_first(double first, const double *__restrict__ part, int b0, int b1,
+                                                   int k, int nrhs) {
+    double s0 = 0.0, s1 = 0.0;
+    int b = b0 + static_cast<int>(threadIdx.x);
+    if (b + BLOCK < b1) {
+        // a segment longer than the block: seg_thread's loops, the head
+        // element taken from `first`
+        s0 += first;
+        s1 += part[(b + BLOCK) * nrhs + k];
+        for (b += 2 * BLOCK; b + BLOCK < b1; b += 2 * BLOCK) {
+            s0 += part[b * nrhs + k];
+            s1 += part[(b + BLOCK) * nrhs + k];
+        }
+        for (; b < b1; b += BLOCK) s0 += part[b * nrhs + k];
+    } else if (b < b1) {
+        s0 += first;
+    }
+    return s0 + s1;
+}
+
+// Clamped row of a prefetch load (the value is used only for i < r1): an
+// unconditional load, no branch between the loads and the partial sums.
+__device__ __forceinline__ int64_t clamp_row(int64_t i, int64_t r1) {
+    const int64_t c = i < r1 ? i : r1 - 1;
+    return c > 0 ? c : 0;
+}
+
 template <int BLOCK>
 __device__ __forceinline__ double seg_sum(const double *__restrict__ part, int b0, int b1, int k, int nrhs,
                                           double *sh) {
@@ -313,22 +350,33 @@ __global__ __launch_bounds__(BLOCK) void kb_update_xr(BatchArgs ba, float *__res
     }
     const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
     const int64_t i0 = r0 + threadIdx.x;
-    float d0 = 0.f, r0v[NR], q0[NR];
-    if (i0 < r1) {
-        d0 = dinv[i0];
+    const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
+    const int c0 = pq_b0[mesh], c1 = pq_b1[mesh];
+    // the first partial of each sum, then the row's values (see seg_first)
+    double f_rz[NR], f_pq[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        f_rz[j] = seg_first<BLOCK>(rz_cur, b0, b1, j, NR);
+        f_pq[j] = seg_first<BLOCK>(part_pq, c0, c1, j, NR);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    float d0, r0v[NR], q0[NR];
+    {
+        const int64_t ic = clamp_row(i0, r1);
+        d0 = dinv[ic];
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            r0v[j] = r[i0 * NR + j];
-            q0[j] = __builtin_nontemporal_load(q + i0 * NR + j);
+            r0v[j] = r[ic * NR + j];
+            q0[j] = __builtin_nontemporal_load(q + ic * NR + j);
         }
     }
-    const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
+    __builtin_amdgcn_sched_barrier(0);
     // the 2 NR per-mesh reductions in one pass (same order per value)
     double red[2 * NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
-        red[j] = seg_thread<BLOCK>(rz_cur, b0, b1, j, NR);
-        red[NR + j] = seg_thread<BLOCK>(part_pq, pq_b0[mesh], pq_b1[mesh], j, NR);
+        red[j] = seg_thread_first<BLOCK>(f_rz[j], rz_cur, b0, b1, j, NR);
+        red[NR + j] = seg_thread_first<BLOCK>(f_pq[j], part_pq, c0, c1, j, NR);
     }
     block_sum_n<BLOCK, 2 * NR>(red, sh);
     float alpha[NR];
@@ -433,25 +481,36 @@ __global__ __launch_bounds__(BLOCK) void kb_update_p(BatchArgs ba, float *__rest
     if (!any) return;
     const int64_t r0 = ba.blk_r0[lb], r1 = ba.blk_r1[lb];
     const int64_t i0 = r0 + threadIdx.x;
-    float d0 = 0.f, rv0[NR], pv0[NR], xv0[NR];
-    if (i0 < r1) {
-        d0 = dinv[i0];
+    const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
+    // the first partial of each sum, then the row's values (see seg_first)
+    double f_zn[NR], f_zc[NR], f_rr[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        f_zn[j] = seg_first<BLOCK>(rz_next, b0, b1, j, NR);
+        f_zc[j] = seg_first<BLOCK>(rz_cur, b0, b1, j, NR);
+        f_rr[j] = seg_first<BLOCK>(part_rr, b0, b1, j, NR);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    float d0, rv0[NR], pv0[NR], xv0[NR];
+    {
+        const int64_t ic = clamp_row(i0, r1);
+        d0 = dinv[ic];
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            rv0[j] = r[i0 * NR + j];
-            pv0[j] = p[i0 * NR + j];
-            xv0[j] = ldx(x + i0 * NR + j);
+            rv0[j] = r[ic * NR + j];
+            pv0[j] = p[ic * NR + j];
+            xv0[j] = ldx(x + ic * NR + j);
         }
     }
-    const int b0 = ba.mesh_b0[mesh], b1 = ba.mesh_b1[mesh];
+    __builtin_amdgcn_sched_barrier(0);
     // the per-mesh reductions in one pass (r.r only in the mesh's first
     // block, which keeps the convergence state)
     double red[3 * NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
-        red[j] = seg_thread<BLOCK>(rz_next, b0, b1, j, NR);
-        red[NR + j] = seg_thread<BLOCK>(rz_cur, b0, b1, j, NR);
-        red[2 * NR + j] = lb == b0 ? seg_thread<BLOCK>(part_rr, b0, b1, j, NR) : 0.0;
+        red[j] = seg_thread_first<BLOCK>(f_zn[j], rz_next, b0, b1, j, NR);
+        red[NR + j] = seg_thread_first<BLOCK>(f_zc[j], rz_cur, b0, b1, j, NR);
+        red[2 * NR + j] = lb == b0 ? seg_thread_first<BLOCK>(f_rr[j], part_rr, b0, b1, j, NR) : 0.0;
     }
     block_sum_n<BLOCK, 3 * NR>(red, sh);
     float beta[NR];
