@@ -110,7 +110,8 @@ __global__ __launch_bounds__(256) void k_assemble(const int4 *__restrict__ tet, 
 // records. The cost of this kernel is the number of per-lane addresses its
 // gathers issue more than their bytes: at n=160 (24.6 M elements) the
 // separate X, Y, Z arrays took 1.21 ms, records 1.05 ms, records without the
-// own node 1.01 ms, with bit-identical values. Staging the coordinates of the
+// own node 1.01 ms, with bit-identical values (a numbering without locality
+// pays for the records' larger footprint, see below). Staging the coordinates of the
 // block's columns in LDS instead (no per-incidence coordinate gathers, 32
 // bytes of LDS per value) took 1.27-1.88 ms: the LDS then holds about 5
 // waves per CU.
@@ -120,11 +121,14 @@ __global__ __launch_bounds__(256) void k_assemble(const int4 *__restrict__ tet, 
 // adds nothing): the loads of the group (incidence words, then element nodes
 // and coefficients, then coordinates) are in flight together, and no branch
 // sits between a load and its use (a branch there makes the compiler wait
-// for all outstanding loads). With records and the own node, 6 per group
-// (166 VGPRs) measured 1.01 ms, 8 1.02 ms, 4 1.13 ms. The sums run in the
-// same (element) order either way.
+// for all outstanding loads). With records, the own node and the 8-byte
+// incidence, 8 per group (210 VGPRs) measured 1.00 ms, 6 0.995 ms, 4 1.07
+// ms; on the same mesh with its nodes numbered by reverse Cuthill-McKee
+// (elements in cell order, so element and node order no longer follow each
+// other) 1.88, 1.95 and 2.33 ms, where the separate X, Y, Z arrays took
+// 1.76 ms. The sums run in the same (element) order either way.
 constexpr uint32_t kNoInc = 0xffffffffu;
-constexpr int kGatherUnroll = 6;
+constexpr int kGatherUnroll = 8;
 constexpr int kNewtonUnroll = 4;
 
 // The 12 coordinates (and the 4 w fields when W) of an element's nodes in

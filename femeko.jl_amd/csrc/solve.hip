@@ -109,13 +109,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const int32_t *__restrict__ 
 // reduction (a band of 4096 rows is 4 vectors per thread in FP32).
 constexpr int kPrefetch = 4;
 // Partials per thread that K2 and K3 load first (EarlyPartials): K2's SpMV
-// partials as pairs (16 per thread cover the 8152 blocks of the FP32 stencil
-// SpMV at n=160, 2 the 1024 of the FP64 grid), and the vector kernels' own
-// 1024 partials. Issued before the vector loads, the sums no longer wait for
+// partials as pairs, PQ per thread, instantiated for 2, 4, 8 and 16 and
+// picked per launch from the SpMV's block count (16 cover the 8152 blocks of
+// the FP32 stencil SpMV at n=160, 4 its 2016 at n=100, 2 the 1024 of the
+// FP64 grid), and the vector kernels' own 1024 partials. Issued before the vector loads, the sums no longer wait for
 // those: K2 alone on n=161^3 took 13.2 us against 15.4 us with the partials
 // loaded after the vectors (12.1 us with no reduction at all).
-template <typename T>
-constexpr int kEarlyPq = sizeof(T) == 4 ? 16 : 2;
 constexpr int kEarlyBlk = 4;
 
 // Clamped index of a prefetched vector: every prefetch load unconditional
@@ -134,7 +133,7 @@ __device__ __forceinline__ int64_t clamp_vec(int64_t j, int64_t v1, int64_t nvec
 // same value from the same partials) is left in st->alpha for K3, which
 // applies x += alpha p while it reads p anyway: 12 bytes per row less than
 // updating x here.
-template <typename T>
+template <typename T, int PQ>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ r, const T *__restrict__ q,
                                                          const T *__restrict__ dinv,
                                                          const double *__restrict__ part_pq, int npq,
@@ -144,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(T *__restrict__ r, cons
                                                          int nblocks, CgState *__restrict__ st, int parity) {
     if (st->done_at[parity]) return;
     __shared__ double sh[2 * (kBlock / 64 + 1)];
-    EarlyPartials<kBlock, kEarlyPq<T>, 2> epq;
+    EarlyPartials<kBlock, PQ, 2> epq;
     EarlyPartials<kBlock, kEarlyBlk, 1> erz;
     epq.load(part_pq, npq);
     erz.load(part_rz_cur, nblocks);
@@ -1600,9 +1599,20 @@ static void enqueue_iteration(fk_mesh *m, PcgWork<T> &w, int parity, bool timed)
         FK_HIP(hipEventRecord(te.ev[te.used + 1], st));
         te.used += 2;
     }
-    k_cg_update_xr<T><<<nb, kBlock, 0, st>>>(w.r.get(), w.q.get(), w.dinv.get(),
-                                             w.part_pq.get(), spmv_blocks<T>(m), rz_cur, rz_next, w.part_rr.get(),
-                                             m->nv, m->chunk, nb, w.state.get(), parity);
+    const int npq = spmv_blocks<T>(m);
+    const int pairs = (npq + 1) / 2;
+    auto xr = [&](auto kern) {
+        kern<<<nb, kBlock, 0, st>>>(w.r.get(), w.q.get(), w.dinv.get(), w.part_pq.get(), npq, rz_cur, rz_next,
+                                    w.part_rr.get(), m->nv, m->chunk, nb, w.state.get(), parity);
+    };
+    if (pairs <= 2 * kBlock)
+        xr(k_cg_update_xr<T, 2>);
+    else if (pairs <= 4 * kBlock)
+        xr(k_cg_update_xr<T, 4>);
+    else if (pairs <= 8 * kBlock)
+        xr(k_cg_update_xr<T, 8>);
+    else
+        xr(k_cg_update_xr<T, 16>);
     k_cg_update_p<T><<<nb, kBlock, 0, st>>>(w.x.get(), w.p.get(), w.r.get(), w.dinv.get(), rz_next, rz_cur,
                                             w.part_rr.get(),
                                             m->nv, m->chunk, nb, w.state.get(), parity);
