@@ -238,14 +238,14 @@ def bench_batch(args):
 
 def bench_froehlich(args):
     """n=100 Froehlich magnet, Picard/Newton mu loop on the GPU, update solves
-    to 1e-10 in FP64 PCG or (--loop-precision f32) FP32 PCG with FP64 residual
+    to 1e-10 in FP64 PCG or (--inner-precision f32) FP32 PCG with FP64 residual
     replacement."""
     from femeko_hip import workloads as W
     from femeko_hip.device import DeviceMesh, context
     pr = W.froehlich()
     dm = DeviceMesh(pr.mesh, context(0))
     dm.set_dirichlet(pr.mask, pr.g)
-    dm.set_loop_precision(args.loop_precision)
+    dm.set_inner_precision(args.inner_precision)
     reg = pr.magnet.astype(np.uint8)
     for _ in range(args.warmup):
         out = dm.nonlinear_froehlich(reg, pr.meta["Ms"], pr.meta["a"])
@@ -254,14 +254,14 @@ def bench_froehlich(args):
         out = dm.nonlinear_froehlich(reg, pr.meta["Ms"], pr.meta["a"])
     t = (time.perf_counter() - t0) / args.steps
     log(f"froehlich: {t*1e3:.1f} ms, outer {out['outer']}, CG {out['cg_iterations']}, change {out['change']:.2e}")
-    prec = "FP64" if args.loop_precision == "f64" else "FP32-refined update solves"
+    prec = "FP64" if args.inner_precision == "f64" else "FP32-refined update solves"
     res = _common(args, f"nonlinear magnetostatics DOF/s (Picard/Newton mu loop, {prec})", pr.mesh.nv / t, "DOF/s",
-                  t * 1e3, "f64" if args.loop_precision == "f64" else "f32+f64",
+                  t * 1e3, "f64" if args.inner_precision == "f64" else "f32+f64",
                   {"workload": "froehlich100: Kuhn n=100, Froehlich mu_r = 1 + 1.2e6/(500+|H|) in [-0.25,0.25]^3, "
                                "|H0| = 1e4", "nodes": pr.mesh.nv, "tets": pr.mesh.nt})
     res.update({"outer_iterations": out["outer"], "cg_iterations": out["cg_iterations"],
                 "final_mu_change": out["change"]})
-    if args.loop_precision == "f32":
+    if args.inner_precision == "f32":
         _loop_vs_f64(dm, lambda: dm.nonlinear_froehlich(reg, pr.meta["Ms"], pr.meta["a"]), res)
     dm.close()
     return res
@@ -272,7 +272,7 @@ def _loop_vs_f64(dm, run, res):
     solves (the second timed), and the relative 2-norm difference of the
     potentials."""
     phi = dm.potential()
-    dm.set_loop_precision("f64")
+    dm.set_inner_precision("f64")
     run()
     t0 = time.perf_counter()
     run()
@@ -287,7 +287,7 @@ def _loop_vs_f64(dm, run, res):
 def bench_tabulated(args):
     """n=100 gadolinium cube (B(H) table at 270 K, the reference's material)
     under 0.5 T, mu(|H|) loop with the tabulated law, update solves to 1e-10
-    (FP32-refined or FP64, --loop-precision), stop at mu0 max| |H| - |H|_prev | < 1e-4 T (NewtonRaphson.jl:36). Picard
+    (FP32-refined or FP64, --inner-precision), stop at mu0 max| |H| - |H|_prev | < 1e-4 T (NewtonRaphson.jl:36). Picard
     steps until that change is below 1e-2 T, Newton steps after: undamped
     Newton from the second step (the reference damps it with lineSearch)
     drives a few cube-corner elements, where the field is singular, past the
@@ -301,7 +301,7 @@ def bench_tabulated(args):
     pr = W.tabulated()
     dm = DeviceMesh(pr.mesh, context(0))
     dm.set_dirichlet(pr.mask, pr.g)
-    dm.set_loop_precision(args.loop_precision)
+    dm.set_inner_precision(args.inner_precision)
     reg = pr.magnet.astype(np.uint8)
     for _ in range(args.warmup):
         out = dm.nonlinear_spline(reg, spl, spl_dmu, rtol=1e-10, tol_H=1e-4, switch_tol=1e-2)
@@ -310,16 +310,16 @@ def bench_tabulated(args):
         out = dm.nonlinear_spline(reg, spl, spl_dmu, rtol=1e-10, tol_H=1e-4, switch_tol=1e-2)
     t = (time.perf_counter() - t0) / args.steps
     log(f"tabulated: {t*1e3:.1f} ms, outer {out['outer']}, CG {out['cg_iterations']}, change {out['change']:.2e}")
-    prec = "FP64" if args.loop_precision == "f64" else "FP32-refined update solves"
+    prec = "FP64" if args.inner_precision == "f64" else "FP32-refined update solves"
     res = _common(args, f"nonlinear magnetostatics DOF/s (tabulated mu(|H|) Newton loop, {prec})", pr.mesh.nv / t,
-                  "DOF/s", t * 1e3, "f64" if args.loop_precision == "f64" else "f32+f64",
+                  "DOF/s", t * 1e3, "f64" if args.inner_precision == "f64" else "f32+f64",
                   {"workload": "tabulated100: Kuhn n=100, Gd B(H) at 270 K (62-point table, cubic splines of mu "
                                "and dmu) in [-0.25,0.25]^3, Bext = 0.5 T along x; Newton once the change "
                                "is below 1e-2 T", "nodes": pr.mesh.nv,
                    "tets": pr.mesh.nt})
     res.update({"outer_iterations": out["outer"], "cg_iterations": out["cg_iterations"],
                 "final_H_change_T": out["change"]})
-    if args.loop_precision == "f32":
+    if args.inner_precision == "f32":
         _loop_vs_f64(dm, lambda: dm.nonlinear_spline(reg, spl, spl_dmu, rtol=1e-10, tol_H=1e-4, switch_tol=1e-2),
                      res)
     dm.close()
@@ -331,8 +331,9 @@ def bench_demag(args):
     (Micromagnetics/LL.jl:162-209): n=100 Kuhn box, magnet [-0.25,0.25]^3
     with a unit magnetization that rotates a little each step, A =
     stiffnessMatrix(mesh) resident, one fk_demag_field per step at the
-    reference cg's tolerance sqrt(eps). A step = load vector + FP64 PCG +
-    nodal field, host copies of M and Hd included."""
+    reference cg's tolerance sqrt(eps). A step = load vector + Neumann PCG
+    (FP32-refined or FP64, --inner-precision) + nodal field, host copies of M
+    and Hd included."""
     from femeko_hip import workloads as W
     from femeko_hip.device import DeviceMesh, context
     pr = W.linear(100, seed=102)
@@ -344,6 +345,7 @@ def bench_demag(args):
     dm = DeviceMesh(m, context(0))
     dm.set_coefficient(np.ones(m.nt))
     dm.assemble()
+    dm.set_inner_precision(args.inner_precision)
     ins = inside.astype(np.uint8)
     innodes = vol > 0
 
@@ -368,12 +370,27 @@ def bench_demag(args):
     t, iters = run(True)
     log(f"demag: {t*1e3:.1f} ms per step starting from the previous potential (PCG iterations {iters}); "
         f"{t_cold*1e3:.1f} ms from zero as the reference's cg ({it_cold})")
-    res = _common(args, "demagnetizing-field solves per second (fixed matrix, new right-hand side each step, FP64)",
-                  1.0 / t, "solves/s", t * 1e3, "f64",
+    f32 = args.inner_precision == "f32"
+    prec = "FP32-refined" if f32 else "FP64"
+    res = _common(args, f"demagnetizing-field solves per second (fixed matrix, new right-hand side each step, {prec})",
+                  1.0 / t, "solves/s", t * 1e3, "f32+f64" if f32 else "f64",
                   {"workload": "demag100: Kuhn n=100, magnet [-0.25,0.25]^3, unit M rotating 0.05 rad per step, "
                                "rtol sqrt(eps), PCG started from the previous step's potential", "nodes": m.nv,
                    "tets": m.nt})
     res.update({"pcg_iterations": iters, "from_zero": {"ms_per_step": t_cold * 1e3, "pcg_iterations": it_cold}})
+    if f32:
+        # after the timed region: the same steps from zero in both precisions
+        # at rtol 1e-12, the fields compared, and the FP64 path timed
+        k = args.warmup + args.steps
+        H32, _i, _r = dm.demag_field(ins, M_at(k), vol, rtol=1e-12)
+        dm.set_inner_precision("f64")
+        t64, it64 = run(True)
+        H64, _i, _r = dm.demag_field(ins, M_at(k), vol, rtol=1e-12)
+        ok = ~np.isnan(H64)
+        d = float(np.abs(H32[ok] - H64[ok]).max() / np.abs(H64[ok]).max())
+        log(f"FP64 Neumann solves: {t64*1e3:.1f} ms per step ({it64}); field difference at rtol 1e-12 {d:.2e}")
+        res.update({"f64_ms_per_step": t64 * 1e3, "speedup_vs_f64": t64 / t, "f64_pcg_iterations": it64,
+                    "field_max_diff_vs_f64_at_1e-12": d})
     dm.close()
     return res
 
@@ -449,8 +466,8 @@ def main():
     ap.add_argument("--reorder", default="auto", choices=["auto", "none", "rcm"],
                     help="the library's internal node renumbering (fk_mesh_create_ex)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--loop-precision", default="f32", choices=["f64", "f32"],
-                    help="froehlich/tabulated workloads: precision of the mu loop's update solves")
+    ap.add_argument("--inner-precision", default="f32", choices=["f64", "f32"],
+                    help="froehlich, tabulated and demag workloads: precision of the solves inside the loop")
     ap.add_argument("--batch-update-block", type=int, default=0, choices=[0, 128, 256, 512],
                     help="batch workload: the update kernels' block size instead of the autotuner's")
     ap.add_argument("--workload", default="boxes",

@@ -135,6 +135,7 @@ struct PcgWork {
     hipGraphExec_t ru_graph = nullptr;
     int ru_graph_cfg = -1;
     const double *ru_graph_u = nullptr, *ru_graph_b = nullptr;  // the unknown and RHS it baked in
+    const uint8_t *ru_graph_mask = nullptr;                        // and the mask
     void reset_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
@@ -143,6 +144,7 @@ struct PcgWork {
         ru_graph = nullptr;
         ru_graph_cfg = -1;
         ru_graph_u = ru_graph_b = nullptr;
+        ru_graph_mask = nullptr;
     }
     ~PcgWork() { reset_graph(); }
 };
@@ -184,7 +186,8 @@ struct fk_mesh {
     bool have_coef = false;
     fk::DevBuf<double> dmu;           // d mu / d|H| per element (colour order), nonlinear loop
     fk::DevBuf<double> hn_prev;       // |H| of the previous step (colour order), tabulated-law loop
-    int loop_precision = FK_PREC_F64; // update solves of the mu(|H|) loops
+    int inner_precision = FK_PREC_F64; // solves inside the mu(|H|) loops, Neumann and demag
+    fk::DevBuf<uint8_t> mask_free;    // all-zero mask: the FP32 solver with every node free
     fk::DevBuf<double> inc64;         // FP64 increment of an FP32-refined update solve
     fk::DevBuf<double> demag_u;       // potential of the last demagnetizing-field call
     bool have_demag_u = false;
@@ -321,6 +324,8 @@ inline void values_changed(fk_mesh *m) {
 }
 CgState run_pcg_f64(fk_mesh *m, double rtol, int64_t max_iter, double bb_ref);
 int solve_f32_increment(fk_mesh *m, double *u, const double *b, double rtol, int64_t max_iter, int64_t *iters);
+int solve_f32_free(fk_mesh *m, double *u, const double *b, double rtol, int64_t max_iter, double bb_ref,
+                   int64_t *iters, double *rr_out, double *bb_out);
 struct NeumannResult {
     double multiplier;
     int64_t iters;

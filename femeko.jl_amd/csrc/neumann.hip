@@ -238,7 +238,19 @@ NeumannResult solve_neumann_dev(fk_mesh *m, double rtol, int64_t max_iter, const
                                                                     m->val64.get(), u0, w.r.get(), nv);
         FK_HIP(hipGetLastError());
     }
-    const CgState cs = run_pcg_f64(m, rtol, max_iter, bb_ref);
+    CgState cs{};
+    if (m->inner_precision == FK_PREC_F32_REFINED) {
+        // FP32 recursion with FP64 residual replacement on every row; the
+        // solution (from zero, the correction when warm) into w.x
+        double rr = 0.0, bb = 0.0;
+        int64_t its = 0;
+        cs.status = solve_f32_free(m, w.x.get(), w.r.get(), rtol, max_iter, bb_ref, &its, &rr, &bb);
+        cs.iters = its;
+        cs.rr = rr;
+        cs.bb = bb;
+    } else {
+        cs = run_pcg_f64(m, rtol, max_iter, bb_ref);
+    }
     if (cs.status != FK_OK) fail(cs.status, std::string("Neumann solve: ") + fk_status_string(cs.status));
     if (u0) {
         k_axpy_inplace<<<grid_for(nv, 256), 256, 0, st>>>(w.x.get(), u0, 1.0, nv);

@@ -374,7 +374,7 @@ static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const La
     for (it = 1; it <= max_outer; ++it) {
         const int newton = dev < switch_tol ? 1 : 0;
         newton_values(m, newton, w.r.get());
-        if (m->loop_precision == FK_PREC_F32_REFINED) {
+        if (m->inner_precision == FK_PREC_F32_REFINED) {
             // FP32 recursion on the step's system, FP64 true residual
             // w.r - (A + At) du replaced as the recursion falls
             m->inc64.ensure(nv);
@@ -647,9 +647,9 @@ using fk::guard;
 
 extern "C" {
 
-int fk_set_loop_precision(fk_mesh *m, int32_t precision) {
+int fk_set_inner_precision(fk_mesh *m, int32_t precision) {
     if (!m || (precision != FK_PREC_F64 && precision != FK_PREC_F32_REFINED)) return FK_ERR_INVALID_ARG;
-    m->loop_precision = precision;
+    m->inner_precision = precision;
     return FK_OK;
 }
 

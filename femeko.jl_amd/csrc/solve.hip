@@ -1935,7 +1935,8 @@ static int solve_f32_replace(fk_mesh *m, double *u, const double *b, double rtol
         hipGraphExec_t *gx = nullptr;
         if (m->ctx->graphs) {
             const int cfg = solver_config(m) * 64 + G;
-            if (!w.ru_graph || w.ru_graph_cfg != cfg || w.ru_graph_u != u || w.ru_graph_b != b) {
+            if (!w.ru_graph || w.ru_graph_cfg != cfg || w.ru_graph_u != u || w.ru_graph_b != b ||
+                w.ru_graph_mask != m->mask.get()) {
                 if (w.ru_graph) (void)hipGraphExecDestroy(w.ru_graph);
                 w.ru_graph = nullptr;
                 hipGraph_t graph;
@@ -1948,6 +1949,7 @@ static int solve_f32_replace(fk_mesh *m, double *u, const double *b, double rtol
                 w.ru_graph_cfg = cfg;
                 w.ru_graph_u = u;
                 w.ru_graph_b = b;
+                w.ru_graph_mask = m->mask.get();
             }
             gx = &w.ru_graph;
         }
@@ -2050,7 +2052,8 @@ static double neg_residual(fk_mesh *m, const double *phi, double *r, const doubl
 // nodes (u, b FP64 on the device, nv values; u is overwritten), to
 // ||mask(b - A u)|| < rtol ||mask b||: the increment of a Picard or Newton
 // step. Returns the status and the FP32 iterations.
-int solve_f32_increment(fk_mesh *m, double *u, const double *b, double rtol, int64_t max_iter, int64_t *iters) {
+static int solve_f32_from_zero(fk_mesh *m, double *u, const double *b, double rtol, int64_t max_iter,
+                               double bb_ref, int64_t *iters, double *rr_out, double *bb_out) {
     hipStream_t st = m->ctx->stream;
     m->r64.ensure(m->nv);
     ensure_work<float>(m);
@@ -2060,12 +2063,47 @@ int solve_f32_increment(fk_mesh *m, double *u, const double *b, double rtol, int
         m->dinv32_valid = true;
     }
     FK_HIP(hipMemsetAsync(u, 0, m->nv * sizeof(double), st));
-    const double bb = neg_residual(m, u, m->r64.get(), b);  // ||mask b||^2
+    const double bb0 = neg_residual(m, u, m->r64.get(), b);  // ||mask b||^2
+    const double bb = bb_ref >= 0.0 ? bb_ref : bb0;
     *iters = 0;
-    if (!(bb > 0.0)) return std::isfinite(bb) ? FK_OK : FK_ERR_NONFINITE;
+    *rr_out = bb0;
+    *bb_out = bb;
+    if (!std::isfinite(bb0) || !std::isfinite(bb)) return FK_ERR_NONFINITE;
+    if (!(bb0 > 0.0)) return FK_OK;
     int32_t nrep = 0;
-    double rr = 0.0;
-    return solve_f32_replace(m, u, b, rtol, max_iter, bb, 0.95, iters, &nrep, &rr);
+    return solve_f32_replace(m, u, b, rtol, max_iter, bb, 0.95, iters, &nrep, rr_out);
+}
+
+int solve_f32_increment(fk_mesh *m, double *u, const double *b, double rtol, int64_t max_iter, int64_t *iters) {
+    double rr = 0.0, bb = 0.0;
+    return solve_f32_from_zero(m, u, b, rtol, max_iter, -1.0, iters, &rr, &bb);
+}
+
+// The same with every node free, for the consistent singular Neumann system
+// (A 1 = 0, b summing to zero): the FP32 solver runs with an all-zero mask
+// swapped in for the Dirichlet one (D^-1 and the FP64 residual on every row),
+// which is swapped back, and D^-1 marked stale, on every way out. The
+// difference-form product maps constants to zero exactly, as A does. Stops at
+// rtol relative to sqrt(bb_ref) (bb_ref < 0: ||b||); *rr_out and *bb_out
+// return the final FP64 ||b - A u||^2 and the reference.
+int solve_f32_free(fk_mesh *m, double *u, const double *b, double rtol, int64_t max_iter, double bb_ref,
+                   int64_t *iters, double *rr_out, double *bb_out) {
+    if (m->mask_free.n < static_cast<size_t>(m->nv)) {
+        m->mask_free.ensure(m->nv);
+        FK_HIP(hipMemsetAsync(m->mask_free.get(), 0, m->nv, m->ctx->stream));
+    }
+    struct Swap {
+        fk_mesh *m;
+        explicit Swap(fk_mesh *mm) : m(mm) {
+            std::swap(m->mask, m->mask_free);
+            m->dinv32_valid = false;
+        }
+        ~Swap() {
+            std::swap(m->mask, m->mask_free);
+            m->dinv32_valid = false;
+        }
+    } swap(m);
+    return solve_f32_from_zero(m, u, b, rtol, max_iter, bb_ref, iters, rr_out, bb_out);
 }
 
 void ensure_work_f64(fk_mesh *m) { ensure_work<double>(m); }

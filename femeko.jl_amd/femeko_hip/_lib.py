@@ -91,7 +91,7 @@ SIGNATURES = {
     "fk_solve_neumann": (_c.c_int, [_vp, _dp, _c.c_double, _c.c_int64, _dp, _dp, _i64p, _dp]),
     "fk_nonlinear_froehlich": (_c.c_int, [_vp, _up, _c.c_double, _c.c_double, _c.c_double, _c.c_double,
                                           _c.c_int32, _c.c_double, _ip, _i64p, _dp]),
-    "fk_set_loop_precision": (_c.c_int, [_vp, _c.c_int32]),
+    "fk_set_inner_precision": (_c.c_int, [_vp, _c.c_int32]),
     "fk_nonlinear_spline": (_c.c_int, [_vp, _up, _dp, _c.c_int32, _dp, _dp, _c.c_int32, _dp, _c.c_double,
                                        _c.c_double, _c.c_int32, _c.c_double, _ip, _i64p, _dp]),
     "fk_demag_field": (_c.c_int, [_vp, _up, _dp, _dp, _c.c_double, _c.c_int64, _c.c_int32, _dp, _i64p, _dp]),

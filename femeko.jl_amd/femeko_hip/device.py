@@ -291,13 +291,14 @@ class DeviceMesh:
         self.ctx.check(self._L.fk_get_coefficient(self.handle, dptr(f)), "fk_get_coefficient")
         return f
 
-    def set_loop_precision(self, precision: str) -> None:
-        """Precision of the mu(|H|) loops' update solves: 'f64' (default) or
-        'f32' (FP32-refined, fk_set_loop_precision)."""
+    def set_inner_precision(self, precision: str) -> None:
+        """Precision of the solves inside the mu(|H|) loops, the Neumann
+        solves (solve_neumann, demag_field, newton_saddle): 'f64' (default)
+        or 'f32' (FP32-refined, fk_set_inner_precision)."""
         code = {"f64": _lib.FK_PREC_F64, "f32": _lib.FK_PREC_F32_REFINED}.get(precision)
         if code is None:
-            raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, f"unknown loop precision {precision!r}")
-        self.ctx.check(self._L.fk_set_loop_precision(self.handle, code), "fk_set_loop_precision")
+            raise FemekoHIPError(_lib.FK_ERR_INVALID_ARG, f"unknown inner precision {precision!r}")
+        self.ctx.check(self._L.fk_set_inner_precision(self.handle, code), "fk_set_inner_precision")
 
     def nonlinear_froehlich(self, region, Ms: float, a: float, rtol: float = 1e-10, tol_mu: float = 1e-6,
                             max_outer: int = 20, switch_tol: float = 0.25, raise_on_error: bool = True):

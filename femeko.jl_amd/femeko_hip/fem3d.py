@@ -104,12 +104,14 @@ def BoundaryIntegral(mesh, F, shell_id) -> np.ndarray:
     return device_mesh(mesh).boundary_integral(mesh.surfaceT, mesh.normal, mesh.AE, F, shell_id)
 
 
-def solveSaddle(mesh, mu, b, rtol: float = 1e-10):
+def solveSaddle(mesh, mu, b, rtol: float = 1e-10, precision: str = "f64"):
     """u of [A Lag; Lag' 0] \\ [b; 0] (Magnetostatics/magneticField.jl:99-108)
-    with A = stiffnessMatrix(mesh, mu). Returns (u, multiplier)."""
+    with A = stiffnessMatrix(mesh, mu); precision 'f64' or 'f32' (FP32-refined
+    PCG on the consistent singular system). Returns (u, multiplier)."""
     dm = device_mesh(mesh)
     dm.set_coefficient(mu)
     dm.assemble()
+    dm.set_inner_precision(precision)
     u, mult, _it, _rel = dm.solve_neumann(b, rtol)
     return u, mult
 

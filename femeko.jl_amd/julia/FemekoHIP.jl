@@ -296,7 +296,7 @@ end
 # Spline1D(HofM, dmu), Dierckx.jl: fields t and c are FITPACK's knots and
 # coefficients) on the elements listed in mesh.InsideElements, Dirichlet data
 # (fixed, g). precision = :f32 runs each step's update solve as FP32 PCG
-# with FP64 residual replacement (fk_set_loop_precision), :f64 in FP64.
+# with FP64 residual replacement (fk_set_inner_precision), :f64 in FP64.
 # Returns (u, mu, (outer, cg_iterations, change)).
 function newtonSpline(mesh, spl, spl_dmu, fixed::AbstractVector{Bool}, g::Vector{Float64};
                       rtol = 1e-10, maxDeviation = 1e-4, maxAtt = 100, switch_tol = Inf, precision = :f64)
@@ -304,8 +304,8 @@ function newtonSpline(mesh, spl, spl_dmu, fixed::AbstractVector{Bool}, g::Vector
     region = zeros(UInt8, mesh.nt)
     region[mesh.InsideElements] .= 0x01
     set_dirichlet!(dm, fixed, g)
-    check(dm.ctx, ccall((:fk_set_loop_precision, LIB), Cint, (Ptr{Cvoid}, Int32), dm.handle,
-                        precision == :f64 ? FK_PREC_F64 : FK_PREC_F32_REFINED), "fk_set_loop_precision")
+    check(dm.ctx, ccall((:fk_set_inner_precision, LIB), Cint, (Ptr{Cvoid}, Int32), dm.handle,
+                        precision == :f64 ? FK_PREC_F64 : FK_PREC_F32_REFINED), "fk_set_inner_precision")
     nc_mu = length(spl.t) - 4
     nc_dmu = length(spl_dmu.t) - 4
     outer = Ref{Int32}(0)

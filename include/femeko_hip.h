@@ -287,14 +287,17 @@ int fk_demag_field(fk_mesh *mesh, const uint8_t *inside, const double *M, const 
 int fk_nonlinear_froehlich(fk_mesh *mesh, const uint8_t *region, double Ms, double a, double rtol,
                            double tol_mu, int32_t max_outer, double switch_tol, int32_t *outer,
                            int64_t *cg_iterations, double *final_change);
-/* Precision of the update solves of fk_nonlinear_froehlich and
- * fk_nonlinear_spline: FK_PREC_F64 (default, as the reference: FP64
- * Jacobi-PCG) or FK_PREC_F32_REFINED (FP32 values and PCG recursion, the
- * FP64 residual b - A du of the step's own system replaced as in fk_solve,
- * to the same rtol; the mu update and its convergence test stay FP64).
- * cg_iterations then counts FP32 iterations. No counterpart in the
+/* Precision of the linear solves that the composite entry points run
+ * inside: the update solves of fk_nonlinear_froehlich and
+ * fk_nonlinear_spline, and the singular Neumann solves of fk_solve_neumann,
+ * fk_demag_field and fk_newton_saddle. FK_PREC_F64 (default, as the
+ * reference: FP64 Jacobi-PCG) or FK_PREC_F32_REFINED (FP32 values and PCG
+ * recursion, the FP64 residual b - A x of the solve's own system replaced as
+ * in fk_solve, stopped on that FP64 residual at the same rtol; for the
+ * Neumann solves every node is free and the constant is projected out as in
+ * FP64). Iteration counts then count FP32 iterations. No counterpart in the
  * reference (its solves are FP64). */
-int fk_set_loop_precision(fk_mesh *mesh, int32_t precision);
+int fk_set_inner_precision(fk_mesh *mesh, int32_t precision);
 /* The same loop with the tabulated law of Magnetostatics/NewtonRaphson.jl:
  * 57-63, 204-207: mu = spl(|H|), dmu = spl_dmu(|H|) on the region, two cubic
  * splines in FITPACK form as Dierckx.jl's Spline1D(H, mu; k=3, s=0) holds

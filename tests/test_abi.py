@@ -57,7 +57,7 @@ def test_null_arguments_rejected_without_gpu():
     assert L.fk_set_spmv_format(None, 1) == _lib.FK_ERR_INVALID_ARG
     assert L.fk_batch_set_format(None, 4) == _lib.FK_ERR_INVALID_ARG
     assert L.fk_spmv_burst(None, 1, 10, ctypes.byref(ms)) == _lib.FK_ERR_INVALID_ARG
-    assert L.fk_set_loop_precision(None, 1) == _lib.FK_ERR_INVALID_ARG
+    assert L.fk_set_inner_precision(None, 1) == _lib.FK_ERR_INVALID_ARG
     assert L.fk_nonlinear_spline(None, None, None, 8, None, None, 8, None, 1e-10, 1e-4, 10, 1.0, None, None,
                                  None) == _lib.FK_ERR_INVALID_ARG
     assert L.fk_status_string(_lib.FK_ERR_DOMAIN) == b"argument outside the table range"

@@ -1,6 +1,6 @@
 """GPU parity of the demagnetizing field of Micromagnetics/LL.jl:162-209
 (fk_demag_field) against the oracle's magnetostatic_field (direct bordered
-solve). Bars: nodal field within 1e-9 of max|Hd| at rtol 1e-12, the same NaN
+solve), with FP64 and with FP32-refined Neumann solves. Bars: nodal field within 1e-9 of max|Hd| at rtol 1e-12, the same NaN
 pattern (0/0 outside the magnet), the exact patch test H = -M for uniform M
 everywhere, and repeated calls on the resident matrix (one per time step)."""
 import numpy as np
@@ -12,19 +12,21 @@ from femeko_hip.device import DeviceMesh, context
 pytestmark = pytest.mark.gpu
 
 
-def _device_A(mesh):
+def _device_A(mesh, precision="f64"):
     dm = DeviceMesh(mesh, context(0))
     dm.set_coefficient(np.ones(mesh.nt))
     dm.assemble()
+    dm.set_inner_precision(precision)
     return dm
 
 
-def test_demag_matches_oracle_over_steps(oracle):
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_demag_matches_oracle_over_steps(oracle, precision):
     pr = W.linear(10)
     m = pr.mesh
     inside = pr.magnet
     vol = oracle.node_volumes(m, inside)
-    dm = _device_A(m)
+    dm = _device_A(m, precision)
     try:
         rng = np.random.default_rng(17)
         for step in range(3):
@@ -45,14 +47,15 @@ def test_demag_matches_oracle_over_steps(oracle):
         dm.close()
 
 
-def test_demag_uniform_magnetization_patch_test(oracle):
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_demag_uniform_magnetization_patch_test(oracle, precision):
     pr = W.linear(8)
     m = pr.mesh
     inside = np.ones(m.nt, np.uint8)
     Mvec = np.array([0.3, -1.2, 0.7])
     M = np.repeat(Mvec[:, None], m.nv, axis=1)
     vol = oracle.node_volumes(m, inside.astype(bool))
-    dm = _device_A(m)
+    dm = _device_A(m, precision)
     try:
         Hd, _its, _rel = dm.demag_field(inside, M, vol, rtol=1e-13)
         np.testing.assert_allclose(Hd, -M, rtol=0, atol=1e-9)
@@ -60,7 +63,8 @@ def test_demag_uniform_magnetization_patch_test(oracle):
         dm.close()
 
 
-def test_demag_warm_start_same_field_fewer_iterations(oracle):
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_demag_warm_start_same_field_fewer_iterations(oracle, precision):
     # slowly rotating magnetization, one call per step: the start from the
     # previous potential reaches the same field in fewer iterations
     pr = W.linear(10)
@@ -68,7 +72,7 @@ def test_demag_warm_start_same_field_fewer_iterations(oracle):
     inside = pr.magnet
     vol = oracle.node_volumes(m, inside)
     innodes = vol > 0
-    dm = _device_A(m)
+    dm = _device_A(m, precision)
     try:
         cold, warm = [], []
         for k in range(4):

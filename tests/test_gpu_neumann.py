@@ -1,6 +1,7 @@
 """GPU Neumann load and Lagrange saddle solve of Magnetostatics/magneticField.jl
 against the oracle's direct solve of the bordered system.
-Bars: RHS within 1e-13 relative; u within 1e-9 relative (PCG rtol 1e-11)."""
+Bars: RHS within 1e-13 relative; u within 1e-9 relative (PCG rtol 1e-11, FP64
+or FP32-refined)."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,8 @@ from femeko_hip.mesh import MU0, boundary_faces, centroids, elements_in_box, kuh
 pytestmark = pytest.mark.gpu
 
 
-def test_saddle_solve_matches_direct(oracle):
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_saddle_solve_matches_direct(oracle, precision):
     m = boundary_faces(kuhn_box_mesh(10, 0.2, seed=41))
     inside = elements_in_box(m, (-0.3,) * 3, (0.3,) * 3)
     mu = np.where(inside, MU0 * 3.0, MU0)          # mu0 (1 + 2) as magneticField.jl:91-92
@@ -20,7 +22,7 @@ def test_saddle_solve_matches_direct(oracle):
     assert np.abs(RHS - RHS_ref).max() <= 1e-13 * np.abs(RHS_ref).max()
     A = oracle.stiffnessMatrix(m, mu)
     u_ref, mult_ref = oracle.saddle_solve(A, oracle.lagrange(m), -RHS_ref)
-    u, mult = F.solveSaddle(m, mu, -RHS, rtol=1e-11)
+    u, mult = F.solveSaddle(m, mu, -RHS, rtol=1e-11, precision=precision)
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-9
     assert abs(mult - mult_ref) <= 1e-9 * max(1.0, abs(mult_ref))
     # Lag' u = 0

@@ -17,9 +17,9 @@ import linesearch_cases as LS
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("loop_precision", ["f64", "f32"])
+@pytest.mark.parametrize("inner_precision", ["f64", "f32"])
 @pytest.mark.parametrize("n,seed", [(8, 12), (12, 100)])
-def test_froehlich_loop_matches_oracle(oracle, n, seed, loop_precision):
+def test_froehlich_loop_matches_oracle(oracle, n, seed, inner_precision):
     # f32: the update solves run FP32-refined (FP64 true residual of each
     # step's own system) to the same rtol; same bars
     pr = W.froehlich(n=n, seed=seed)
@@ -28,7 +28,7 @@ def test_froehlich_loop_matches_oracle(oracle, n, seed, loop_precision):
     dm = DeviceMesh(pr.mesh, context(0))
     try:
         dm.set_dirichlet(pr.mask, pr.g)
-        dm.set_loop_precision(loop_precision)
+        dm.set_inner_precision(inner_precision)
         out = dm.nonlinear_froehlich(pr.magnet.astype(np.uint8), Ms, a, rtol=1e-10, tol_mu=1e-6, max_outer=20)
         assert out["outer"] == ref["iters"]
         assert out["change"] < 1e-6
@@ -147,8 +147,9 @@ def _saddle_case(oracle, tabulated):
     return m, pr.magnet, RHS, (lambda h: oracle.froehlich(h, 1e5, 500.0)), ("froehlich", 1e5, 500.0), 1e-9
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("tabulated", [False, True])
-def test_newton_saddle_matches_oracle(oracle, tabulated):
+def test_newton_saddle_matches_oracle(oracle, tabulated, precision):
     """fk_newton_saddle (NewtonRaphson.jl:109-212 on the GPU) against the
     oracle's restatement with direct bordered solves: the same number of
     attempts, potential within 1e-8, mu within 1e-6, the last lineSearch
@@ -157,6 +158,7 @@ def test_newton_saddle_matches_oracle(oracle, tabulated):
     ref = oracle.newton_saddle(m, RHS, magnet, law_ref, tol_H=tol)
     dm = DeviceMesh(m, context(0))
     try:
+        dm.set_inner_precision(precision)
         out = dm.newton_saddle(magnet.astype(np.uint8), RHS, law, rtol=1e-12, tol_H=tol)
         assert out["outer"] == ref["iters"]
         assert out["change"] <= tol

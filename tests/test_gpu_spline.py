@@ -31,9 +31,9 @@ def gd_splines():
     return Spline1D(tab["H"], mu), Spline1D(tab["H"], dmu)
 
 
-@pytest.mark.parametrize("loop_precision", ["f64", "f32"])
+@pytest.mark.parametrize("inner_precision", ["f64", "f32"])
 @pytest.mark.parametrize("n,seed", [(8, 21), (12, 22)])
-def test_spline_loop_matches_oracle(oracle, n, seed, loop_precision):
+def test_spline_loop_matches_oracle(oracle, n, seed, inner_precision):
     spl, spl_dmu = gd_splines()
     pr = W.tabulated(n=n, seed=seed)
     ref = oracle.newton_spline(pr.mesh, pr.mask, pr.g, pr.magnet, (spl.t, spl.c), (spl_dmu.t, spl_dmu.c),
@@ -41,7 +41,7 @@ def test_spline_loop_matches_oracle(oracle, n, seed, loop_precision):
     dm = DeviceMesh(pr.mesh, context(0))
     try:
         dm.set_dirichlet(pr.mask, pr.g)
-        dm.set_loop_precision(loop_precision)
+        dm.set_inner_precision(inner_precision)
         out = dm.nonlinear_spline(pr.magnet.astype(np.uint8), spl, spl_dmu, rtol=1e-10, tol_H=1e-8, max_outer=30)
         assert out["outer"] == ref["iters"]
         assert out["change"] < 1e-8
