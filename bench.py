@@ -396,8 +396,9 @@ def bench_demag(args):
 
 
 def bench_adjoint(args):
-    """n=80 adjoint gradient of the field-target objective; central differences
-    (step 1e-6) on 32 seeded elements."""
+    """n=80 adjoint gradient of the field-target objective, state and adjoint
+    solves FP32-refined or FP64 (--inner-precision); central differences
+    (step 1e-6, FP64 solves to 1e-13) on 32 seeded elements."""
     from femeko_hip import adjoint as ADJ
     from femeko_hip import fem3d
     from femeko_hip import workloads as W
@@ -406,11 +407,16 @@ def bench_adjoint(args):
     rho = pr.meta["rho"]
     tg = pr.magnet.astype(np.uint8)
     Hs = pr.meta["Hstar"]
-    J, grad, its = ADJ.fieldObjectiveGradient(m, rho, tg, Hs, pr.mask, pr.g)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        J, grad, its = ADJ.fieldObjectiveGradient(m, rho, tg, Hs, pr.mask, pr.g)
-    t = (time.perf_counter() - t0) / args.steps
+    prec = args.inner_precision
+
+    def timed(precision):
+        out = ADJ.fieldObjectiveGradient(m, rho, tg, Hs, pr.mask, pr.g, precision=precision)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = ADJ.fieldObjectiveGradient(m, rho, tg, Hs, pr.mask, pr.g, precision=precision)
+        return (time.perf_counter() - t0) / args.steps, out
+
+    t, (J, grad, its) = timed(prec)
     # 32 seeded elements: the 16 largest gradients and 16 uniform draws; the
     # differences use FP64 solves to rtol 1e-13 so their noise stays far
     # below the step's truncation error
@@ -431,9 +437,18 @@ def bench_adjoint(args):
             err_rel_large.append(abs(fd - grad[e]) / abs(grad[e]))
     log(f"adjoint: {t*1e3:.1f} ms per state+adjoint+gradient; FD max |fd-g|/max|g| {max(err_scaled):.2e}, "
         f"max rel err where |g| >= 1e-2 max|g| {max(err_rel_large):.2e} ({len(err_rel_large)} elements)")
-    res = _common(args, "state+adjoint+gradient DOF/s (FP64)", m.nv / t, "DOF/s", t * 1e3, "f64",
+    if prec == "f32":
+        # after the timed region: the FP64 path timed, its gradient compared
+        t64, (J64, grad64, its64) = timed("f64")
+        dg = float(np.abs(grad - grad64).max() / gmax)
+        log(f"FP64 state and adjoint solves: {t64*1e3:.1f} ms; gradient difference {dg:.2e} of max|g|")
+    res = _common(args, "state+adjoint+gradient DOF/s (" + ("FP32-refined" if prec == "f32" else "FP64") + " solves)",
+                  m.nv / t, "DOF/s", t * 1e3, "f32+f64" if prec == "f32" else "f64",
                   {"workload": "adjoint80: Kuhn n=80, mu_r = 1 + 999 rho^3, target [-0.1,0.1]^3, H* = (2,0,0)",
                    "nodes": m.nv, "tets": m.nt})
+    if prec == "f32":
+        res.update({"f64_ms": t64 * 1e3, "speedup_vs_f64": t64 / t, "grad_max_diff_vs_f64_over_max_grad": dg,
+                    "J_rel_diff_vs_f64": abs(J - J64) / abs(J64), "f64_adjoint_iterations": its64})
     res.update({"J": J, "adjoint_iterations": its, "fd_elements": int(len(idx)), "fd_step": h,
                 "fd_max_err_over_max_grad": float(max(err_scaled)),
                 "fd_max_rel_err_large_grad": float(max(err_rel_large)),
@@ -467,7 +482,7 @@ def main():
                     help="the library's internal node renumbering (fk_mesh_create_ex)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inner-precision", default="f32", choices=["f64", "f32"],
-                    help="froehlich, tabulated and demag workloads: precision of the solves inside the loop")
+                    help="froehlich, tabulated, demag and adjoint workloads: precision of their inner solves")
     ap.add_argument("--batch-update-block", type=int, default=0, choices=[0, 128, 256, 512],
                     help="batch workload: the update kernels' block size instead of the autotuner's")
     ap.add_argument("--workload", default="boxes",

@@ -143,16 +143,28 @@ void adjoint_field_target(fk_mesh *m, const uint8_t *target_host, const double *
                                                               Hstar[2], w.r.get(), e0, e1);
     }
     FK_HIP(hipGetLastError());
-    const CgState s = run_pcg_f64(m, rtol, 1000000, -1.0);
-    if (s.status != FK_OK) fail(s.status, std::string("adjoint solve: ") + fk_status_string(s.status));
+    // A lambda = rhs on the free rows, lambda = 0 on the fixed nodes: FP64
+    // PCG into w.x, or the FP32-refined solve into inc64
+    const double *lam = w.x.get();
+    int64_t its = 0;
+    if (m->inner_precision == FK_PREC_F32_REFINED) {
+        m->inc64.ensure(nv);
+        const int s32 = solve_f32_increment(m, m->inc64.get(), w.r.get(), rtol, 1000000, &its);
+        if (s32 != FK_OK) fail(s32, std::string("adjoint solve: ") + fk_status_string(s32));
+        lam = m->inc64.get();
+    } else {
+        const CgState s = run_pcg_f64(m, rtol, 1000000, -1.0);
+        if (s.status != FK_OK) fail(s.status, std::string("adjoint solve: ") + fk_status_string(s.status));
+        its = s.iters;
+    }
     DevBuf<double> grad(nt);
     k_adjoint_grad<<<grid_for(nt, 256), 256, 0, st>>>(m->tet.get(), m->X.get(), m->Y.get(), m->Z.get(),
-                                                      m->perm.get(), m->phi.get(), w.x.get(), nt, grad.get());
+                                                      m->perm.get(), m->phi.get(), lam, nt, grad.get());
     FK_HIP(hipGetLastError());
     FK_HIP(hipMemcpyAsync(grad_host, grad.get(), nt * sizeof(double), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
     if (J_out) *J_out = J;
-    if (iters_out) *iters_out = s.iters;
+    if (iters_out) *iters_out = its;
 }
 
 }  // namespace fk

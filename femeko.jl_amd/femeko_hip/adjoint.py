@@ -23,33 +23,39 @@ def dsimp(r):
     return 3.0 * 999.0 * r ** 2
 
 
-def _state(mesh, rho, mask, g, mur, rtol):
+def _state(mesh, rho, mask, g, mur, rtol, precision):
+    """State solve and the precision of the adjoint solve: 'f64' (FP64 PCG
+    for both) or 'f32' (FP32-refined, fk_solve and fk_set_inner_precision)."""
     dm = device_mesh(mesh)
     dm.set_coefficient(MU0 * mur(np.asarray(rho, dtype=np.float64)))
     dm.assemble()
     dm.set_dirichlet(mask, g)
-    dm.solve("f64", rtol=rtol)
+    dm.solve(precision, rtol=rtol)
+    dm.set_inner_precision(precision)
     return dm
 
 
-def fieldObjective(mesh, rho, target, Hstar, mask, g, mur=simp, rtol: float = 1e-12) -> float:
-    dm = _state(mesh, rho, mask, g, mur, rtol)
+def fieldObjective(mesh, rho, target, Hstar, mask, g, mur=simp, rtol: float = 1e-12,
+                   precision: str = "f64") -> float:
+    dm = _state(mesh, rho, mask, g, mur, rtol, precision)
     J, _grad, _it = dm.adjoint_field_target(target, Hstar, rtol)
     return J
 
 
-def fieldObjectiveGradient(mesh, rho, target, Hstar, mask, g, mur=simp, dmur=dsimp, rtol: float = 1e-12):
+def fieldObjectiveGradient(mesh, rho, target, Hstar, mask, g, mur=simp, dmur=dsimp, rtol: float = 1e-12,
+                           precision: str = "f64"):
     """(J, dJ/drho, adjoint iterations)."""
-    dm = _state(mesh, rho, mask, g, mur, rtol)
+    dm = _state(mesh, rho, mask, g, mur, rtol, precision)
     J, dJdmu, it = dm.adjoint_field_target(target, Hstar, rtol)
     return J, dJdmu * MU0 * dmur(np.asarray(rho, dtype=np.float64)), it
 
 
-def rrule(f, mesh, rho, target, Hstar, mask, g, mur=simp, dmur=dsimp, rtol: float = 1e-12):
+def rrule(f, mesh, rho, target, Hstar, mask, g, mur=simp, dmur=dsimp, rtol: float = 1e-12,
+          precision: str = "f64"):
     """ChainRulesCore-style rrule of fieldObjective with respect to rho."""
     if f is not fieldObjective:
         raise TypeError("rrule is defined for fieldObjective")
-    J, grad, _it = fieldObjectiveGradient(mesh, rho, target, Hstar, mask, g, mur, dmur, rtol)
+    J, grad, _it = fieldObjectiveGradient(mesh, rho, target, Hstar, mask, g, mur, dmur, rtol, precision)
 
     def pullback(dJ):
         return None, dJ * grad

@@ -242,13 +242,21 @@ function BoundaryIntegral(mesh, F::Vector{Float64}, shell_id)
     return RHS
 end
 
+# precision of the solves inside the composite calls below: :f64 or :f32
+# (FP32-refined; fk_set_inner_precision)
+set_inner_precision!(dm::DeviceMesh, precision::Symbol) =
+    check(dm.ctx, ccall((:fk_set_inner_precision, LIB), Cint, (Ptr{Cvoid}, Int32), dm.handle,
+                        precision == :f64 ? FK_PREC_F64 : FK_PREC_F32_REFINED), "fk_set_inner_precision")
+
 # u of [A Lag; Lag' 0] \ [b; 0] with A = stiffnessMatrix(mesh, mu)
 # (magneticField.jl:99-108; e.g. b = -BoundaryIntegral(mesh, mu0*Hext, shell_id)).
 # Returns (u, multiplier).
-function solveSaddle(mesh, mu::Vector{Float64}, b::Vector{Float64}; rtol = 1e-10, maxiter = 1_000_000)
+function solveSaddle(mesh, mu::Vector{Float64}, b::Vector{Float64}; rtol = 1e-10, maxiter = 1_000_000,
+                     precision = :f64)
     dm = device_mesh(mesh)
     set_coefficient!(dm, mu)
     assemble!(dm)
+    set_inner_precision!(dm, precision)
     u = Vector{Float64}(undef, mesh.nv)
     mult = Ref{Float64}(0.0); iters = Ref{Int64}(0); relres = Ref{Float64}(0.0)
     check(dm.ctx, ccall((:fk_solve_neumann, LIB), Cint,
@@ -304,8 +312,7 @@ function newtonSpline(mesh, spl, spl_dmu, fixed::AbstractVector{Bool}, g::Vector
     region = zeros(UInt8, mesh.nt)
     region[mesh.InsideElements] .= 0x01
     set_dirichlet!(dm, fixed, g)
-    check(dm.ctx, ccall((:fk_set_inner_precision, LIB), Cint, (Ptr{Cvoid}, Int32), dm.handle,
-                        precision == :f64 ? FK_PREC_F64 : FK_PREC_F32_REFINED), "fk_set_inner_precision")
+    set_inner_precision!(dm, precision)
     nc_mu = length(spl.t) - 4
     nc_dmu = length(spl_dmu.t) - 4
     outer = Ref{Int32}(0)
@@ -341,8 +348,9 @@ end
 const _demag_gen = WeakKeyDict{DeviceMesh, Int64}()
 
 function magnetostaticField(mesh, inside::AbstractVector{Bool}, M::Matrix{Float64}, Volumes::Vector{Float64};
-                            rtol = sqrt(eps()), maxiter = 100_000, warm = false)
+                            rtol = sqrt(eps()), maxiter = 100_000, warm = false, precision = :f64)
     dm = device_mesh(mesh)
+    set_inner_precision!(dm, precision)
     if get(_demag_gen, dm, -1) != values_generation(dm)
         # A = stiffnessMatrix(mesh) (permalloy.jl:99), assembled again only
         # when another call replaced the resident values
@@ -381,16 +389,18 @@ end
 # J(rho) = sum_{k: target[k]} VE_k |H_k - Hstar|^2 with mu_k = mu0 mur(rho_k)
 # (default mur = 1 + 999 rho^3) and H from the Dirichlet problem u = g on
 # fixed. The gradient comes from the adjoint kernel fk_adjoint_field_target
-# (dJ/dmu) times mu0 mur'(rho).
+# (dJ/dmu) times mu0 mur'(rho). precision = :f64 (FP64 PCG) or :f32
+# (FP32-refined state and adjoint solves, fk_set_inner_precision).
 simp(r) = 1.0 + 999.0 * r^3
 dsimp(r) = 3.0 * 999.0 * r^2
 
-function _adjoint_state(mesh, rho, target, Hstar, fixed, g, mur, rtol)
+function _adjoint_state(mesh, rho, target, Hstar, fixed, g, mur, rtol, precision)
     dm = device_mesh(mesh)
     set_coefficient!(dm, mu0 .* mur.(rho))
     assemble!(dm)
     set_dirichlet!(dm, fixed, g)
-    solve!(dm; rtol = rtol, precision = :f64)
+    solve!(dm; rtol = rtol, precision = precision)
+    set_inner_precision!(dm, precision)
     J = Ref{Float64}(0.0); iters = Ref{Int64}(0)
     dJdmu = Vector{Float64}(undef, mesh.nt)
     check(dm.ctx, ccall((:fk_adjoint_field_target, LIB), Cint,
@@ -401,23 +411,24 @@ function _adjoint_state(mesh, rho, target, Hstar, fixed, g, mur, rtol)
 end
 
 function fieldObjective(mesh, rho::Vector{Float64}, target::AbstractVector{Bool}, Hstar, fixed::AbstractVector{Bool},
-                        g::Vector{Float64}; mur = simp, rtol = 1e-12)
-    J, _ = _adjoint_state(mesh, rho, target, Hstar, fixed, g, mur, rtol)
+                        g::Vector{Float64}; mur = simp, rtol = 1e-12, precision = :f64)
+    J, _ = _adjoint_state(mesh, rho, target, Hstar, fixed, g, mur, rtol, precision)
     return J
 end
 
 "(J, dJ/drho) of fieldObjective from one state solve, one adjoint solve and the gradient kernel."
 function fieldObjectiveGradient(mesh, rho::Vector{Float64}, target::AbstractVector{Bool}, Hstar,
                                 fixed::AbstractVector{Bool}, g::Vector{Float64}; mur = simp, dmur = dsimp,
-                                rtol = 1e-12)
-    J, dJdmu = _adjoint_state(mesh, rho, target, Hstar, fixed, g, mur, rtol)
+                                rtol = 1e-12, precision = :f64)
+    J, dJdmu = _adjoint_state(mesh, rho, target, Hstar, fixed, g, mur, rtol, precision)
     return J, dJdmu .* mu0 .* dmur.(rho)
 end
 
 function ChainRulesCore.rrule(::typeof(fieldObjective), mesh, rho::Vector{Float64}, target::AbstractVector{Bool},
                               Hstar, fixed::AbstractVector{Bool}, g::Vector{Float64}; mur = simp, dmur = dsimp,
-                              rtol = 1e-12)
-    J, grad = fieldObjectiveGradient(mesh, rho, target, Hstar, fixed, g; mur = mur, dmur = dmur, rtol = rtol)
+                              rtol = 1e-12, precision = :f64)
+    J, grad = fieldObjectiveGradient(mesh, rho, target, Hstar, fixed, g; mur = mur, dmur = dmur, rtol = rtol,
+                                     precision = precision)
     function fieldObjective_pullback(dJ)
         NT = ChainRulesCore.NoTangent()
         return (NT, NT, ChainRulesCore.unthunk(dJ) .* grad, NT, NT, NT, NT)

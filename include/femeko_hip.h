@@ -289,8 +289,9 @@ int fk_nonlinear_froehlich(fk_mesh *mesh, const uint8_t *region, double Ms, doub
                            int64_t *cg_iterations, double *final_change);
 /* Precision of the linear solves that the composite entry points run
  * inside: the update solves of fk_nonlinear_froehlich and
- * fk_nonlinear_spline, and the singular Neumann solves of fk_solve_neumann,
- * fk_demag_field and fk_newton_saddle. FK_PREC_F64 (default, as the
+ * fk_nonlinear_spline, the adjoint solve of fk_adjoint_field_target, and the
+ * singular Neumann solves of fk_solve_neumann, fk_demag_field and
+ * fk_newton_saddle. FK_PREC_F64 (default, as the
  * reference: FP64 Jacobi-PCG) or FK_PREC_F32_REFINED (FP32 values and PCG
  * recursion, the FP64 residual b - A x of the solve's own system replaced as
  * in fk_solve, stopped on that FP64 residual at the same rtol; for the
