@@ -129,7 +129,7 @@ int fk_pattern(fk_mesh *mesh, int32_t index_base, int32_t *rowptr, int32_t *coli
  * Non-finite values return FK_ERR_NONFINITE. */
 int fk_set_coefficient(fk_mesh *mesh, const double *f);
 /* Assemble A(f) values on the GPU (FP64; FP32 copy made when a mixed solve
- * needs it). method: FK_ASM_GATHER (fastest: 0.99 ms vs 13.6 ms coloured at 24.6M
+ * needs it). method: FK_ASM_GATHER (fastest: 1.0-1.15 ms vs 13.6 ms coloured at 24.6M
  * tets), FK_ASM_COLOURED or FK_ASM_ATOMIC; all three agree to rounding. The
  * first FK_ASM_GATHER call on a mesh builds its incidence lists on the host. */
 int fk_assemble(fk_mesh *mesh, int32_t method);
