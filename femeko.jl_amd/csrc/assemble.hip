@@ -426,6 +426,19 @@ void upload_coefficient(fk_mesh *m, const double *f) {
     }
     m->have_coef = true;
     m->coef_nat_valid = true;
+    m->coef_g_valid = false;
+}
+
+// The row-gather kernels' elements: the caller's order, or the sorted copy
+// when the caller's elements are not sorted by their lowest node.
+static const int4 *gather_tet(const fk_mesh *m) { return m->gather_sorted ? m->tet_g.get() : m->tet_nat.get(); }
+
+// Colour order -> the row-gather kernels' element order (the caller's, or
+// the sorted one).
+static void to_gather_order(fk_mesh *m, const double *src, double *dst) {
+    k_coef_to_caller<<<grid_for(m->nt, 256), 256, 0, m->ctx->stream>>>(
+        src, m->gather_sorted ? m->perm_g.get() : m->perm.get(), m->nt, dst);
+    FK_HIP(hipGetLastError());
 }
 
 // Incidence lists of the row-gather assembly (host build, once per mesh).
@@ -440,7 +453,7 @@ void build_gather(fk_mesh *m) {
     if (nt >= (int64_t(1) << 30)) return;
     std::vector<int4> tet(static_cast<size_t>(nt));
     std::vector<int32_t> rp(static_cast<size_t>(nv + 1)), ci(static_cast<size_t>(nnz));
-    FK_HIP(hipMemcpyAsync(tet.data(), m->tet_nat.get(), nt * sizeof(int4), hipMemcpyDeviceToHost, st));
+    FK_HIP(hipMemcpyAsync(tet.data(), gather_tet(m), nt * sizeof(int4), hipMemcpyDeviceToHost, st));
     FK_HIP(hipMemcpyAsync(rp.data(), m->rowptr.get(), (nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     FK_HIP(hipMemcpyAsync(ci.data(), m->colidx.get(), nnz * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     FK_HIP(hipStreamSynchronize(st));
@@ -528,18 +541,21 @@ bool assemble_newton_gather(fk_mesh *m, const double *mu, const double *dmu, con
     build_gather(m);
     if (!m->gather_ok) return false;
     hipStream_t st = m->ctx->stream;
-    m->coef_nat.ensure(m->nt);
+    // mu and dmu in the kernels' element order (coef_nat or coef_g holds mu
+    // until the loop rewrites coef right after this assembly)
+    DevBuf<double> &mu_g = m->gather_sorted ? m->coef_g : m->coef_nat;
+    mu_g.ensure(m->nt);
     m->dmu_nat.ensure(m->nt);
-    k_coef_to_caller<<<grid_for(m->nt, 256), 256, 0, st>>>(mu, m->perm.get(), m->nt, m->coef_nat.get());
-    if (newton)
-        k_coef_to_caller<<<grid_for(m->nt, 256), 256, 0, st>>>(dmu, m->perm.get(), m->nt, m->dmu_nat.get());
-    m->coef_nat_valid = false;  // the loop rewrites coef right after this assembly
+    to_gather_order(m, mu, mu_g.get());
+    if (newton) to_gather_order(m, dmu, m->dmu_nat.get());
+    m->coef_nat_valid = false;
+    m->coef_g_valid = false;
     k_records_w<<<grid_for(m->nv, 256), 256, 0, st>>>(phi, m->nv, m->xyzw.get());
     const int64_t nb = gather_blocks(m->nv);
     k_assemble_newton_gather<<<static_cast<unsigned>(nb), kGatherRows,
                                static_cast<size_t>(std::max(m->gather_lds, 1)) * sizeof(double), st>>>(
-        m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->tet_nat.get(), m->xyzw.get(),
-        m->coef_nat.get(), m->dmu_nat.get(), newton, m->nv, m->val64.get(), r);
+        m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), gather_tet(m), m->xyzw.get(),
+        mu_g.get(), m->dmu_nat.get(), newton, m->nv, m->val64.get(), r);
     FK_HIP(hipGetLastError());
     return true;
 }
@@ -561,6 +577,17 @@ const double *coef_caller_order(fk_mesh *m, bool may_allocate) {
     return m->coef_nat.get();
 }
 
+// The coefficient in the row-gather kernels' element order.
+static const double *coef_gather_order(fk_mesh *m) {
+    if (!m->gather_sorted) return coef_caller_order(m, true);
+    if (!m->coef_g_valid) {
+        m->coef_g.ensure(m->nt);
+        to_gather_order(m, m->coef.get(), m->coef_g.get());
+        m->coef_g_valid = true;
+    }
+    return m->coef_g.get();
+}
+
 // Enqueue the value assembly on the context stream (no host sync).
 void assemble(fk_mesh *m, int method) {
     if (!m->have_coef) fail(FK_ERR_STATE, "no coefficient set");
@@ -569,11 +596,11 @@ void assemble(fk_mesh *m, int method) {
     if (method == FK_ASM_GATHER) {
         build_gather(m);
         if (m->gather_ok) {
-            const double *coef = coef_caller_order(m, true);
+            const double *coef = coef_gather_order(m);
             const int64_t nb = gather_blocks(m->nv);
             k_assemble_gather<<<static_cast<unsigned>(nb), kGatherRows,
                                 static_cast<size_t>(std::max(m->gather_lds, 1)) * sizeof(double), st>>>(
-                m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), m->tet_nat.get(),
+                m->rowptr.get(), m->inc_sptr.get(), m->inc_width.get(), m->inc.get(), gather_tet(m),
                 m->xyzw.get(), coef, m->nv, m->val64.get());
             FK_HIP(hipGetLastError());
             values_changed(m);

@@ -257,8 +257,17 @@ struct fk_mesh {
     fk::DevBuf<uint2> inc;            // row-gather incidences (element << 2 | local, 4 node positions)
     fk::DevBuf<int4> tet_nat;         // elements in caller order (uploaded with the mesh)
     fk::DevBuf<double> coef_nat;      // coefficient in caller order
-    fk::DevBuf<double> dmu_nat;       // dmu in caller order (gather Newton assembly)
+    fk::DevBuf<double> dmu_nat;       // dmu in the row-gather kernels' element order
     bool coef_nat_valid = false;
+    // The row-gather kernels' element order: the caller's when its elements
+    // are already sorted by their lowest (internal) node, else that sort
+    // (gather_sorted): tet_g, perm_g (colour position -> sorted position)
+    // and coef_g (the coefficient in that order) exist only then.
+    bool gather_sorted = false;
+    fk::DevBuf<int4> tet_g;
+    fk::DevBuf<int32_t> perm_g;
+    fk::DevBuf<double> coef_g;
+    bool coef_g_valid = false;
 
     // internal node numbering (reorder.hip): internal node k is the caller's
     // node node_perm[k], node_inv is the inverse; empty unless reordered

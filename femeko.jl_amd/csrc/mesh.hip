@@ -192,6 +192,37 @@ void build_mesh(fk_mesh *m, const double *p, const int32_t *t, int32_t base, con
         tet[e] = make_int4(nd[0] - base, nd[1] - base, nd[2] - base, nd[3] - base);
     }
     FK_HIP(hipMemcpy(m->tet_nat.get(), tet.data(), nt * sizeof(int4), hipMemcpyHostToDevice));
+    // the row-gather kernels' element order: by lowest node (stable counting
+    // sort), so that the elements of neighbouring rows lie near each other in
+    // memory; kept only when the caller's order is not already sorted so
+    {
+        std::vector<int32_t> key(static_cast<size_t>(nt));
+        bool sorted = true;
+        for (int64_t e = 0; e < nt; ++e) {
+            const int4 q = tet[e];
+            key[e] = std::min(std::min(q.x, q.y), std::min(q.z, q.w));
+            if (e > 0 && key[e] < key[e - 1]) sorted = false;
+        }
+        m->gather_sorted = !sorted;
+        if (!sorted) {
+            std::vector<int64_t> start(static_cast<size_t>(nv) + 1, 0);
+            for (int64_t e = 0; e < nt; ++e) ++start[key[e] + 1];
+            for (int64_t i = 0; i < nv; ++i) start[i + 1] += start[i];
+            std::vector<int32_t> c2s(static_cast<size_t>(nt));  // caller -> sorted position
+            std::vector<int4> tg(static_cast<size_t>(nt));
+            for (int64_t e = 0; e < nt; ++e) {
+                const int64_t s = start[key[e]]++;
+                c2s[e] = static_cast<int32_t>(s);
+                tg[s] = tet[e];
+            }
+            // colour position -> sorted position
+            for (int64_t e = 0; e < nt; ++e) perm[e] = c2s[perm[e]];
+            m->tet_g.alloc(nt);
+            m->perm_g.alloc(nt);
+            FK_HIP(hipMemcpy(m->tet_g.get(), tg.data(), nt * sizeof(int4), hipMemcpyHostToDevice));
+            FK_HIP(hipMemcpy(m->perm_g.get(), perm.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
+        }
+    }
     tet.clear(); tet.shrink_to_fit();
     perm.clear(); perm.shrink_to_fit();
     // ---- element checks

@@ -363,6 +363,7 @@ static void nonlinear_loop_body(fk_mesh *m, const uint8_t *region_host, const La
     k_init_phi_dev(m, 0);  // phi = g on fixed nodes, 0 elsewhere
     m->have_coef = true;
     m->coef_nat_valid = false;  // the loop updates coef (colour order) on the device
+    m->coef_g_valid = false;
     m->have_phi = true;
     const int nb = std::max(1, m->nblocks);
     m->part_misc.ensure(nb);
@@ -544,6 +545,7 @@ static void newton_saddle_body(fk_mesh *m, const uint8_t *region_host, const dou
     FK_HIP(hipGetLastError());
     m->have_coef = true;
     m->coef_nat_valid = false;
+    m->coef_g_valid = false;
     m->have_phi = true;
     DevBuf<double> rhs(nv), y(nv), junk(nv);
     {
