@@ -2,8 +2,10 @@
 (reverse Cuthill-McKee): every entry point that takes or returns per-node
 arrays or a matrix, through the C ABI, against the oracle in the caller's
 numbering. Bars: pattern bit-exact, values to the 1e-12 term metric, solves
-to their tolerance, loads and nodal fields to 1e-12; and the same results,
-within rounding, as the same mesh kept in its own numbering."""
+to their tolerance, loads and nodal fields to 1e-12, the nonlinear loop to
+the bars of its structured-mesh test (its Newton assembly walking the
+library's sorted copy of the elements); and the same results, within
+rounding, as the same mesh kept in its own numbering."""
 import numpy as np
 import pytest
 
@@ -143,5 +145,35 @@ def test_structured_numbering_is_kept():
     try:
         assert not dm.numbering()[0]
         assert dm.solver_format()[0] == "stencil"
+    finally:
+        dm.close()
+
+
+@pytest.mark.parametrize("inner_precision", ["f64", "f32"])
+def test_scattered_mesh_froehlich_loop(oracle, inner_precision):
+    """The nonlinear loop on a scattered mesh: nodes renumbered inside and the
+    row-gather Newton assembly walking its sorted copy of the elements (the
+    caller's elements do not follow the internal numbering); the same bars
+    as on the structured mesh."""
+    pr = W.froehlich(n=8, seed=12)
+    perm = np.random.default_rng(21).permutation(pr.mesh.nv)
+    pr = W.renumbered(pr, perm, "random")
+    Ms, a = pr.meta["Ms"], pr.meta["a"]
+    ref = oracle.newton_froehlich(pr.mesh, pr.mask, pr.g, pr.magnet, Ms, a)
+    dm = DeviceMesh(pr.mesh, context(0))
+    try:
+        assert dm.numbering()[0]
+        dm.set_dirichlet(pr.mask, pr.g)
+        dm.set_inner_precision(inner_precision)
+        out = dm.nonlinear_froehlich(pr.magnet.astype(np.uint8), Ms, a, rtol=1e-10, tol_mu=1e-6, max_outer=20)
+        assert out["outer"] == ref["iters"]
+        phi = dm.potential()
+        assert np.linalg.norm(phi - ref["phi"]) / np.linalg.norm(ref["phi"]) < 1e-8
+        np.testing.assert_allclose(dm.get_coefficient(), ref["mu"], rtol=1e-6)
+        # the plain assembly after the loop reads the loop's mu in the sorted
+        # element order: the oracle's matrix of that mu
+        dm.assemble()
+        A = oracle.stiffnessMatrix(pr.mesh, dm.get_coefficient())
+        assert np.all(np.abs(dm.values() - A.nzval) <= 1e-12 * oracle.term_scale(pr.mesh, dm.get_coefficient()))
     finally:
         dm.close()
